@@ -1,0 +1,150 @@
+/*
+ * viefft.h -- C ABI of the MI355X (gfx950) VIE-FFT operator and Krylov solvers.
+ *
+ * Drop-in boundary for the reference's hot path.  The reference is one C++
+ * translation unit whose entry points are void(void) functions over global
+ * arrays; each entry point below replaces one of them (cited file:line,
+ * relative to the reference root) with explicit arguments:
+ *
+ *   Green_Function_Compute   vie_fft/GreenFunction.h:8-141   -> vie_green_compute
+ *   Incident_Wave            vie_fft/GreenFunction.h:144-248 -> vie_incident
+ *   GetMaterial/GetFlag out  vie_fft/Model.h:305-398         -> vie_set_medium[_eps]
+ *   X_FFT/Y_FFT/Z_FFT +      vie_fft/CG_Iteration.h:131-173, -> vie_apply
+ *     Current_Reset            vie_fft/fft.h:13-35
+ *   *_FFT_Conjugate          vie_fft/CG_Iteration.h:86-128   -> vie_apply(adjoint=1)
+ *   FFT_Program              vie_fft/CG_Iteration.h:1365-1388-> vie_solve
+ *   FarField1                vie_fft/NearFar.h:16-109        -> vie_forward_scatter
+ *   NearField (Ene_Metal)    vie_fft/NearFar.h:180-250       -> vie_absorbed
+ *
+ * Data layout is the reference's: complex numbers are interleaved
+ * {double re, double im} (MKL_Complex16, vie_fft/common.h:68-94), 3-D arrays
+ * are C-order [i][j][k] with k fastest, and a field is the three components
+ * x|y|z stored back to back (3*V complex, V = Nx*Ny*Nz).  Host buffers are
+ * owned by the caller and never freed by the library; device buffers are
+ * owned by the context.  Every call returns a vie_status; no exception
+ * crosses the ABI.  One context per host thread; calls on a context are
+ * synchronous unless named *_async.
+ */
+#ifndef VIEFFT_H
+#define VIEFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vie_ctx vie_ctx;
+
+typedef enum {
+    VIE_OK = 0,
+    VIE_ERR_ARG = 1,          /* bad argument or call order            */
+    VIE_ERR_DEVICE = 2,       /* HIP runtime error / no device         */
+    VIE_ERR_OOM = 3,          /* device allocation failed              */
+    VIE_NOT_CONVERGED = 4,    /* maxit reached before the stop rule    */
+    VIE_BREAKDOWN = 5,        /* zero or non-finite Krylov denominator */
+    VIE_ERR_NAN = 6,          /* non-finite input                      */
+    VIE_ERR_UNSUPPORTED = 7   /* grid size without a compiled kernel   */
+} vie_status;
+
+typedef enum { VIE_FP64 = 0, VIE_FP32 = 1 } vie_precision;
+
+/* Iteration_Type of vie_fft/CG_Iteration.h:47 */
+typedef enum {
+    VIE_CG = 1,        /* CG on the normal equations (CG_FFT)     */
+    VIE_BICG = 2,      /* complex-symmetric BiCG (BICG_FFT)       */
+    VIE_BICGSTAB = 3,  /* BiCGSTAB (BICGSTAB_FFT), the default    */
+    VIE_CORS = 4,      /* CORS (CORS_FFT)                         */
+    VIE_BICORSTAB = 5  /* BiCOR-STAB (BICORSTAB_FFT)              */
+} vie_method;
+
+typedef struct {
+    int n[3];          /* Nx, Ny, Nz: cells per axis (Model.h:163-165)          */
+    double d[3];       /* delta_x, delta_y, delta_z (Model.h:139-160)           */
+    double k0;         /* 2*pi/lambda in the same length unit (Model.h:106)     */
+    int precision;     /* vie_precision of the operator and the plain solves    */
+} vie_desc;
+
+typedef struct {
+    int iterations;        /* loop passes run (the reference prints iterations-1) */
+    int converged;         /* method's own stop rule met                          */
+    double err;            /* last value of the method's own stop quantity        */
+    double true_relres;    /* ||b - A x|| / ||b||, FP64 operator, after the solve */
+    int outer;             /* refinement steps (mixed solve), else 0              */
+    double seconds;        /* device time of the solve                            */
+} vie_info;
+
+/* ---- context ---------------------------------------------------------- */
+int vie_create(vie_ctx** ctx, const vie_desc* desc);
+int vie_destroy(vie_ctx* ctx);
+const char* vie_last_error(const vie_ctx* ctx);
+/* Supported axis lengths are products of 2, 3 and 5 from 2 to 256 (the FFT
+   sub-length is the axis length; the padded length is twice it). */
+int vie_supported_n(int n);
+
+/* ---- medium ------------------------------------------------------------- */
+/* material: V bytes (Material, vie_fft/Model.h:17), eps_table: n_mat complex
+   (Real_EPR/Imag_EPR, vie_fft/Model.h:82-83, engineering sign), flags: 3*V
+   bytes Flag_X|Flag_Y|Flag_Z (vie_fft/Model.h:21-23). */
+int vie_set_medium(vie_ctx* ctx, const unsigned char* material, const double* eps_table,
+                   int n_mat, const unsigned char* flags);
+/* per-voxel permittivity, V complex */
+int vie_set_medium_eps(vie_ctx* ctx, const double* eps_voxel, const unsigned char* flags);
+
+/* ---- set-up ------------------------------------------------------------- */
+/* Green_Function_Compute (vie_fft/GreenFunction.h:8-141): computed on the
+   device; only the octant that the even extension makes unique is kept. */
+int vie_green_compute(vie_ctx* ctx);
+/* Incident_Wave (vie_fft/GreenFunction.h:144-248); angles in degrees;
+   E: host, 3V complex, written unmasked like Ex/Ey/Ez_inc. */
+int vie_incident(vie_ctx* ctx, double theta, double phi, double psi, double* E);
+
+/* ---- operator ----------------------------------------------------------- */
+/* y = A x (adjoint=0) or A^H x (adjoint=1) including the output mask;
+   host buffers, 3V complex each, FP64 interface whatever the precision. */
+int vie_apply(vie_ctx* ctx, const double* x, double* y, int adjoint);
+/* same on device buffers of the context's precision (complex<double> for
+   VIE_FP64, complex<float> for VIE_FP32), enqueued on `stream` (a hipStream_t,
+   NULL = the context's stream) without a host synchronisation. */
+int vie_apply_async(vie_ctx* ctx, const void* x_dev, void* y_dev, int adjoint, void* stream);
+
+/* ---- solve (FFT_Program, vie_fft/CG_Iteration.h:1365-1388) --------------- */
+/* b is masked by the flags first (Current_Reset of E_inc, e.g.
+   vie_fft/CG_Iteration.h:559-562); x starts from 0 like Current_* does
+   (vie_fft/Model.h:189-191).  maxit <= 0 means V (the reference's limit). */
+int vie_solve(vie_ctx* ctx, int method, double tol, int maxit, const double* b, double* x,
+              vie_info* info);
+/* resident variant: b is uploaded once by vie_set_rhs and the solution stays
+   on the device (vie_get_solution copies it out); nothing crosses PCIe inside. */
+int vie_set_rhs(vie_ctx* ctx, const double* b);
+int vie_solve_resident(vie_ctx* ctx, int method, double tol, int maxit, vie_info* info);
+int vie_get_solution(vie_ctx* ctx, double* x);
+/* FP32-complex inner BiCGSTAB wrapped in FP64 iterative refinement: the FP64
+   residual b - A64 x and the correction update stay in FP64.  Needs a
+   context created with VIE_FP64; inner_tol <= 0 picks a default. */
+int vie_solve_mixed_resident(vie_ctx* ctx, double tol, double inner_tol, int maxit,
+                             vie_info* info);
+
+/* ---- post-processing ----------------------------------------------------- */
+/* Forward_RCS of FarField1 (vie_fft/NearFar.h:16-109) of the resident
+   solution; divide by the geometric cross-section for Q_ext. */
+int vie_forward_scatter(vie_ctx* ctx, double theta, double phi, double psi, double* value);
+/* Ene_Metal of NearField (vie_fft/NearFar.h:180-250) for material id `mat`
+   (needs vie_set_medium); times Z0 / area = Q_abs. */
+int vie_absorbed(vie_ctx* ctx, int mat, double* value);
+
+/* ---- instrumentation ----------------------------------------------------- */
+/* Per-launch HIP-event timing of one kernel class while solving:
+   0 off, 1 z-forward, 2 y-forward, 3 x spectral pass, 4 y-inverse,
+   5 z-inverse of the FP64 operator; 11-15 the same passes of the FP32
+   operator.  vie_kernel_time returns launches and mean milliseconds
+   (launches enqueued after convergence, which exit at once, are left out). */
+int vie_time_kernel(vie_ctx* ctx, int kernel_class);
+int vie_kernel_time(vie_ctx* ctx, int* launches, double* mean_ms);
+/* bytes the algorithmic model charges one operator application */
+double vie_model_bytes_matvec(const vie_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1368 @@
+// vie_api.hip -- C ABI of the gfx950 VIE-FFT path (declared in include/viefft.h).
+//
+// Host side of the operator and of the five Krylov loops.  Reference
+// correspondence (paths relative to the reference root):
+//   Green_Function_Compute  vie_fft/GreenFunction.h:8-141   green_compute()
+//   Incident_Wave           vie_fft/GreenFunction.h:144-248 vie_incident()
+//   X/Y/Z_FFT (+Conjugate)  vie_fft/CG_Iteration.h:86-173   Op<R>::apply()
+//   CG_FFT                  vie_fft/CG_Iteration.h:176-357  solve_cg()
+//   BICG_FFT                vie_fft/CG_Iteration.h:360-528  solve_bicg()
+//   BICGSTAB_FFT            vie_fft/CG_Iteration.h:531-776  solve_bicgstab()
+//   CORS_FFT                vie_fft/CG_Iteration.h:780-1073 solve_cors()
+//   BICORSTAB_FFT           vie_fft/CG_Iteration.h:1076-1362 solve_bicorstab()
+//   FFT_Program             vie_fft/CG_Iteration.h:1365-1388 vie_solve*()
+//   FarField1, NearField    vie_fft/NearFar.h:16-109, :180-250
+#include "../../include/viefft.h"
+#include "vie_krylov.h"
+#include "vie_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace vie;
+
+namespace {
+
+const double kPi = 3.1415926535897932384626433832795;
+const double kZ0 = 120.0 * kPi;  // vie_fft/common.h:18
+const double kEps = 1.0e-7;      // vie_fft/common.h:16
+
+struct Err : std::runtime_error {
+    int code;
+    Err(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            throw Err(e_ == hipErrorOutOfMemory ? VIE_ERR_OOM : VIE_ERR_DEVICE,               \
+                      std::string(#x) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+template <class T> T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw Err(VIE_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    return (T*)p;
+}
+template <class T> void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+unsigned grid1(long n, int nt = NT, long cap = 4096)
+{
+    long g = (n + nt - 1) / nt;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+template <class R> cx<R> hc(std::complex<double> z)
+{
+    cx<R> v;
+    v.x = (R)z.real();
+    v.y = (R)z.imag();
+    return v;
+}
+
+// Kernel-class timing: HIP events around every launch of one pass.
+struct Timer {
+    int cls = 0;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    void reset()
+    {
+        used = 0;
+    }
+    void mark(hipStream_t st)
+    {
+        if (!cls) return;
+        if (used >= ev.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(ev[used++], st));
+    }
+    ~Timer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// ------------------------------------------------------------------ operator
+template <class R> struct Op {
+    using T = cx<R>;
+    Geo g{};
+    int n[3];
+    double d[3], k0;
+    T* tw[3] = {};
+    T* twist[3] = {};
+    T* U[3] = {};
+    T* Vv[3] = {};
+    T* Ua[3] = {};
+    T* Va[3] = {};
+    int* fold[3] = {};
+    T* G = nullptr;
+    T* dg = nullptr;
+    uint8_t* flags = nullptr;
+    T* W1 = nullptr;
+    T* W2 = nullptr;
+    bool own_w = true;
+    MixTabs<R> mf{}, ma{};
+
+    Op(const int* nn, const double* dd, double kk, T* w1, T* w2)
+    {
+        for (int a = 0; a < 3; ++a) {
+            n[a] = nn[a];
+            d[a] = dd[a];
+        }
+        k0 = kk;
+        g.nx = n[0];
+        g.ny = n[1];
+        g.nz = n[2];
+        g.V = (long)n[0] * n[1] * n[2];
+        const double NP = 8.0 * (double)g.V;
+        for (int a = 0; a < 3; ++a) {
+            const int N = n[a];
+            std::vector<T> h_tw(N), h_twist(N), h_u(2 * N), h_v(2 * N), h_ua(2 * N), h_va(2 * N);
+            std::vector<int> h_fold(2 * N);
+            for (int t = 0; t < N; ++t) {
+                const long double a1 = -2.0L * (long double)kPi * t / N;
+                const long double a2 = -2.0L * (long double)kPi * t / (2 * N);
+                h_tw[t] = hc<R>(std::complex<double>((double)cosl(a1), (double)sinl(a1)));
+                h_twist[t] = hc<R>(std::complex<double>((double)cosl(a2), (double)sinl(a2)));
+            }
+            const std::complex<double> J(0, 1);
+            for (int t = 0; t < 2 * N; ++t) {
+                const int k = 2 * (t % N) + t / N;
+                const long double th = (long double)kPi * k / N;
+                const double s = (double)sinl(th), h = (double)sinl(th / 2);
+                // e^{j th} - 1 = -2 sin^2(th/2) + j sin th ; 1 - e^{-j th} = 2 sin^2(th/2) + j sin th
+                const std::complex<double> u(-2.0 * h * h, s), w(2.0 * h * h, s);
+                const std::complex<double> vv = J * kZ0 / (k0 * d[a] * NP) * w;
+                h_u[t] = hc<R>(u);
+                h_v[t] = hc<R>(vv);
+                h_ua[t] = hc<R>(std::conj(vv));
+                h_va[t] = hc<R>(std::conj(u));
+                const int kf = std::min(k, 2 * N - k);
+                h_fold[t] = (kf % 2 == 0) ? kf / 2 : (N / 2 + 1) + (kf - 1) / 2;
+            }
+            tw[a] = dalloc<T>(N);
+            twist[a] = dalloc<T>(N);
+            U[a] = dalloc<T>(2 * N);
+            Vv[a] = dalloc<T>(2 * N);
+            Ua[a] = dalloc<T>(2 * N);
+            Va[a] = dalloc<T>(2 * N);
+            fold[a] = dalloc<int>(2 * N);
+            HIPCHK(hipMemcpy(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice));
+        }
+        const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+        G = dalloc<T>(gsz);
+        dg = dalloc<T>(3 * g.V);
+        flags = dalloc<uint8_t>(3 * g.V);
+        if (w1) {
+            W1 = w1;
+            W2 = w2;
+            own_w = false;
+        } else {
+            W1 = dalloc<T>(3 * 2 * g.V);
+            W2 = dalloc<T>(3 * 4 * g.V);
+        }
+        for (int a = 0; a < 3; ++a) {
+            mf.U[a] = U[a];
+            mf.Vv[a] = Vv[a];
+            mf.fold[a] = fold[a];
+            ma.U[a] = Ua[a];
+            ma.Vv[a] = Va[a];
+            ma.fold[a] = fold[a];
+            const std::complex<double> D = std::complex<double>(0, 1) * k0 * kZ0 * d[a] / NP;
+            mf.D[a][0] = (R)D.real();
+            mf.D[a][1] = (R)D.imag();
+            ma.D[a][0] = (R)D.real();
+            ma.D[a][1] = (R)-D.imag();
+        }
+        mf.G = ma.G = G;
+        mf.gconj = 0;
+        ma.gconj = 1;
+    }
+    ~Op()
+    {
+        for (int a = 0; a < 3; ++a) {
+            dfree(tw[a]);
+            dfree(twist[a]);
+            dfree(U[a]);
+            dfree(Vv[a]);
+            dfree(Ua[a]);
+            dfree(Va[a]);
+            dfree(fold[a]);
+        }
+        dfree(G);
+        dfree(dg);
+        dfree(flags);
+        if (own_w) {
+            dfree(W1);
+            dfree(W2);
+        }
+    }
+
+    // y = A z (or A^H z) where z is the Lin prologue; src is the vector the
+    // diagonal term multiplies (z itself).  Returns the partial-sum count.
+    long apply(const Lin<R>& lin, const T* src, T* y, Dots<R> dots, bool adj, const int* done,
+               hipStream_t st, Timer* tm)
+    {
+        long nblk = 0;
+        const int off = sizeof(R) == 8 ? 0 : 10;  // timing classes 1-5 FP64, 11-15 FP32
+        auto t0 = [&](int c) { if (tm && tm->cls == c + off) tm->mark(st); };
+        t0(1);
+        HIPCHK(run_zfwd<R>(g, lin, W1, tw[2], twist[2], done, st));
+        t0(1);
+        t0(2);
+        HIPCHK(run_yfwd<R>(g, W1, W2, tw[1], twist[1], done, st));
+        t0(2);
+        t0(3);
+        HIPCHK(run_xmix<R>(g, W2, tw[0], twist[0], adj ? ma : mf, done, st));
+        t0(3);
+        t0(4);
+        HIPCHK(run_yinv<R>(g, W2, W1, tw[1], twist[1], done, st));
+        t0(4);
+        t0(5);
+        HIPCHK(run_zinv<R>(g, W1, src, y, dg, adj ? 1 : 0, flags, dots, tw[2], twist[2], done, st, &nblk));
+        t0(5);
+        return nblk;
+    }
+};
+
+template <class R> Lin<R> lin1(const cx<R>* x)
+{
+    Lin<R> l{};
+    l.x[0] = x;
+    return l;
+}
+
+template <class R> Dots<R> nodots()
+{
+    Dots<R> d{};
+    return d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------- ctx
+struct vie_ctx {
+    vie_desc desc{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    long V = 0;
+    Op<double>* op64 = nullptr;
+    Op<float>* op32 = nullptr;
+    double2* Goff = nullptr;       // FP64 octant of G~ (parity order)
+    bool green_ready = false;
+    bool medium_ready = false;
+    std::vector<double> eps_h;     // per-voxel eps, 2V doubles
+    std::vector<uint8_t> mat_h;    // material ids (if given)
+    std::vector<uint8_t> flags_h;  // 3V
+    double2* eps_dev = nullptr;
+    uint8_t* mat_dev = nullptr;
+    // solver storage
+    KState* state = nullptr;
+    KState* aux = nullptr;
+    double* hist = nullptr;
+    double2* partial = nullptr;
+    long partial_cap = 0;
+    double2* b64 = nullptr;        // resident, masked right-hand side
+    double2* x64 = nullptr;        // resident solution
+    bool rhs_ready = false;
+    bool sol_ready = false;
+    std::vector<void*> vec64, vec32;
+    double2* t64 = nullptr;        // FP64 scratch (residual / product)
+    double2* r64 = nullptr;
+    Timer timer;
+    int timed_launches = 0;
+    double timed_ms = 0.0;
+    int ahead = 4;                 // iterations enqueued per host check
+};
+
+namespace {
+
+int fail(vie_ctx* c, const Err& e)
+{
+    if (c) c->err = e.what();
+    return e.code;
+}
+
+template <class R> Op<R>* op_of(vie_ctx* c);
+template <> Op<double>* op_of<double>(vie_ctx* c) { return c->op64; }
+template <> Op<float>* op_of<float>(vie_ctx* c) { return c->op32; }
+
+template <class R> std::vector<void*>& vecs_of(vie_ctx* c);
+template <> std::vector<void*>& vecs_of<double>(vie_ctx* c) { return c->vec64; }
+template <> std::vector<void*>& vecs_of<float>(vie_ctx* c) { return c->vec32; }
+
+// k-th solver work vector (3V entries) of precision R, allocated on demand
+template <class R> cx<R>* vec(vie_ctx* c, int k)
+{
+    auto& v = vecs_of<R>(c);
+    while ((int)v.size() <= k) v.push_back(nullptr);
+    if (!v[k]) v[k] = dalloc<cx<R>>(3 * c->V);
+    return (cx<R>*)v[k];
+}
+
+void ensure_partial(vie_ctx* c, long need)
+{
+    if (need <= c->partial_cap) return;
+    dfree(c->partial);
+    c->partial = dalloc<double2>(need);
+    c->partial_cap = need;
+}
+
+template <class R> void fill0(vie_ctx* c, cx<R>* x, long n)
+{
+    hipLaunchKernelGGL(k_fill_zero<R>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, x);
+    HIPCHK(hipGetLastError());
+}
+
+template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
+{
+    const unsigned gsz = grid1(L.n, NT, 1024);
+    L.dots.partial = c->partial;
+    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz), dim3(NT), 0, c->stream, L, done);
+    HIPCHK(hipGetLastError());
+    return gsz;
+}
+
+void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)
+{
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(NT), 0, c->stream, op, s, c->partial, nblk, nd, c->hist);
+    HIPCHK(hipGetLastError());
+}
+
+void state_init(vie_ctx* c, KState* s, double tol, int maxit)
+{
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit);
+    HIPCHK(hipGetLastError());
+}
+
+KState read_state(vie_ctx* c, KState* s)
+{
+    KState h;
+    HIPCHK(hipMemcpyAsync(&h, s, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+template <class R> Dots<R> dots_with(vie_ctx* c, int n, std::initializer_list<int> ab, const cx<R>* e1,
+                                     const cx<R>* e2, std::initializer_list<int> nc = {})
+{
+    Dots<R> d{};
+    d.n = n;
+    int i = 0;
+    for (int v : ab) {
+        if (i % 2 == 0) d.a[i / 2] = v; else d.b[i / 2] = v;
+        ++i;
+    }
+    i = 0;
+    for (int v : nc) d.nc[i++] = v;
+    d.e1 = e1;
+    d.e2 = e2;
+    d.partial = c->partial;
+    return d;
+}
+
+template <class R> Op<R>* make_op(vie_ctx* c);
+
+void green_compute(vie_ctx* c)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const long L0 = n[0] + 1, L1 = n[1] + 1, L2 = n[2] + 1, tot = L0 * L1 * L2;
+    // ping-pong buffers: offsets -> x cosine sums -> y -> z
+    double2* buf[2] = {nullptr, nullptr};
+    double* dct = nullptr;
+    try {
+        buf[0] = dalloc<double2>(tot);
+        buf[1] = dalloc<double2>(tot);
+        hipLaunchKernelGGL(k_green_offsets, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, n[0],
+                           n[1], n[2], d[0], d[1], d[2], c->desc.k0, buf[0]);
+        HIPCHK(hipGetLastError());
+        const int nmax = std::max(n[0], std::max(n[1], n[2]));
+        dct = dalloc<double>(2 * nmax);
+        int cur = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            const int N = n[ax];
+            std::vector<double> ctab(2 * N);
+            for (int j = 0; j < 2 * N; ++j) ctab[j] = (double)cosl((long double)kPi * j / N);
+            HIPCHK(hipMemcpyAsync(dct, ctab.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_green_cos, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, (int)L0,
+                               (int)L1, (int)L2, ax, dct, buf[cur], buf[1 - cur]);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            cur = 1 - cur;
+        }
+        dfree(dct);
+        dfree(c->Goff);
+        c->Goff = buf[cur];
+        buf[cur] = nullptr;
+        dfree(buf[1 - cur]);
+    } catch (...) {
+        dfree(dct);
+        dfree(buf[0]);
+        dfree(buf[1]);
+        throw;
+    }
+    c->green_ready = true;
+    if (c->op64) HIPCHK(hipMemcpyAsync(c->op64->G, c->Goff, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    if (c->op32) {
+        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                           c->Goff, c->op32->G);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// d_f (vie_fft/fft.h:405-588) from per-voxel eps, on the host.
+std::vector<std::complex<double>> diag_host(vie_ctx* c)
+{
+    const int* n = c->desc.n;
+    const long V = c->V;
+    const long st[3] = {(long)n[1] * n[2], n[2], 1};
+    std::vector<std::complex<double>> dg(3 * V);
+    const std::complex<double>* eps = (const std::complex<double>*)c->eps_h.data();
+    for (int f = 0; f < 3; ++f) {
+        const std::complex<double> cf = std::complex<double>(0, -1) * (c->desc.d[f] / 2.0) * kZ0 / c->desc.k0;
+        for (long v = 0; v < V; ++v) {
+            const int idx[3] = {(int)(v / st[0]), (int)((v / n[2]) % n[1]), (int)(v % n[2])};
+            std::complex<double> val = 0;
+            if (idx[f] < n[f] - 1) val = cf * (1.0 / (eps[v + st[f]] - 1.0) + 1.0 / (eps[v] - 1.0));
+            dg[f * V + v] = val;
+        }
+    }
+    return dg;
+}
+
+template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
+{
+    auto dg = diag_host(c);
+    std::vector<cx<R>> h(dg.size());
+    for (size_t i = 0; i < dg.size(); ++i) h[i] = hc<R>(dg[i]);
+    HIPCHK(hipMemcpy(op->dg, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice));
+}
+
+template <> Op<double>* make_op<double>(vie_ctx* c)
+{
+    if (!c->op64) {
+        double2* w1 = c->op32 ? (double2*)c->op32->W1 : nullptr;
+        double2* w2 = c->op32 ? (double2*)c->op32->W2 : nullptr;
+        // an FP32 operator's work buffers are too small to share; allocate fresh
+        (void)w1;
+        (void)w2;
+        c->op64 = new Op<double>(c->desc.n, c->desc.d, c->desc.k0, nullptr, nullptr);
+        if (c->green_ready)
+            HIPCHK(hipMemcpy(c->op64->G, c->Goff, (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1) *
+                                                      sizeof(double2), hipMemcpyDeviceToDevice));
+        if (c->medium_ready) upload_medium<double>(c, c->op64);
+    }
+    return c->op64;
+}
+
+template <> Op<float>* make_op<float>(vie_ctx* c)
+{
+    if (!c->op32) {
+        // the FP32 pass buffers fit in the FP64 ones: share them
+        float2* w1 = c->op64 ? (float2*)c->op64->W1 : nullptr;
+        float2* w2 = c->op64 ? (float2*)c->op64->W2 : nullptr;
+        c->op32 = new Op<float>(c->desc.n, c->desc.d, c->desc.k0, w1, w2);
+        const long tot = (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1);
+        if (c->green_ready) {
+            hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                               c->Goff, c->op32->G);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        if (c->medium_ready) upload_medium<float>(c, c->op32);
+    }
+    return c->op32;
+}
+
+void require(vie_ctx* c, bool ok, const char* what)
+{
+    if (!ok) throw Err(VIE_ERR_ARG, what);
+    (void)c;
+}
+
+// ---------------------------------------------------------------- solvers
+// Vectors: index constants into the per-precision work-vector pool.
+enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, NVEC };
+
+template <class R> struct SolveOut {
+    KState s;
+};
+
+template <class R> using T_ = cx<R>;
+
+// Drives iterations until the device flags done; `iter` enqueues one.
+template <class F> KState drive(vie_ctx* c, KState* st, F iter)
+{
+    for (;;) {
+        for (int k = 0; k < c->ahead; ++k) iter();
+        KState h = read_state(c, st);
+        if (h.done) return h;
+    }
+}
+
+// BiCGSTAB (vie_fft/CG_Iteration.h:531-776) on b -> x, both of precision R.
+template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    KState* st = c->state;
+    const long n = 3 * c->V;
+    cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
+    const int* done = &st->done;
+    state_init(c, st, tol, maxit);
+    fill0<R>(c, p, n);
+    fill0<R>(c, Ap, n);
+    {   // x = 0 ; r = b ; b^H b
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {1, 1}, b, nullptr);
+        long nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_INIT, nb, 1);
+    }
+    auto iter = [&]() {
+        Lin<R> lp{};
+        lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
+        lp.coef = &st->cf[CF_BS_P];
+        lp.out = p;
+        long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
+        Lin<R> lq{};
+        lq.x[0] = r; lq.x[1] = Ap;
+        lq.coef = &st->cf[CF_BS_Q];
+        lq.out = q;
+        nb = op->apply(lq, q, Aq, dots_with<R>(c, 2, {0, 1, 0, 0}, q, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_OMEGA, nb, 2);
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = x; L.x[1] = p; L.x[2] = q; L.x[3] = Aq;
+        L.coef = &st->cf[CF_BS_U];
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 2, {3, 3, 1, 3}, b, nullptr);
+        nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_END, nb, 2);
+    };
+    return drive(c, st, iter);
+}
+
+// CG on the normal equations (vie_fft/CG_Iteration.h:176-357).
+template <class R> KState solve_cg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    KState* st = c->state;
+    const long n = 3 * c->V;
+    cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP);
+    const int* done = &st->done;
+    state_init(c, st, tol, maxit);
+    {   // x = 0 ; r = -b
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = -1.0;
+        L.out = x;
+        L.out2 = r;
+        lincomb<R>(c, L, done);
+        // p = A^H b ; ||p||^2, ||b||^2
+        long nb = op->apply(lin1<R>(b), b, p, dots_with<R>(c, 2, {0, 0, 1, 1}, b, nullptr), true, done, c->stream,
+                            &c->timer);
+        scalar(c, st, SC_CG_INIT, nb, 2);
+    }
+    auto iter = [&]() {
+        long nb = op->apply(lin1<R>(p), p, Ap, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), false, done, c->stream,
+                            &c->timer);
+        scalar(c, st, SC_CG_ALPHA, nb, 1);
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = x; L.x[1] = p; L.x[2] = r; L.x[3] = Ap;
+        L.coef = &st->cf[CF_CG_U];
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {3, 3}, nullptr, nullptr);
+        nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_CG_V4, nb, 1);
+        nb = op->apply(lin1<R>(r), r, q, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), true, done, c->stream,
+                       &c->timer);
+        scalar(c, st, SC_CG_BETA, nb, 1);
+        LinComb<R> P{};
+        P.n = n;
+        P.x[0] = p; P.x[1] = q;
+        P.coef = &st->cf[CF_CG_P];
+        P.out = p;
+        lincomb<R>(c, P, done);
+    };
+    return drive(c, st, iter);
+}
+
+// Complex-symmetric BiCG (vie_fft/CG_Iteration.h:360-528).
+template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    KState* st = c->state;
+    const long n = 3 * c->V;
+    cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *Ap = vec<R>(c, VAP);
+    const int* done = &st->done;
+    state_init(c, st, tol, maxit);
+    {
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {3, 3}, nullptr, nullptr, {1});
+        long nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICG_INIT, nb, 1);
+        LinComb<R> S{};  // p = r / (r^T r)
+        S.n = n;
+        S.x[0] = r;
+        S.coef = &st->cf[CF_BI_S];
+        S.out = p;
+        lincomb<R>(c, S, done);
+    }
+    auto iter = [&]() {
+        // Ap ; Ap^T p (unconjugated)
+        long nb = op->apply(lin1<R>(p), p, Ap, dots_with<R>(c, 1, {0, 1}, p, nullptr, {1}), false, done, c->stream,
+                            &c->timer);
+        scalar(c, st, SC_BICG_ALPHA, nb, 1);
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = x; L.x[1] = p; L.x[2] = r; L.x[3] = Ap;
+        L.coef = &st->cf[CF_BI_U];
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {3, 3}, nullptr, nullptr, {1});
+        nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICG_END, nb, 1);
+        LinComb<R> P{};
+        P.n = n;
+        P.x[0] = p; P.x[1] = r;
+        P.coef = &st->cf[CF_BI_P];
+        P.out = p;
+        lincomb<R>(c, P, done);
+    };
+    return drive(c, st, iter);
+}
+
+// CORS (vie_fft/CG_Iteration.h:780-1073).
+template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    KState* st = c->state;
+    const long n = 3 * c->V;
+    cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
+    cx<R>*rc = vec<R>(c, VRC), *qc = vec<R>(c, VQC), *pc = vec<R>(c, VPC);
+    const int* done = &st->done;
+    state_init(c, st, tol, maxit);
+    fill0<R>(c, q, n);
+    fill0<R>(c, rc, n);
+    fill0<R>(c, pc, n);
+    {
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {1, 1}, b, nullptr);
+        long nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BNRM, nb, 1);
+    }
+    auto iter = [&]() {
+        long nb = op->apply(lin1<R>(r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+                            &c->timer);
+        scalar(c, st, SC_CORS_RHO, nb, 1);
+        LinComb<R> L1{};  // p = r + b pc ; qc = Ap + b rc
+        L1.n = n;
+        L1.x[0] = r; L1.x[1] = pc; L1.x[2] = Ap; L1.x[3] = rc;
+        L1.coef = &st->cf[CF_CO_L1];
+        L1.out = p;
+        L1.out2 = qc;
+        lincomb<R>(c, L1, done);
+        Lin<R> lq{};  // q = qc + b rc + b^2 q, then Aq
+        lq.x[0] = qc; lq.x[1] = rc; lq.x[2] = q;
+        lq.coef = &st->cf[CF_CO_Q];
+        lq.out = q;
+        nb = op->apply(lq, q, Aq, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_CORS_ALPHA, nb, 1);
+        LinComb<R> L3{};  // pc = p - a q ; rc = qc - a Aq
+        L3.n = n;
+        L3.x[0] = p; L3.x[1] = q; L3.x[2] = qc; L3.x[3] = Aq;
+        L3.coef = &st->cf[CF_CO_L3];
+        L3.out = pc;
+        L3.out2 = rc;
+        lincomb<R>(c, L3, done);
+        LinComb<R> LX{};  // x += 2a p - a^2 q
+        LX.n = n;
+        LX.x[0] = x; LX.x[1] = p; LX.x[2] = q;
+        LX.coef = &st->cf[CF_CO_X];
+        LX.out = x;
+        lincomb<R>(c, LX, done);
+        LinComb<R> LR{};  // r += -2a qc + a^2 Aq ; ||r||^2
+        LR.n = n;
+        LR.x[0] = r; LR.x[1] = qc; LR.x[2] = Aq;
+        LR.coef = &st->cf[CF_CO_R];
+        LR.out = r;
+        LR.dots = dots_with<R>(c, 1, {0, 0}, nullptr, nullptr);
+        nb = lincomb<R>(c, LR, done);
+        scalar(c, st, SC_RES_END, nb, 1);
+    };
+    return drive(c, st, iter);
+}
+
+// BiCOR-STAB (vie_fft/CG_Iteration.h:1076-1362).
+template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    KState* st = c->state;
+    const long n = 3 * c->V;
+    cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
+    cx<R>*qc = vec<R>(c, VQC), *pc = vec<R>(c, VPC);
+    const int* done = &st->done;
+    state_init(c, st, tol, maxit);
+    fill0<R>(c, p, n);
+    fill0<R>(c, q, n);
+    fill0<R>(c, Aq, n);
+    {
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {1, 1}, b, nullptr);
+        long nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BNRM, nb, 1);
+    }
+    auto iter = [&]() {
+        long nb = op->apply(lin1<R>(r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+                            &c->timer);
+        scalar(c, st, SC_BICOR_RHO, nb, 1);
+        LinComb<R> L1{};  // p = r + b p - b w q
+        L1.n = n;
+        L1.x[0] = r; L1.x[1] = p; L1.x[2] = q;
+        L1.coef = &st->cf[CF_BC_P];
+        L1.out = p;
+        lincomb<R>(c, L1, done);
+        Lin<R> lq{};  // q = Ap + b q - b w Aq, then Aq
+        lq.x[0] = Ap; lq.x[1] = q; lq.x[2] = Aq;
+        lq.coef = &st->cf[CF_BC_Q];
+        lq.out = q;
+        nb = op->apply(lq, q, Aq, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICOR_ALPHA, nb, 1);
+        LinComb<R> L2{};  // pc = r - a q ; qc = Ap - a Aq ; qc^H pc, qc^H qc
+        L2.n = n;
+        L2.x[0] = r; L2.x[1] = q; L2.x[2] = Ap; L2.x[3] = Aq;
+        L2.coef = &st->cf[CF_BC_L2];
+        L2.out = pc;
+        L2.out2 = qc;
+        L2.dots = dots_with<R>(c, 2, {3, 0, 3, 3}, nullptr, nullptr);
+        nb = lincomb<R>(c, L2, done);
+        scalar(c, st, SC_BICOR_OMEGA, nb, 2);
+        LinComb<R> LX{};  // x += a p + w pc
+        LX.n = n;
+        LX.x[0] = x; LX.x[1] = p; LX.x[2] = pc;
+        LX.coef = &st->cf[CF_BC_X];
+        LX.out = x;
+        lincomb<R>(c, LX, done);
+        LinComb<R> LR{};  // r = pc - w qc ; ||r||^2
+        LR.n = n;
+        LR.x[0] = pc; LR.x[1] = qc;
+        LR.coef = &st->cf[CF_BC_R];
+        LR.out = r;
+        LR.dots = dots_with<R>(c, 1, {0, 0}, nullptr, nullptr);
+        nb = lincomb<R>(c, LR, done);
+        scalar(c, st, SC_RES_END, nb, 1);
+    };
+    return drive(c, st, iter);
+}
+
+template <class R> KState solve_method(vie_ctx* c, int method, const cx<R>* b, cx<R>* x, double tol, int maxit)
+{
+    Op<R>* op = make_op<R>(c);
+    switch (method) {
+    case VIE_CG: return solve_cg<R>(c, op, b, x, tol, maxit);
+    case VIE_BICG: return solve_bicg<R>(c, op, b, x, tol, maxit);
+    case VIE_BICGSTAB: return solve_bicgstab<R>(c, op, b, x, tol, maxit);
+    case VIE_CORS: return solve_cors<R>(c, op, b, x, tol, maxit);
+    case VIE_BICORSTAB: return solve_bicorstab<R>(c, op, b, x, tol, maxit);
+    default: throw Err(VIE_ERR_ARG, "unknown method");
+    }
+}
+
+// ||b - A64 x|| / ||b|| with the FP64 operator, on resident b64 / x64.
+double true_relres(vie_ctx* c)
+{
+    Op<double>* op = make_op<double>(c);
+    const long n = 3 * c->V;
+    KState* s = c->aux;
+    state_init(c, s, 0.0, 1);
+    LinComb<double> L{};
+    L.n = n;
+    L.x[0] = c->b64;
+    L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+    long nb = lincomb<double>(c, L, nullptr);
+    scalar(c, s, SC_BNRM, nb, 1);
+    op->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, nullptr);
+    LinComb<double> M{};
+    M.n = n;
+    M.x[0] = c->b64;
+    M.x[1] = c->t64;
+    M.hc[0][0] = 1.0;
+    M.hc[1][0] = -1.0;
+    M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+    nb = lincomb<double>(c, M, nullptr);
+    scalar(c, s, SC_RELRES, nb, 1);
+    return read_state(c, s).err;
+}
+
+int status_of(const KState& h)
+{
+    if (h.status == KS_BREAKDOWN) return VIE_BREAKDOWN;
+    if (h.status == KS_NAN) return VIE_ERR_NAN;
+    return h.converged ? VIE_OK : VIE_NOT_CONVERGED;
+}
+
+void check_ready(vie_ctx* c)
+{
+    require(c, c->green_ready, "vie_green_compute has not been called");
+    require(c, c->medium_ready, "vie_set_medium has not been called");
+}
+
+void timing_begin(vie_ctx* c) { c->timer.reset(); }
+void timing_end(vie_ctx* c)
+{
+    if (!c->timer.cls) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // launches enqueued after convergence exit at once; drop them (far below
+    // the median) from the mean
+    std::vector<float> t;
+    for (size_t i = 0; i + 1 < c->timer.used; i += 2) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->timer.ev[i], c->timer.ev[i + 1]));
+        t.push_back(ms);
+    }
+    if (t.empty()) return;
+    std::vector<float> srt = t;
+    std::nth_element(srt.begin(), srt.begin() + srt.size() / 2, srt.end());
+    const float med = srt[srt.size() / 2];
+    for (float v : t)
+        if (v >= 0.25f * med) {
+            c->timed_ms += v;
+            c->timed_launches += 1;
+        }
+}
+
+}  // namespace
+
+namespace {
+template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjoint, hipStream_t st)
+{
+    Op<R>* op = make_op<R>(c);
+    op->apply(lin1<R>(x), x, y, nodots<R>(), adjoint != 0, nullptr, st, nullptr);
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int vie_supported_n(int n) { return size_supported(n) ? 1 : 0; }
+
+int vie_create(vie_ctx** out, const vie_desc* desc)
+{
+    if (!out || !desc) return VIE_ERR_ARG;
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (!size_supported(desc->n[a])) return VIE_ERR_UNSUPPORTED;
+        if (!(desc->d[a] > 0) || !std::isfinite(desc->d[a])) return VIE_ERR_ARG;
+    }
+    if (!(desc->k0 > 0) || !std::isfinite(desc->k0)) return VIE_ERR_ARG;
+    if (desc->precision != VIE_FP64 && desc->precision != VIE_FP32) return VIE_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VIE_ERR_DEVICE;
+    vie_ctx* c = new vie_ctx();
+    c->desc = *desc;
+    c->V = (long)desc->n[0] * desc->n[1] * desc->n[2];
+    try {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->state = dalloc<KState>(1);
+        c->aux = dalloc<KState>(1);
+        c->hist = dalloc<double>(MAX_HIST);
+        ensure_partial(c, 3 * std::max<long>(std::max(zinv_blocks<double>(Geo{desc->n[0], desc->n[1], desc->n[2], c->V}),
+                                                      zinv_blocks<float>(Geo{desc->n[0], desc->n[1], desc->n[2], c->V})),
+                                             4096));
+        c->b64 = dalloc<double2>(3 * c->V);
+        c->x64 = dalloc<double2>(3 * c->V);
+        c->t64 = dalloc<double2>(3 * c->V);
+        c->r64 = dalloc<double2>(3 * c->V);
+        if (desc->precision == VIE_FP64) make_op<double>(c); else make_op<float>(c);
+    } catch (const Err& e) {
+        int code = e.code;
+        vie_destroy(c);
+        return code;
+    }
+    *out = c;
+    return VIE_OK;
+}
+
+int vie_destroy(vie_ctx* c)
+{
+    if (!c) return VIE_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->op32;   // shares buffers owned by op64: delete first
+    delete c->op64;
+    dfree(c->Goff);
+    dfree(c->eps_dev);
+    dfree(c->mat_dev);
+    dfree(c->state);
+    dfree(c->aux);
+    dfree(c->hist);
+    dfree(c->partial);
+    dfree(c->b64);
+    dfree(c->x64);
+    dfree(c->t64);
+    dfree(c->r64);
+    for (void* p : c->vec64) if (p) (void)hipFree(p);
+    for (void* p : c->vec32) if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VIE_OK;
+}
+
+const char* vie_last_error(const vie_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static int set_medium_common(vie_ctx* c, const unsigned char* flags)
+{
+    c->flags_h.assign(flags, flags + 3 * c->V);
+    for (long i = 0; i < 2 * c->V; ++i)
+        if (!std::isfinite(c->eps_h[i])) throw Err(VIE_ERR_NAN, "non-finite permittivity");
+    // 1/(eps - 1) must be finite on every face the operator uses
+    for (long v = 0; v < c->V; ++v)
+        if (c->eps_h[2 * v] == 1.0 && c->eps_h[2 * v + 1] == 0.0) {
+            // eps = 1 only matters where a face term reads it
+            const int* n = c->desc.n;
+            const long st[3] = {(long)n[1] * n[2], n[2], 1};
+            const int idx[3] = {(int)(v / st[0]), (int)((v / n[2]) % n[1]), (int)(v % n[2])};
+            for (int f = 0; f < 3; ++f) {
+                if (idx[f] < n[f] - 1 && c->flags_h[f * c->V + v])
+                    throw Err(VIE_ERR_ARG, "eps = 1 on an active face (1/(eps-1) is infinite)");
+                if (idx[f] > 0 && c->flags_h[f * c->V + v - st[f]])
+                    throw Err(VIE_ERR_ARG, "eps = 1 on an active face (1/(eps-1) is infinite)");
+            }
+        }
+    dfree(c->eps_dev);
+    c->eps_dev = dalloc<double2>(c->V);
+    HIPCHK(hipMemcpy(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice));
+    c->medium_ready = true;
+    if (c->op64) upload_medium<double>(c, c->op64);
+    if (c->op32) upload_medium<float>(c, c->op32);
+    return VIE_OK;
+}
+
+int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_table, int n_mat,
+                   const unsigned char* flags)
+{
+    if (!c || !material || !eps_table || !flags || n_mat < 1) return VIE_ERR_ARG;
+    try {
+        c->eps_h.resize(2 * c->V);
+        for (long v = 0; v < c->V; ++v) {
+            const int m = material[v];
+            if (m >= n_mat) throw Err(VIE_ERR_ARG, "material id outside the eps table");
+            c->eps_h[2 * v] = eps_table[2 * m];
+            c->eps_h[2 * v + 1] = eps_table[2 * m + 1];
+        }
+        c->mat_h.assign(material, material + c->V);
+        dfree(c->mat_dev);
+        c->mat_dev = dalloc<uint8_t>(c->V);
+        HIPCHK(hipMemcpy(c->mat_dev, material, c->V, hipMemcpyHostToDevice));
+        return set_medium_common(c, flags);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags)
+{
+    if (!c || !eps || !flags) return VIE_ERR_ARG;
+    try {
+        c->eps_h.assign(eps, eps + 2 * c->V);
+        c->mat_h.clear();
+        dfree(c->mat_dev);
+        return set_medium_common(c, flags);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_green_compute(vie_ctx* c)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        green_compute(c);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+// Incident_Wave (vie_fft/GreenFunction.h:144-248): exp(-j k.r) on each
+// component's face centre times the cell integral along its own axis.
+int vie_incident(vie_ctx* c, double theta, double phi, double psi, double* E)
+{
+    if (!c || !E) return VIE_ERR_ARG;
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const double k0 = c->desc.k0;
+    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+    const double kv[3] = {k0 * std::sin(th) * std::cos(ph), k0 * std::sin(th) * std::sin(ph), k0 * std::cos(th)};
+    const double pol[3] = {std::cos(th) * std::cos(ph) * std::cos(ps) - std::sin(ph) * std::sin(ps),
+                           std::cos(th) * std::sin(ph) * std::cos(ps) + std::cos(ph) * std::sin(ps),
+                           -std::sin(th) * std::cos(ps)};
+    std::complex<double>* out = (std::complex<double>*)E;
+    for (int cpt = 0; cpt < 3; ++cpt) {
+        std::complex<double> seg;
+        if (std::fabs(kv[cpt]) < kEps) {
+            seg = d[cpt];
+        } else {
+            // integral of exp(-j k s) over a cell of length d centred on 0
+            seg = 2.0 * std::sin(kv[cpt] * d[cpt] / 2.0) / kv[cpt];
+        }
+        seg *= pol[cpt];
+        for (int i = 0; i < n[0]; ++i)
+            for (int j = 0; j < n[1]; ++j)
+                for (int k = 0; k < n[2]; ++k) {
+                    double pos[3] = {(i + 0.5) * d[0], (j + 0.5) * d[1], (k + 0.5) * d[2]};
+                    pos[cpt] += 0.5 * d[cpt];
+                    const double a = kv[0] * pos[0] + kv[1] * pos[1] + kv[2] * pos[2];
+                    out[cpt * c->V + ((long)i * n[1] + j) * n[2] + k] = std::polar(1.0, -a) * seg;
+                }
+    }
+    return VIE_OK;
+}
+
+int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* stream)
+{
+    if (!c || !x || !y) return VIE_ERR_ARG;
+    try {
+        check_ready(c);
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        if (c->desc.precision == VIE_FP64)
+            apply_dev<double>(c, (const double2*)x, (double2*)y, adjoint, st);
+        else
+            apply_dev<float>(c, (const float2*)x, (float2*)y, adjoint, st);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
+{
+    if (!c || !x || !y) return VIE_ERR_ARG;
+    try {
+        check_ready(c);
+        const long n = 3 * c->V;
+        HIPCHK(hipMemcpy(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice));
+        if (c->desc.precision == VIE_FP64) {
+            apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
+        } else {
+            float2* xs = vec<float>(c, VT);
+            float2* ys = vec<float>(c, VT + 1);
+            hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->t64, xs);
+            apply_dev<float>(c, xs, ys, adjoint, c->stream);
+            hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost));
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_rhs(vie_ctx* c, const double* b)
+{
+    if (!c || !b) return VIE_ERR_ARG;
+    try {
+        require(c, c->medium_ready, "vie_set_medium has not been called");
+        const long n = 3 * c->V;
+        for (long i = 0; i < 2 * n; ++i)
+            if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
+        HIPCHK(hipMemcpy(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice));
+        Op<double>* op = make_op<double>(c);
+        // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
+        hipLaunchKernelGGL(k_mask<double>, dim3(grid1(n, NT, 1L << 30)), dim3(NT), 0, c->stream, n, c->t64, op->flags,
+                           c->b64);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->rhs_ready = true;
+        c->sol_ready = false;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_solve_resident(vie_ctx* c, int method, double tol, int maxit, vie_info* info)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        check_ready(c);
+        require(c, c->rhs_ready, "vie_set_rhs has not been called");
+        require(c, tol > 0, "tol must be positive");
+        if (maxit <= 0) maxit = (int)std::min<long>(c->V, 1L << 30);
+        const long n = 3 * c->V;
+        timing_begin(c);
+        auto t0 = std::chrono::steady_clock::now();
+        KState h;
+        if (c->desc.precision == VIE_FP64) {
+            h = solve_method<double>(c, method, c->b64, c->x64, tol, maxit);
+        } else {
+            float2* b32 = vec<float>(c, VB);
+            float2* x32 = vec<float>(c, VX);
+            hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->b64, b32);
+            h = solve_method<float>(c, method, b32, x32, tol, maxit);
+            hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, x32, c->x64);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        timing_end(c);
+        c->sol_ready = true;
+        if (info) {
+            info->iterations = h.iter;
+            info->converged = h.converged;
+            info->err = h.err;
+            info->outer = 0;
+            info->seconds = std::chrono::duration<double>(t1 - t0).count();
+            info->true_relres = true_relres(c);
+        }
+        return status_of(h);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_solve_mixed_resident(vie_ctx* c, double tol, double inner_tol, int maxit, vie_info* info)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        check_ready(c);
+        require(c, c->rhs_ready, "vie_set_rhs has not been called");
+        require(c, c->desc.precision == VIE_FP64, "mixed solve needs an FP64 context");
+        require(c, tol > 0, "tol must be positive");
+        if (maxit <= 0) maxit = (int)std::min<long>(c->V, 1L << 30);
+        if (inner_tol <= 0) inner_tol = 1e-4;
+        const long n = 3 * c->V;
+        Op<double>* op64 = make_op<double>(c);
+        make_op<float>(c);
+        float2* b32 = vec<float>(c, VB);
+        float2* e32 = vec<float>(c, VX);
+        KState* s = c->aux;
+        timing_begin(c);
+        auto t0 = std::chrono::steady_clock::now();
+        // bnrm2 and x = 0
+        state_init(c, s, tol, 1);
+        LinComb<double> L{};
+        L.n = n;
+        L.x[0] = c->b64;
+        L.out2 = c->x64;  // x = 0 * b
+        L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+        long nb = lincomb<double>(c, L, nullptr);
+        scalar(c, s, SC_BNRM, nb, 1);
+        int total = 0, outer = 0, status = VIE_OK;
+        double err = 1.0, rnorm = 0.0;
+        KState last{};
+        for (;;) {
+            // r64 = b - A64 x64 (FP64 residual), ||r64||
+            if (outer == 0) {
+                HIPCHK(hipMemcpyAsync(c->r64, c->b64, n * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+                KState h = read_state(c, s);
+                if (h.status) throw Err(h.status == KS_NAN ? VIE_ERR_NAN : VIE_BREAKDOWN, "zero or non-finite rhs");
+                rnorm = std::sqrt(h.bnrm2);
+                err = 1.0;
+            } else {
+                op64->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream,
+                            &c->timer);
+                LinComb<double> M{};
+                M.n = n;
+                M.x[0] = c->b64;
+                M.x[1] = c->t64;
+                M.hc[0][0] = 1.0;
+                M.hc[1][0] = -1.0;
+                M.out = c->r64;
+                M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+                nb = lincomb<double>(c, M, nullptr);
+                scalar(c, s, SC_RELRES, nb, 1);
+                KState h = read_state(c, s);
+                err = h.err;
+                rnorm = std::sqrt(h.sums[0].x);
+            }
+            if (!std::isfinite(err)) { status = VIE_ERR_NAN; break; }
+            if (err < tol) break;
+            if (total >= maxit) { status = VIE_NOT_CONVERGED; break; }
+            // inner FP32 solve of A e = r / ||r||
+            const double sc = 1.0 / rnorm;
+            hipLaunchKernelGGL(k_scale_from64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, sc, c->r64, b32);
+            HIPCHK(hipGetLastError());
+            const double itol = std::max(inner_tol, 0.5 * tol / err);
+            last = solve_method<float>(c, VIE_BICGSTAB, b32, e32, std::min(itol, 0.5), maxit - total);
+            total += last.iter;
+            ++outer;
+            if (last.status == KS_NAN) { status = VIE_ERR_NAN; break; }
+            hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, rnorm, e32, c->x64);
+            HIPCHK(hipGetLastError());
+            if (last.status == KS_BREAKDOWN && last.iter == 0) { status = VIE_BREAKDOWN; break; }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        timing_end(c);
+        c->sol_ready = true;
+        if (info) {
+            info->iterations = total;
+            info->converged = status == VIE_OK;
+            info->err = err;
+            info->outer = outer;
+            info->seconds = std::chrono::duration<double>(t1 - t0).count();
+            info->true_relres = err;
+        }
+        return status;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_get_solution(vie_ctx* c, double* x)
+{
+    if (!c || !x) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        HIPCHK(hipMemcpy(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost));
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_solve(vie_ctx* c, int method, double tol, int maxit, const double* b, double* x, vie_info* info)
+{
+    if (!c || !b || !x) return VIE_ERR_ARG;
+    int rc = vie_set_rhs(c, b);
+    if (rc != VIE_OK) return rc;
+    rc = vie_solve_resident(c, method, tol, maxit, info);
+    if (rc != VIE_OK && rc != VIE_NOT_CONVERGED && rc != VIE_BREAKDOWN) return rc;
+    int rc2 = vie_get_solution(c, x);
+    return rc2 != VIE_OK ? rc2 : rc;
+}
+
+int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double* value)
+{
+    if (!c || !value) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        const int* n = c->desc.n;
+        const double* d = c->desc.d;
+        const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+        const double vd[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+        const unsigned gsz = grid1(c->V, NT, 1024);
+        hipLaunchKernelGGL(k_farsum, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], c->desc.k0,
+                           vd[0], vd[1], vd[2], c->x64, c->partial);
+        HIPCHK(hipGetLastError());
+        std::vector<double2> h(3 * gsz);
+        HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::complex<double> S[3];
+        const double dV = d[0] * d[1] * d[2];
+        for (int cpt = 0; cpt < 3; ++cpt) {
+            std::complex<double> acc = 0;
+            for (unsigned b = 0; b < gsz; ++b) acc += std::complex<double>(h[cpt * gsz + b].x, h[cpt * gsz + b].y);
+            S[cpt] = std::polar(1.0, c->desc.k0 * 0.5 * d[cpt] * vd[cpt]) * acc * dV;
+        }
+        const double tm = kZ0 * (S[0].real() * std::cos(th) * std::cos(ph) + S[1].real() * std::cos(th) * std::sin(ph) -
+                                 S[2].real() * std::sin(th));
+        const double te = kZ0 * (S[0].real() * std::sin(ph) - S[1].real() * std::cos(ph));
+        *value = std::cos(ps) * tm - std::sin(ps) * te;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_absorbed(vie_ctx* c, int mat, double* value)
+{
+    if (!c || !value) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, c->mat_dev != nullptr, "vie_absorbed needs material ids (vie_set_medium)");
+        const int* n = c->desc.n;
+        const double* d = c->desc.d;
+        const unsigned gsz = grid1(c->V, NT, 1024);
+        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0] * d[1] * d[2],
+                           c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat, c->partial);
+        HIPCHK(hipGetLastError());
+        std::vector<double2> h(gsz);
+        HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double acc = 0;
+        for (unsigned b = 0; b < gsz; ++b) acc += h[b].x;
+        *value = acc;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_time_kernel(vie_ctx* c, int kernel_class)
+{
+    if (!c || kernel_class < 0 || kernel_class > 15 || (kernel_class > 5 && kernel_class < 11)) return VIE_ERR_ARG;
+    c->timer.cls = kernel_class;
+    c->timed_launches = 0;
+    c->timed_ms = 0;
+    return VIE_OK;
+}
+
+int vie_kernel_time(vie_ctx* c, int* launches, double* mean_ms)
+{
+    if (!c) return VIE_ERR_ARG;
+    if (launches) *launches = c->timed_launches;
+    if (mean_ms) *mean_ms = c->timed_launches ? c->timed_ms / c->timed_launches : 0.0;
+    return VIE_OK;
+}
+
+double vie_model_bytes_matvec(const vie_ctx* c)
+{
+    if (!c) return 0.0;
+    const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
+    const double V = (double)c->V;
+    const double oct = (double)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1);
+    return 81.0 * V * s + oct * s + 4.0 * V;
+}
+
+}  // extern "C"

@@ -1,0 +1,541 @@
+// vie_krylov.h -- vector kernels and device-resident scalar updates of the
+// Krylov loops of vie_fft/CG_Iteration.h (paths relative to the reference).
+//
+// The reference calls CBLAS level-1 routines three times per logical vector
+// operation (one per component) and keeps the scalars on the host.  Here a
+// vector operation covers all 3V entries in one kernel, dot products are
+// fused into the kernel that produces their operand (per-workgroup partial
+// sums, summed in a fixed order by a one-workgroup scalar kernel, so results
+// are deterministic), and scalars live in device memory (KState), so an
+// iteration needs no host synchronisation.  Every kernel exits at once when
+// KState::done is set, which lets the host enqueue iterations ahead.
+#pragma once
+
+#include "vie_passes.h"
+
+namespace vie {
+
+constexpr int MAX_HIST = 1 << 16;
+
+// Solver scalars.  cf[] holds the coefficients that Lin prologues (3 entries)
+// and LinComb kernels (8 entries: 4 for out, 4 for out2) read at fixed
+// per-solver offsets.
+struct KState {
+    double2 rho, rho1, alpha, omega, beta;
+    double2 cf[48];
+    double2 sums[4];
+    double bnrm2, err, tol;
+    int iter, maxit, done, converged, status;
+    int pad;
+};
+
+enum { KS_OK = 0, KS_BREAKDOWN = 5, KS_NAN = 6 };
+
+// Coefficient offsets per solver (see the host drivers).
+enum {
+    CF_BS_P = 0, CF_BS_Q = 3, CF_BS_U = 8,                                        // BiCGSTAB
+    CF_CG_U = 8, CF_CG_P = 16,                                                    // CG(NR)
+    CF_BI_S = 0, CF_BI_U = 8, CF_BI_P = 16,                                       // BiCG
+    CF_CO_L1 = 0, CF_CO_Q = 8, CF_CO_L3 = 12, CF_CO_X = 20, CF_CO_R = 28,         // CORS
+    CF_BC_P = 0, CF_BC_Q = 8, CF_BC_L2 = 12, CF_BC_X = 20, CF_BC_R = 28           // BiCORSTAB
+};
+
+// ------------------------------------------------------------ lincomb
+// out = sum_i c_i x_i and optionally out2 = sum_i c2_i x_i over n entries
+// (c from device coef[0..3], c2 from coef[4..7], or from hc/hc2), with up to
+// three fused dot products (operands 0 = out, 1 = e1, 2 = e2, 3 = out2).
+template <class R> struct LinComb {
+    long n;
+    const cx<R>* x[4];
+    const double2* coef;
+    double hc[4][2];
+    double hc2[4][2];
+    cx<R>* out;
+    cx<R>* out2;
+    Dots<R> dots;
+};
+
+template <class R>
+__global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
+{
+    using T = cx<R>;
+    __shared__ double2 red[NT / 64];
+    if (done && *done) return;
+    T c[4], c2[4];
+    for (int q = 0; q < 4; ++q) {
+        if (L.coef) {
+            c[q] = mkc((R)L.coef[q].x, (R)L.coef[q].y);
+            c2[q] = mkc((R)L.coef[4 + q].x, (R)L.coef[4 + q].y);
+        } else {
+            c[q] = mkc((R)L.hc[q][0], (R)L.hc[q][1]);
+            c2[q] = mkc((R)L.hc2[q][0], (R)L.hc2[q][1]);
+        }
+    }
+    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    const long stride = (long)gridDim.x * NT;
+    for (long i = (long)blockIdx.x * NT + threadIdx.x; i < L.n; i += stride) {
+        T v = czero<T>(), v2 = czero<T>();
+        for (int q = 0; q < 4; ++q)
+            if (L.x[q]) {
+                const T xv = L.x[q][i];
+                v = cadd(v, cmul(c[q], xv));
+                if (L.out2) v2 = cadd(v2, cmul(c2[q], xv));
+            }
+        for (int d = 0; d < L.dots.n; ++d) {
+            const int sa = L.dots.a[d], sb = L.dots.b[d];
+            const T a = sa == 0 ? v : sa == 1 ? L.dots.e1[i] : sa == 2 ? L.dots.e2[i] : v2;
+            const T b = sb == 0 ? v : sb == 1 ? L.dots.e1[i] : sb == 2 ? L.dots.e2[i] : v2;
+            dot_acc(acc[d], a, b, L.dots.nc[d]);
+        }
+        if (L.out) L.out[i] = v;
+        if (L.out2) L.out2[i] = v2;
+    }
+    for (int d = 0; d < L.dots.n; ++d) {
+        double2 s = block_sum(acc[d], red);
+        if (threadIdx.x == 0) L.dots.partial[d * (long)gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------ scalar helpers
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b)
+{
+    // Smith's algorithm for a / b
+    if (fabs(b.x) >= fabs(b.y)) {
+        double r = b.y / b.x, den = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+    }
+    double r = b.x / b.y, den = b.x * r + b.y;
+    return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+__device__ __forceinline__ double2 zneg(double2 a) { return make_double2(-a.x, -a.y); }
+__device__ __forceinline__ double2 zsc(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ bool zbad(double2 a)
+{
+    return !(isfinite(a.x) && isfinite(a.y)) || (a.x == 0.0 && a.y == 0.0);
+}
+__device__ __forceinline__ double2 zc(double re, double im) { return make_double2(re, im); }
+
+// Sum nd partial arrays (each nblk long) in a fixed order into out[0..nd).
+__device__ __forceinline__ void reduce_partials(const double2* part, long nblk, int nd, double2* out,
+                                                double2* red)
+{
+    for (int d = 0; d < nd; ++d) {
+        double2 a = make_double2(0, 0);
+        for (long i = threadIdx.x; i < nblk; i += NT) {
+            a.x += part[d * nblk + i].x;
+            a.y += part[d * nblk + i].y;
+        }
+        double2 s = block_sum(a, red);
+        if (threadIdx.x == 0) out[d] = s;
+        __syncthreads();
+    }
+}
+
+// Scalar programs, one per solver step; v[] are the reduced dot products.
+enum ScalarOp {
+    SC_BNRM = 1,          // v0 = b^H b -> bnrm2
+    SC_BICGSTAB_INIT,     // v0 = b^H b -> bnrm2, rho, p = r
+    SC_BICGSTAB_ALPHA,    // v0 = b^H Ap
+    SC_BICGSTAB_OMEGA,    // v0 = Aq^H q, v1 = Aq^H Aq
+    SC_BICGSTAB_END,      // v0 = r^H r, v1 = b^H r
+    SC_RELRES,            // v0 = z^H z -> err = sqrt(re / bnrm2), no iteration count
+    SC_CG_INIT,           // v0 = p^H p (p = A^H b), v1 = b^H b
+    SC_CG_ALPHA,          // v0 = Ap^H Ap
+    SC_CG_V4,             // v0 = r^H r
+    SC_CG_BETA,           // v0 = q^H q
+    SC_BICG_INIT,         // v0 = r^T r
+    SC_BICG_ALPHA,        // v0 = Ap^T p
+    SC_BICG_END,          // v0 = r^T r
+    SC_CORS_RHO,          // v0 = b^H Ap
+    SC_CORS_ALPHA,        // v0 = b^H Aq
+    SC_RES_END,           // v0 = r^H r: err, iteration end, rho1 = rho
+    SC_BICOR_RHO,         // v0 = b^H Ap
+    SC_BICOR_ALPHA,       // v0 = b^H Aq
+    SC_BICOR_OMEGA        // v0 = qc^H pc, v1 = qc^H qc
+};
+
+__device__ __forceinline__ void ks_fail(KState* s, int status)
+{
+    s->status = status;
+    s->done = 1;
+}
+
+// End-of-iteration bookkeeping common to all methods.
+__device__ __forceinline__ void ks_finish(KState* s, double err, double* hist)
+{
+    s->err = err;
+    if (hist && s->iter < MAX_HIST) hist[s->iter] = err;
+    s->iter += 1;
+    if (!isfinite(err)) { ks_fail(s, KS_NAN); return; }
+    if (err < s->tol) { s->converged = 1; s->done = 1; return; }
+    if (s->iter >= s->maxit) s->done = 1;
+}
+
+__device__ __forceinline__ void set3(double2* c, double2 a, double2 b, double2 d)
+{
+    c[0] = a; c[1] = b; c[2] = d;
+}
+__device__ __forceinline__ void set8(double2* c, double2 a0, double2 a1, double2 a2, double2 a3,
+                                     double2 b0, double2 b1, double2 b2, double2 b3)
+{
+    c[0] = a0; c[1] = a1; c[2] = a2; c[3] = a3;
+    c[4] = b0; c[5] = b1; c[6] = b2; c[7] = b3;
+}
+
+__global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2* part, long nblk,
+                                                int nd, double* hist)
+{
+    __shared__ double2 v[4];
+    __shared__ double2 red[NT / 64];
+    if (s->done) return;
+    reduce_partials(part, nblk, nd, v, red);
+    if (threadIdx.x != 0) return;
+    for (int d = 0; d < nd; ++d) s->sums[d] = v[d];
+    const double2 one = zc(1, 0), O = zc(0, 0);
+    double2* cf = s->cf;
+    switch (op) {
+    case SC_BNRM:
+        s->bnrm2 = v[0].x;
+        s->iter = 0; s->converged = 0; s->status = 0;
+        s->omega = one; s->alpha = O; s->rho1 = O; s->beta = O;
+        if (!(v[0].x > 0.0) || !isfinite(v[0].x)) ks_fail(s, v[0].x == 0.0 ? KS_BREAKDOWN : KS_NAN);
+        break;
+    // ---- BiCGSTAB (vie_fft/CG_Iteration.h:564-748)
+    case SC_BICGSTAB_INIT:
+        s->bnrm2 = v[0].x;
+        s->rho = v[0];  // b^H r with r = b
+        s->omega = one; s->alpha = O;
+        s->iter = 0; s->converged = 0; s->status = 0;
+        set3(cf + CF_BS_P, one, O, O);  // p = r
+        if (!(v[0].x > 0.0) || !isfinite(v[0].x)) ks_fail(s, v[0].x == 0.0 ? KS_BREAKDOWN : KS_NAN);
+        break;
+    case SC_BICGSTAB_ALPHA:
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->alpha = zdiv(s->rho, v[0]);
+        set3(cf + CF_BS_Q, one, zneg(s->alpha), O);  // q = r - alpha Ap
+        break;
+    case SC_BICGSTAB_OMEGA:
+        if (zbad(v[1])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->omega = zdiv(v[0], v[1]);
+        // over {x, p, q, Aq}: x = x + alpha p + omega q ; r = q - omega Aq
+        set8(cf + CF_BS_U, one, s->alpha, s->omega, O, O, O, one, zneg(s->omega));
+        break;
+    case SC_BICGSTAB_END: {
+        ks_finish(s, sqrt(v[0].x / s->bnrm2), hist);
+        if (s->done) break;
+        s->rho1 = s->rho;
+        s->rho = v[1];
+        if (zbad(s->rho1) || zbad(s->omega)) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->beta = zmul(zdiv(s->rho, s->rho1), zdiv(s->alpha, s->omega));
+        // over {r, p, Ap}: p = r + beta p - beta omega Ap
+        set3(cf + CF_BS_P, one, s->beta, zneg(zmul(s->beta, s->omega)));
+        break;
+    }
+    case SC_RELRES:
+        s->err = sqrt(v[0].x / s->bnrm2);
+        break;
+    // ---- CG on the normal equations (vie_fft/CG_Iteration.h:204-332)
+    case SC_CG_INIT:
+        s->sums[2] = v[0];     // v3 = ||A^H b||^2
+        s->bnrm2 = v[1].x;     // vx = ||b||^2
+        s->iter = 0; s->converged = 0; s->status = 0;
+        if (!(v[1].x > 0.0) || !isfinite(v[1].x)) ks_fail(s, v[1].x == 0.0 ? KS_BREAKDOWN : KS_NAN);
+        break;
+    case SC_CG_ALPHA: {
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        const double a = s->sums[2].x / v[0].x;  // alpha = v1 / v2, v1 = v3
+        s->alpha = zc(a, 0);
+        s->sums[3] = s->sums[2];
+        // over {x, p, r, Ap}: x = x + a p ; r = r + a Ap
+        set8(cf + CF_CG_U, one, zc(a, 0), O, O, O, O, one, zc(a, 0));
+        break;
+    }
+    case SC_CG_V4:
+        s->sums[1] = v[0];     // v4 = ||r||^2
+        break;
+    case SC_CG_BETA: {
+        const double v4 = s->sums[1].x;
+        const double beta = v[0].x / s->sums[3].x;  // v3 / v1
+        s->sums[2] = v[0];
+        set8(cf + CF_CG_P, zc(beta, 0), zc(-1, 0), O, O, O, O, O, O);  // p = beta p - q
+        ks_finish(s, v4 / s->bnrm2, hist);  // squared ratio (vie_fft/CG_Iteration.h:323)
+        break;
+    }
+    // ---- complex-symmetric BiCG (vie_fft/CG_Iteration.h:397-506)
+    case SC_BICG_INIT:
+        s->sums[2] = v[0];  // vx = r^T r
+        s->iter = 0; s->converged = 0; s->status = 0;
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        set8(cf + CF_BI_S, zdiv(one, v[0]), O, O, O, O, O, O, O);  // p = r / v1
+        break;
+    case SC_BICG_ALPHA:
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->alpha = zdiv(one, v[0]);
+        // over {x, p, r, Ap}: x = x + a p ; r = r - a Ap
+        set8(cf + CF_BI_U, one, s->alpha, O, O, O, O, one, zneg(s->alpha));
+        break;
+    case SC_BICG_END: {
+        const double2 vx = s->sums[2];
+        const double err = sqrt((v[0].x * v[0].x + v[0].y * v[0].y) / (vx.x * vx.x + vx.y * vx.y));
+        ks_finish(s, err, hist);
+        if (s->done) break;
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->beta = zdiv(one, v[0]);
+        set8(cf + CF_BI_P, one, s->beta, O, O, O, O, O, O);  // p = p + beta r
+        break;
+    }
+    // ---- CORS (vie_fft/CG_Iteration.h:848-1036)
+    case SC_CORS_RHO: {
+        s->rho = v[0];
+        if (s->iter > 0) {
+            if (zbad(s->rho1)) { ks_fail(s, KS_BREAKDOWN); break; }
+            s->beta = zdiv(s->rho, s->rho1);
+        } else {
+            s->beta = O;
+        }
+        const double2 b = s->beta;
+        // over {r, pc, Ap, rc}: p = r + b pc ; qc = Ap + b rc
+        set8(cf + CF_CO_L1, one, b, O, O, O, O, one, b);
+        // over {qc, rc, q}: q = qc + b rc + b^2 q
+        set3(cf + CF_CO_Q, one, b, zmul(b, b));
+        break;
+    }
+    case SC_CORS_ALPHA: {
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->alpha = zdiv(s->rho, v[0]);
+        const double2 a = s->alpha, a2 = zmul(a, a);
+        // over {p, q, qc, Aq}: pc = p - a q ; rc = qc - a Aq
+        set8(cf + CF_CO_L3, one, zneg(a), O, O, O, O, one, zneg(a));
+        // over {x, p, q}: x = x + 2a p - a^2 q
+        set8(cf + CF_CO_X, one, zsc(a, 2.0), zneg(a2), O, O, O, O, O);
+        // over {r, qc, Aq}: r = r - 2a qc + a^2 Aq
+        set8(cf + CF_CO_R, one, zsc(a, -2.0), a2, O, O, O, O, O);
+        break;
+    }
+    case SC_RES_END:
+        ks_finish(s, sqrt(v[0].x / s->bnrm2), hist);
+        s->rho1 = s->rho;
+        break;
+    // ---- BiCOR-STAB (vie_fft/CG_Iteration.h:1140-1328)
+    case SC_BICOR_RHO: {
+        s->rho = v[0];
+        if (s->iter > 0) {
+            if (zbad(s->rho1) || zbad(s->omega)) { ks_fail(s, KS_BREAKDOWN); break; }
+            s->beta = zmul(zdiv(s->rho, s->rho1), zdiv(s->alpha, s->omega));
+        } else {
+            s->beta = O;
+        }
+        const double2 b = s->beta, bw = zneg(zmul(s->beta, s->omega));
+        // over {r, p, q}: p = r + b p - b w q
+        set8(cf + CF_BC_P, one, b, bw, O, O, O, O, O);
+        // over {Ap, q, Aq}: q = Ap + b q - b w Aq
+        set3(cf + CF_BC_Q, one, b, bw);
+        break;
+    }
+    case SC_BICOR_ALPHA:
+        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->alpha = zdiv(s->rho, v[0]);
+        // over {r, q, Ap, Aq}: pc = r - a q ; qc = Ap - a Aq
+        set8(cf + CF_BC_L2, one, zneg(s->alpha), O, O, O, O, one, zneg(s->alpha));
+        break;
+    case SC_BICOR_OMEGA:
+        if (zbad(v[1])) { ks_fail(s, KS_BREAKDOWN); break; }
+        s->omega = zdiv(v[0], v[1]);
+        // over {x, p, pc}: x = x + a p + w pc
+        set8(cf + CF_BC_X, one, s->alpha, s->omega, O, O, O, O, O);
+        // over {pc, qc}: r = pc - w qc
+        set8(cf + CF_BC_R, one, zneg(s->omega), O, O, O, O, O, O);
+        break;
+    default:
+        break;
+    }
+}
+
+// Graph-safe fill (used instead of hipMemsetAsync).
+template <class R>
+__global__ void k_fill_zero(long n, cx<R>* x)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        x[t] = czero<cx<R>>();
+}
+
+// x64 += s * e (e in the inner precision): the correction update of the
+// FP64 iterative refinement.
+template <class R>
+__global__ void k_axpy_to64(long n, double s, const cx<R>* e, double2* x)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        x[t].x += s * (double)e[t].x;
+        x[t].y += s * (double)e[t].y;
+    }
+}
+
+// out = (R)(s * in): scaled conversion of an FP64 residual to the inner precision.
+template <class R>
+__global__ void k_scale_from64(long n, double s, const double2* in, cx<R>* out)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[t] = mkc((R)(s * in[t].x), (R)(s * in[t].y));
+}
+
+// KState initialisation from the host-side tolerance and limit.
+__global__ void k_state_init(KState* s, double tol, int maxit)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        KState z = {};
+        z.tol = tol;
+        z.maxit = maxit;
+        *s = z;
+    }
+}
+
+// ------------------------------------------------------------ Green's function
+// g on the (Nx+1)(Ny+1)(Nz+1) offsets (vie_fft/GreenFunction.h:53-88).
+__global__ void k_green_offsets(int n0, int n1, int n2, double d0, double d1, double d2,
+                                double k0, double2* out)
+{
+    const long total = (long)(n0 + 1) * (n1 + 1) * (n2 + 1);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int k = t % (n2 + 1);
+    const int j = (t / (n2 + 1)) % (n1 + 1);
+    const int i = t / ((long)(n1 + 1) * (n2 + 1));
+    const double pi = 3.1415926535897932384626433832795;
+    const double dV = d0 * d1 * d2;
+    const double D = 2.0 * cbrt(dV / (4.0 / 3.0 * pi));
+    const double p = 0.5 * k0 * D;
+    double2 v;
+    if (i + j + k != 0) {
+        const double x = i * d0, y = j * d1, z = k * d2;
+        const double R = sqrt(x * x + y * y + z * z);
+        const double sh = sin(p) / p - cos(p);
+        const double a = sh / (pi / 3.0 * k0 * k0 * D * D * R) * dV;
+        double sr, cr;
+        sincos(k0 * R, &sr, &cr);
+        v = make_double2(cr * a, -sr * a);
+    } else {
+        double sp, cp;
+        sincos(p, &sp, &cp);
+        // (1 + j p)(cos p - j sin p) - 1
+        const double re = cp + p * sp - 1.0, im = p * cp - sp;
+        const double den = pi / 6.0 * k0 * k0 * D * D * D;
+        v = make_double2(re / den * dV, im / den * dV);
+    }
+    out[t] = v;
+}
+
+// Cosine transform along one axis of an (L0, L1, L2) array, L = N + 1 on that
+// axis: out[p] = sum_n w_n cos(pi k(p) n / N) in[n], w_0 = w_N = 1, else 2,
+// where k(p) is the frequency stored at parity position p (evens then odds).
+// Applied once per axis this is the forward DFT of the even extension of g
+// to 2N points, restricted to the octant 0 <= k <= N that evenness makes
+// unique (vie_fft/GreenFunction.h:91-139).  ctab[j] = cos(pi j / N), j < 2N.
+__global__ void k_green_cos(int L0, int L1, int L2, int axis, const double* ctab,
+                            const double2* in, double2* out)
+{
+    const long total = (long)L0 * L1 * L2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int c2 = t % L2, c1 = (t / L2) % L1, c0 = t / ((long)L1 * L2);
+    const int L = axis == 0 ? L0 : axis == 1 ? L1 : L2;
+    const int N = L - 1;
+    const int pos = axis == 0 ? c0 : axis == 1 ? c1 : c2;
+    const int neven = N / 2 + 1;
+    const int kf = pos < neven ? 2 * pos : 2 * (pos - neven) + 1;
+    const long st = axis == 0 ? (long)L1 * L2 : axis == 1 ? L2 : 1;
+    const long base = t - (long)pos * st;
+    double2 acc = make_double2(0, 0);
+    for (int n = 0; n <= N; ++n) {
+        const double w = (n == 0 || n == N) ? 1.0 : 2.0;
+        const double c = w * ctab[((long)kf * n) % (2 * N)];
+        const double2 a = in[base + (long)n * st];
+        acc.x += c * a.x;
+        acc.y += c * a.y;
+    }
+    out[t] = acc;
+}
+
+template <class R>
+__global__ void k_convert(long n, const double2* in, cx<R>* out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = mkc((R)in[t].x, (R)in[t].y);
+}
+
+template <class R>
+__global__ void k_to_f64(long n, const cx<R>* in, double2* out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = make_double2((double)in[t].x, (double)in[t].y);
+}
+
+// Mask a field by the flags: y = flag ? x : 0 (Current_Reset, vie_fft/fft.h:13-35)
+template <class R>
+__global__ void k_mask(long n, const cx<R>* x, const uint8_t* flags, cx<R>* y)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = flags[t] ? x[t] : czero<cx<R>>();
+}
+
+// Far-field sums of FarField1 (vie_fft/NearFar.h:59-96): per-block partials of
+// sum_cells J_c exp(j k0 r.v) for the three components.
+__global__ __launch_bounds__(NT) void k_farsum(int n0, int n1, int n2, double d0, double d1,
+                                               double d2, double k0, double vx, double vy,
+                                               double vz, const double2* J, double2* partial)
+{
+    __shared__ double2 red[NT / 64];
+    const long V = (long)n0 * n1 * n2;
+    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int k = v % n2, j = (v / n2) % n1, i = v / ((long)n1 * n2);
+        const double a = k0 * ((i + 0.5) * d0 * vx + (j + 0.5) * d1 * vy + (k + 0.5) * d2 * vz);
+        double sa, ca;
+        sincos(a, &sa, &ca);
+        for (int c = 0; c < 3; ++c) {
+            const double2 jv = J[c * V + v];
+            acc[c].x += ca * jv.x - sa * jv.y;
+            acc[c].y += ca * jv.y + sa * jv.x;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        double2 s = block_sum(acc[c], red);
+        if (threadIdx.x == 0) partial[c * (long)gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Absorbed power of NearField (vie_fft/NearFar.h:186-250) for faces with
+// i,j,k < N-1 whose two cells both carry material `mat`.
+__global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV, double k0,
+                                               double z0, const double2* J, const double2* eps,
+                                               const uint8_t* mat, int mid, double2* partial)
+{
+    __shared__ double2 red[NT / 64];
+    const long V = (long)n0 * n1 * n2;
+    const long st[3] = {(long)n1 * n2, n2, 1};
+    double acc = 0;
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int k = v % n2, j = (v / n2) % n1, i = v / ((long)n1 * n2);
+        if (i >= n0 - 1 || j >= n1 - 1 || k >= n2 - 1) continue;
+        for (int f = 0; f < 3; ++f) {
+            const long w = v + st[f];
+            if (!(mat[w] == mid && mat[v] == mid)) continue;
+            const double2 a = zdiv(make_double2(1, 0), make_double2(eps[w].x - 1.0, eps[w].y));
+            const double2 b = zdiv(make_double2(1, 0), make_double2(eps[v].x - 1.0, eps[v].y));
+            const double2 S = make_double2(a.x + b.x, a.y + b.y);
+            const double absS = sqrt(S.x * S.x + S.y * S.y);
+            const double2 jv = J[f * V + v];
+            const double aE = sqrt(jv.x * jv.x + jv.y * jv.y) * absS * (0.5 * z0 / k0);
+            const double2 ee = zdiv(make_double2(2, 0), S);
+            // Im(-(ee + 1)) = -Im(ee)
+            acc += k0 / z0 * (-ee.y) * aE * aE * dV;
+        }
+    }
+    double2 s = block_sum(make_double2(acc, 0), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+}  // namespace vie

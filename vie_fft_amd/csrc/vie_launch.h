@@ -1,0 +1,34 @@
+// vie_launch.h -- host launchers of the pass kernels and the vector kernel,
+// instantiated for double and float in vie_kernels_f64.hip / _f32.hip.
+#pragma once
+
+#include "vie_passes.h"
+
+namespace vie {
+
+template <class R> struct LinComb;
+
+// axis lengths with compiled pass kernels
+bool size_supported(int n);
+
+template <class R>
+hipError_t run_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
+                    const int* done, hipStream_t st);
+template <class R>
+hipError_t run_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx<R>* tw, const cx<R>* twist,
+                    const int* done, hipStream_t st);
+template <class R>
+hipError_t run_xmix(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const MixTabs<R>& mt,
+                    const int* done, hipStream_t st);
+template <class R>
+hipError_t run_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
+                    const int* done, hipStream_t st);
+// *nblk receives the number of partial sums written per dot product
+template <class R>
+hipError_t run_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, cx<R>* y, const cx<R>* dg,
+                    int dconj, const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw,
+                    const cx<R>* twist, const int* done, hipStream_t st, long* nblk);
+// partial-sum count a zinv launch produces for these sizes
+template <class R> long zinv_blocks(const Geo& g);
+
+}  // namespace vie

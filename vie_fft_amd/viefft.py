@@ -1,0 +1,245 @@
+"""Host-side interface of the gfx950 VIE-FFT library (ctypes over ``include/viefft.h``).
+
+The object model mirrors the reference's host flow (paths relative to the
+reference root): ``vie_fft/CG_FFT_VIE.cpp:52-61`` runs Parameter_Input,
+GetMaterial/GetFlag, Green_Function_Compute, Incident_Wave, FFT_Program,
+FarField1 and NearField on global arrays; here a :class:`Solver` owns one
+grid and exposes the same steps with explicit arguments.  All compute runs
+in ``libviefft.so``; loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libviefft.so")
+
+FP64, FP32 = 0, 1
+CG, BICG, BICGSTAB, CORS, BICORSTAB = 1, 2, 3, 4, 5
+METHODS = {"cg": CG, "bicg": BICG, "bicgstab": BICGSTAB, "cors": CORS, "bicorstab": BICORSTAB}
+OK, ERR_ARG, ERR_DEVICE, ERR_OOM, NOT_CONVERGED, BREAKDOWN, ERR_NAN, ERR_UNSUPPORTED = range(8)
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_OOM", 4: "NOT_CONVERGED", 5: "BREAKDOWN",
+          6: "ERR_NAN", 7: "ERR_UNSUPPORTED"}
+
+# every symbol include/viefft.h declares
+EXPORTS = [
+    "vie_create", "vie_destroy", "vie_last_error", "vie_supported_n", "vie_set_medium",
+    "vie_set_medium_eps", "vie_green_compute", "vie_incident", "vie_apply", "vie_apply_async",
+    "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
+    "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
+    "vie_model_bytes_matvec",
+]
+
+
+class Desc(ct.Structure):
+    _fields_ = [("n", ct.c_int * 3), ("d", ct.c_double * 3), ("k0", ct.c_double), ("precision", ct.c_int)]
+
+
+class Info(ct.Structure):
+    _fields_ = [("iterations", ct.c_int), ("converged", ct.c_int), ("err", ct.c_double),
+                ("true_relres", ct.c_double), ("outer", ct.c_int), ("seconds", ct.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VieError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load libviefft.so (built by ``vie_fft_amd.build``); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run vie_fft_amd/build.py (hipcc, gfx950) first")
+    L = ct.CDLL(LIB_PATH)
+    dp = ct.POINTER(ct.c_double)
+    vp = ct.c_void_p
+    ub = ct.POINTER(ct.c_ubyte)
+    L.vie_create.argtypes = [ct.POINTER(vp), ct.POINTER(Desc)]
+    L.vie_destroy.argtypes = [vp]
+    L.vie_last_error.argtypes = [vp]
+    L.vie_last_error.restype = ct.c_char_p
+    L.vie_supported_n.argtypes = [ct.c_int]
+    L.vie_set_medium.argtypes = [vp, ub, dp, ct.c_int, ub]
+    L.vie_set_medium_eps.argtypes = [vp, dp, ub]
+    L.vie_green_compute.argtypes = [vp]
+    L.vie_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double, dp]
+    L.vie_apply.argtypes = [vp, dp, dp, ct.c_int]
+    L.vie_apply_async.argtypes = [vp, vp, vp, ct.c_int, vp]
+    L.vie_solve.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, ct.POINTER(Info)]
+    L.vie_set_rhs.argtypes = [vp, dp]
+    L.vie_solve_resident.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(Info)]
+    L.vie_get_solution.argtypes = [vp, dp]
+    L.vie_solve_mixed_resident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_int, ct.POINTER(Info)]
+    L.vie_forward_scatter.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double, dp]
+    L.vie_absorbed.argtypes = [vp, ct.c_int, dp]
+    L.vie_time_kernel.argtypes = [vp, ct.c_int]
+    L.vie_kernel_time.argtypes = [vp, ct.POINTER(ct.c_int), dp]
+    L.vie_model_bytes_matvec.argtypes = [vp]
+    L.vie_model_bytes_matvec.restype = ct.c_double
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _ub(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_ubyte))
+
+
+class Solver:
+    """One VIE grid on the GPU: operator, Green's function, solves, post-processing."""
+
+    def __init__(self, n, d, k0, precision=FP64):
+        self.lib = load()
+        self.n = tuple(int(v) for v in n)
+        self.V = int(np.prod(self.n))
+        desc = Desc()
+        for a in range(3):
+            desc.n[a] = self.n[a]
+            desc.d[a] = float(d[a])
+        desc.k0 = float(k0)
+        desc.precision = int(precision)
+        self.d = tuple(float(v) for v in d)
+        self.k0 = float(k0)
+        self.precision = int(precision)
+        h = ct.c_void_p()
+        st = self.lib.vie_create(ct.byref(h), ct.byref(desc))
+        if st != OK:
+            raise VieError(st, "vie_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vie_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st, allow=(OK,)):
+        if st not in allow:
+            msg = self.lib.vie_last_error(self.h)
+            raise VieError(st, msg.decode() if msg else "")
+        return st
+
+    def _field(self, x):
+        return np.ascontiguousarray(np.asarray(x, dtype=np.complex128).reshape((3,) + self.n))
+
+    # -- medium and set-up -------------------------------------------------
+    def set_medium(self, material, eps_table, flags):
+        """Material ids (V bytes), eps per id (engineering sign), flags (3,V)."""
+        m = np.ascontiguousarray(material, dtype=np.uint8).reshape(self.n)
+        e = np.ascontiguousarray(eps_table, dtype=np.complex128).ravel()
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape((3,) + self.n)
+        self._check(self.lib.vie_set_medium(self.h, _ub(m), _dp(e), e.size, _ub(f)))
+
+    def set_medium_eps(self, eps, flags):
+        e = np.ascontiguousarray(eps, dtype=np.complex128).reshape(self.n)
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape((3,) + self.n)
+        self._check(self.lib.vie_set_medium_eps(self.h, _dp(e), _ub(f)))
+
+    def green_compute(self):
+        """Green_Function_Compute (vie_fft/GreenFunction.h:8-141)."""
+        self._check(self.lib.vie_green_compute(self.h))
+
+    def incident(self, theta=0.0, phi=0.0, psi=0.0):
+        """Incident_Wave (vie_fft/GreenFunction.h:144-248), unmasked."""
+        E = np.zeros((3,) + self.n, np.complex128)
+        self._check(self.lib.vie_incident(self.h, theta, phi, psi, _dp(E)))
+        return E
+
+    # -- operator ----------------------------------------------------------
+    def apply(self, x, adjoint=False):
+        """y = A x (X_FFT/Y_FFT/Z_FFT + Current_Reset) or A^H x (the _Conjugate set)."""
+        x = self._field(x)
+        y = np.zeros_like(x)
+        self._check(self.lib.vie_apply(self.h, _dp(x), _dp(y), int(adjoint)))
+        return y
+
+    def apply_async(self, x_ptr, y_ptr, adjoint=False, stream=None):
+        """Device-pointer product (e.g. torch tensors' data_ptr()), enqueued on stream."""
+        self._check(self.lib.vie_apply_async(self.h, ct.c_void_p(x_ptr), ct.c_void_p(y_ptr), int(adjoint),
+                                             ct.c_void_p(stream) if stream else None))
+
+    # -- solves --------------------------------------------------------------
+    def solve(self, b, method=BICGSTAB, tol=1e-3, maxit=0):
+        """FFT_Program (vie_fft/CG_Iteration.h:1365-1388): returns (x, info)."""
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        b = self._field(b)
+        x = np.zeros_like(b)
+        info = Info()
+        self._check(self.lib.vie_solve(self.h, int(method), float(tol), int(maxit), _dp(b), _dp(x),
+                                       ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        return x, info.as_dict()
+
+    def set_rhs(self, b):
+        b = self._field(b)
+        self._check(self.lib.vie_set_rhs(self.h, _dp(b)))
+
+    def solve_resident(self, method=BICGSTAB, tol=1e-3, maxit=0):
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        info = Info()
+        self._check(self.lib.vie_solve_resident(self.h, int(method), float(tol), int(maxit), ct.byref(info)),
+                    allow=(OK, NOT_CONVERGED))
+        return info.as_dict()
+
+    def solve_mixed(self, tol=1e-6, inner_tol=0.0, maxit=0):
+        info = Info()
+        self._check(self.lib.vie_solve_mixed_resident(self.h, float(tol), float(inner_tol), int(maxit),
+                                                      ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        return info.as_dict()
+
+    def solution(self):
+        x = np.zeros((3,) + self.n, np.complex128)
+        self._check(self.lib.vie_get_solution(self.h, _dp(x)))
+        return x
+
+    # -- post-processing -----------------------------------------------------
+    def forward_scatter(self, theta=0.0, phi=0.0, psi=0.0):
+        """Forward_RCS of FarField1 (vie_fft/NearFar.h:16-109) for the resident solution."""
+        v = ct.c_double()
+        self._check(self.lib.vie_forward_scatter(self.h, theta, phi, psi, ct.byref(v)))
+        return v.value
+
+    def absorbed(self, mat):
+        """Ene_Metal of NearField (vie_fft/NearFar.h:180-250) for material id `mat`."""
+        v = ct.c_double()
+        self._check(self.lib.vie_absorbed(self.h, int(mat), ct.byref(v)))
+        return v.value
+
+    # -- instrumentation -----------------------------------------------------
+    def time_kernel(self, cls):
+        self._check(self.lib.vie_time_kernel(self.h, int(cls)))
+
+    def kernel_time(self):
+        n = ct.c_int()
+        ms = ct.c_double()
+        self._check(self.lib.vie_kernel_time(self.h, ct.byref(n), ct.byref(ms)))
+        return n.value, ms.value
+
+    def model_bytes_matvec(self):
+        return self.lib.vie_model_bytes_matvec(self.h)
+
+
+def supported_n(n):
+    return bool(load().vie_supported_n(int(n)))
