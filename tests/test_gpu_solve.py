@@ -1,11 +1,17 @@
 """GPU Krylov parity: the five solvers of vie_fft/CG_Iteration.h on the GPU
 against the CPU oracle's restatement.
 
-* fixed iteration count: the GPU and oracle iterates agree to FP64 roundoff
-  (field relative L2 <= 1e-9 after 8 iterations);
-* to convergence: iteration counts agree within +-1 and both solutions meet
-  the method's own stop rule;
-* the true FP64 residual of the GPU solution is reported and checked.
+* Dielectric sphere (eps = 4 + 0.5i): the iterates are insensitive to
+  roundoff (the oracle's two operator schedules, which differ only in
+  roundoff, agree to 1e-16 after 8 iterations), so the GPU must match the
+  oracle iterate for iterate (field relative L2 <= 1e-10 after 8 iterations)
+  and converge in the same number of iterations (+-1).
+* Plasmonic sphere (eps = -4 + 0.7i): BiCGSTAB passes a near breakdown at its
+  third iteration (recursive residual 1.24), which amplifies roundoff: the
+  oracle's own two schedules differ by 9e-4 after 8 iterations.  There the
+  check is on the converged solution (relative L2 <= 1e-4 at tol 1e-6) and on
+  the iteration count within the spread roundoff alone produces.
+* The true FP64 residual of every GPU solution is checked.
 """
 import numpy as np
 import pytest
@@ -18,7 +24,7 @@ from vie_fft_amd import viefft as VF
 pytestmark = pytest.mark.gpu
 
 
-def _case(n=16, R=6.0, eps_phys=-4.0 + 0.7j):
+def _case(n=16, R=6.0, eps_phys=4.0 + 0.5j):
     sc = G.sphere_case(n, radius=R, eps_phys=eps_phys, delta_nm=2.0)
     eps, flags = sc.eps(), sc.flags()
     b = O.incident(sc.n, sc.d, sc.k0) * flags
@@ -32,10 +38,11 @@ def test_fixed_iterations_match_oracle(method):
     with scene_solver(sc) as s:
         x, info = s.solve(b, method, tol=1e-300, maxit=8)
     assert info["iterations"] == ref["iterations"] == 8
-    assert rel(x, ref["x"]) <= 1e-9, rel(x, ref["x"])
+    assert rel(x, ref["x"]) <= 1e-10, rel(x, ref["x"])
 
 
-@pytest.mark.parametrize("method,tol", [(3, 1e-6), (4, 1e-6), (5, 1e-6), (2, 1e-8), (1, 1e-8)])
+# CG and BiCG stop on squared quantities (vie_fft/CG_Iteration.h:323, :495-496)
+@pytest.mark.parametrize("method,tol", [(3, 1e-10), (4, 1e-10), (5, 1e-10), (2, 1e-14), (1, 1e-14)])
 def test_converged_solve_matches_oracle(method, tol):
     sc, eps, flags, b = _case()
     ref = oracle_op(sc.n, sc.d, sc.k0, eps, flags).solve(b, method, tol=tol, maxit=2000)
@@ -44,30 +51,46 @@ def test_converged_solve_matches_oracle(method, tol):
     assert ref["converged"] and info["converged"]
     assert abs(info["iterations"] - ref["iterations"]) <= 1
     assert info["err"] < tol
+    assert rel(x, ref["x"]) <= 1e-8
+    assert info["true_relres"] <= 100 * (tol if method not in (1, 2) else np.sqrt(tol))
+
+
+@pytest.mark.parametrize("method", [3, 4, 5])
+def test_plasmonic_converged_solution_matches_oracle(method):
+    sc, eps, flags, b = _case(eps_phys=-4.0 + 0.7j)
+    op = oracle_op(sc.n, sc.d, sc.k0, eps, flags)
+    ref = op.solve(b, method, tol=1e-6, maxit=2000)
+    # roundoff spread of the oracle itself: same problem through the other schedule
+    alt = O.Operator(sc.n, sc.d, sc.k0, eps, flags, O.SCHED_REFERENCE).solve(b, method, tol=1e-6, maxit=2000)
+    with scene_solver(sc) as s:
+        x, info = s.solve(b, method, tol=1e-6, maxit=2000)
+    assert info["converged"] and info["true_relres"] < 5e-6
+    spread = abs(ref["iterations"] - alt["iterations"])
+    assert abs(info["iterations"] - ref["iterations"]) <= max(4, 2 * spread)
     assert rel(x, ref["x"]) <= 1e-4
-    assert info["true_relres"] <= 50 * (tol if method not in (1, 2) else np.sqrt(tol))
 
 
 def test_sphere_cross_sections_match_oracle():
-    """Config-1 style case: 32^3, sphere R = 10 cells, eps = -4 + 0.7i, 400 nm."""
+    """32^3, sphere R = 10 cells, eps = -4 + 0.7i, 400 nm: Q_ext and Q_abs of the
+    GPU solve (FarField1 / NearField on the device) against the oracle's."""
     sc = G.sphere_case(32)
     eps, flags, mat = sc.eps(), sc.flags(), sc.material()
     b = O.incident(sc.n, sc.d, sc.k0) * flags
-    ref = oracle_op(sc.n, sc.d, sc.k0, eps, flags).solve(b, 3, tol=1e-6)
+    ref = oracle_op(sc.n, sc.d, sc.k0, eps, flags).solve(b, 3, tol=1e-9)
     qe_ref = O.forward_scatter(sc.n, sc.d, sc.k0, ref["x"]) / sc.area
     qa_ref = O.absorbed(sc.n, sc.d, sc.k0, ref["x"], eps, mat, 1) * 120 * np.pi / sc.area
     with scene_solver(sc) as s:
         s.set_rhs(s.incident())
-        info = s.solve_resident(VF.BICGSTAB, tol=1e-6)
+        info = s.solve_resident(VF.BICGSTAB, tol=1e-9)
         qe = s.forward_scatter() / sc.area
         qa = s.absorbed(1) * 120 * np.pi / sc.area
-    assert abs(info["iterations"] - ref["iterations"]) <= 1
-    assert abs(qe - qe_ref) <= 1e-5 * abs(qe_ref)
-    assert abs(qa - qa_ref) <= 1e-5 * abs(qa_ref)
+    assert info["converged"] and info["true_relres"] < 1e-8
+    assert abs(qe - qe_ref) <= 1e-7 * abs(qe_ref)
+    assert abs(qa - qa_ref) <= 1e-7 * abs(qa_ref)
 
 
 def test_mixed_precision_reaches_fp64_residual():
-    sc, eps, flags, b = _case(n=32, R=10.0)
+    sc, eps, flags, b = _case(n=32, R=10.0, eps_phys=-4.0 + 0.7j)
     with scene_solver(sc) as s:
         s.set_rhs(b)
         ref = s.solve_resident(VF.BICGSTAB, tol=1e-8)
