@@ -62,6 +62,15 @@ template <class T> void dfree(T*& p)
     p = nullptr;
 }
 
+// Every copy goes through one stream and completes before returning: the
+// library stream is non-blocking, so null-stream copies would be unordered
+// with its kernels (device-to-device hipMemcpy may also return early).
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t st)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
 unsigned grid1(long n, int nt = NT, long cap = 4096)
 {
     long g = (n + nt - 1) / nt;
@@ -124,7 +133,7 @@ template <class R> struct Op {
     bool own_w = true;
     MixTabs<R> mf{}, ma{};
 
-    Op(const int* nn, const double* dd, double kk, T* w1, T* w2)
+    Op(const int* nn, const double* dd, double kk, T* w1, T* w2, hipStream_t st)
     {
         for (int a = 0; a < 3; ++a) {
             n[a] = nn[a];
@@ -168,13 +177,13 @@ template <class R> struct Op {
             Ua[a] = dalloc<T>(2 * N);
             Va[a] = dalloc<T>(2 * N);
             fold[a] = dalloc<int>(2 * N);
-            HIPCHK(hipMemcpy(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice));
+            copy_sync(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice, st);
         }
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
@@ -464,8 +473,8 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
     auto dg = diag_host(c);
     std::vector<cx<R>> h(dg.size());
     for (size_t i = 0; i < dg.size(); ++i) h[i] = hc<R>(dg[i]);
-    HIPCHK(hipMemcpy(op->dg, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice));
+    copy_sync(op->dg, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice, c->stream);
+    copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
 }
 
 template <> Op<double>* make_op<double>(vie_ctx* c)
@@ -476,10 +485,11 @@ template <> Op<double>* make_op<double>(vie_ctx* c)
         // an FP32 operator's work buffers are too small to share; allocate fresh
         (void)w1;
         (void)w2;
-        c->op64 = new Op<double>(c->desc.n, c->desc.d, c->desc.k0, nullptr, nullptr);
+        c->op64 = new Op<double>(c->desc.n, c->desc.d, c->desc.k0, nullptr, nullptr, c->stream);
         if (c->green_ready)
-            HIPCHK(hipMemcpy(c->op64->G, c->Goff, (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1) *
-                                                      sizeof(double2), hipMemcpyDeviceToDevice));
+            copy_sync(c->op64->G, c->Goff,
+                      (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1) * sizeof(double2),
+                      hipMemcpyDeviceToDevice, c->stream);
         if (c->medium_ready) upload_medium<double>(c, c->op64);
     }
     return c->op64;
@@ -491,7 +501,7 @@ template <> Op<float>* make_op<float>(vie_ctx* c)
         // the FP32 pass buffers fit in the FP64 ones: share them
         float2* w1 = c->op64 ? (float2*)c->op64->W1 : nullptr;
         float2* w2 = c->op64 ? (float2*)c->op64->W2 : nullptr;
-        c->op32 = new Op<float>(c->desc.n, c->desc.d, c->desc.k0, w1, w2);
+        c->op32 = new Op<float>(c->desc.n, c->desc.d, c->desc.k0, w1, w2, c->stream);
         const long tot = (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1);
         if (c->green_ready) {
             hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
@@ -980,7 +990,7 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
         }
     dfree(c->eps_dev);
     c->eps_dev = dalloc<double2>(c->V);
-    HIPCHK(hipMemcpy(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice));
+    copy_sync(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     c->medium_ready = true;
     if (c->op64) upload_medium<double>(c, c->op64);
     if (c->op32) upload_medium<float>(c, c->op32);
@@ -1002,7 +1012,7 @@ int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_
         c->mat_h.assign(material, material + c->V);
         dfree(c->mat_dev);
         c->mat_dev = dalloc<uint8_t>(c->V);
-        HIPCHK(hipMemcpy(c->mat_dev, material, c->V, hipMemcpyHostToDevice));
+        copy_sync(c->mat_dev, material, c->V, hipMemcpyHostToDevice, c->stream);
         return set_medium_common(c, flags);
     } catch (const Err& e) {
         return fail(c, e);
@@ -1090,7 +1100,7 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
     try {
         check_ready(c);
         const long n = 3 * c->V;
-        HIPCHK(hipMemcpy(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice));
+        copy_sync(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         if (c->desc.precision == VIE_FP64) {
             apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
         } else {
@@ -1100,8 +1110,7 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             apply_dev<float>(c, xs, ys, adjoint, c->stream);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
         }
-        HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost));
+        copy_sync(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -1116,7 +1125,7 @@ int vie_set_rhs(vie_ctx* c, const double* b)
         const long n = 3 * c->V;
         for (long i = 0; i < 2 * n; ++i)
             if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
-        HIPCHK(hipMemcpy(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice));
+        copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         Op<double>* op = make_op<double>(c);
         // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
         hipLaunchKernelGGL(k_mask<double>, dim3(grid1(n, NT, 1L << 30)), dim3(NT), 0, c->stream, n, c->t64, op->flags,
@@ -1264,7 +1273,7 @@ int vie_get_solution(vie_ctx* c, double* x)
     if (!c || !x) return VIE_ERR_ARG;
     try {
         require(c, c->sol_ready, "no solution on the device");
-        HIPCHK(hipMemcpy(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost));
+        copy_sync(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

@@ -459,26 +459,27 @@ __global__ void k_green_cos(int L0, int L1, int L2, int axis, const double* ctab
     out[t] = acc;
 }
 
+// Element-wise kernels are grid-stride loops: launches cap the grid size.
 template <class R>
 __global__ void k_convert(long n, const double2* in, cx<R>* out)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) out[t] = mkc((R)in[t].x, (R)in[t].y);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[t] = mkc((R)in[t].x, (R)in[t].y);
 }
 
 template <class R>
 __global__ void k_to_f64(long n, const cx<R>* in, double2* out)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) out[t] = make_double2((double)in[t].x, (double)in[t].y);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[t] = make_double2((double)in[t].x, (double)in[t].y);
 }
 
 // Mask a field by the flags: y = flag ? x : 0 (Current_Reset, vie_fft/fft.h:13-35)
 template <class R>
 __global__ void k_mask(long n, const cx<R>* x, const uint8_t* flags, cx<R>* y)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) y[t] = flags[t] ? x[t] : czero<cx<R>>();
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        y[t] = flags[t] ? x[t] : czero<cx<R>>();
 }
 
 // Far-field sums of FarField1 (vie_fft/NearFar.h:59-96): per-block partials of
