@@ -5,11 +5,13 @@ Workload: 128^3 cells, cell 2 nm, layered slab (z-bands eps 2.1 / 3.6+0.35i /
 normal-incidence x-polarised plane wave at 550 nm, zero-padded 256^3
 transforms, BiCGSTAB to relative residual 1e-6.  A "step" is one complete
 solve from x = 0 with the right-hand side, medium and Green's function
-already resident in HBM.  `value` is the mixed-precision (FP32-complex inner
-BiCGSTAB, FP64 refinement) solve rate; the plain FP64 solve is reported
-beside it.  The roofline entry covers the dominant kernel of the mixed solve
-(the FP32 x-axis spectral pass), timed per launch with HIP events on the
-library's stream during the timed solves.
+already resident in HBM.  `value` is the mixed-precision solve rate
+(FP32-complex BiCGSTAB recurrences and operator, FP64 solution, residual
+replaced by the true FP64 residual every 10x reduction, FP64 stop rule); the
+plain FP64 solve is reported beside it.  The roofline entry covers the
+dominant kernel of the mixed solve (the FP32 x-axis pass: forward transform,
+spectral Green's-function product, inverse transform), timed per launch with
+HIP events on the library's stream during the timed solves.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -51,6 +53,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--n", type=int, default=128, help="grid edge (128 = the metric workload)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time a plain device copy of the x pass's working set (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it "
@@ -63,6 +67,9 @@ def main():
     from vie_fft_amd import viefft as VF
 
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.copy_probe:
+        copy_probe(torch, args.n)
+        return
     sc = G.metric_case() if args.n == 128 else G.layered_case(n=(args.n,) * 3, band_start=args.n // 2 - 24,
                                                                sphere_z=args.n // 2 + 4)
     n = sc.n
@@ -122,14 +129,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 inner / c128 refinement",
+        "dtype": "c64 recurrences / c128 solution and residual",
         "data": "synthetic",
         "config": {
             "workload": f"{n[0]}x{n[1]}x{n[2]} layered slab + 4x4 spheres, cell 2 nm, 550 nm x-pol, "
                         f"padded {2*n[0]}x{2*n[1]}x{2*n[2]} transforms, BiCGSTAB to rel. residual {args.tol:g}",
             "grid": list(n),
         },
-        "mixed": {"solves_per_s": 1.0 / tm, "ms_per_solve": tm * 1e3, "inner_iterations": itm, "outer": outer,
+        "mixed": {"solves_per_s": 1.0 / tm, "ms_per_solve": tm * 1e3, "iterations": itm,
+                  "residual_replacements": outer,
                   "true_relres": infosm[-1]["true_relres"],
                   "model_hbm_frac": solve_bytes_m / tm / HBM_PEAK},
         "fp64": {"solves_per_s": 1.0 / t64, "ms_per_solve": t64 * 1e3, "iterations": it64,
@@ -140,13 +148,33 @@ def main():
         "speedup_mixed_vs_fp64": t64 / tm,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_xmix<float> (x-axis forward + spectral 3x3 product + inverse)",
+                     "kernel": "k_xreg<float,128,32,XPlan<8,8,4>> (x-axis forward + spectral product + inverse)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
     print(json.dumps(out))
     s.close()
+
+
+def copy_probe(torch, n):
+    """Copy 3 x 4 n^3 complex64 (the FP32 x pass's array) 20 times: the
+    reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters and the
+    practical HBM copy rate."""
+    nbytes = 3 * 4 * n ** 3 * 8
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms": ms,
+                      "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
 
 
 def cpu_baseline(sc, flags, iterations):

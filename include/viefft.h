@@ -105,7 +105,8 @@ int vie_incident(vie_ctx* ctx, double theta, double phi, double psi, double* E);
 int vie_apply(vie_ctx* ctx, const double* x, double* y, int adjoint);
 /* same on device buffers of the context's precision (complex<double> for
    VIE_FP64, complex<float> for VIE_FP32), enqueued on `stream` (a hipStream_t,
-   NULL = the context's stream) without a host synchronisation. */
+   NULL = the context's stream) without a host synchronisation; both buffers
+   16-byte aligned. */
 int vie_apply_async(vie_ctx* ctx, const void* x_dev, void* y_dev, int adjoint, void* stream);
 
 /* ---- solve (FFT_Program, vie_fft/CG_Iteration.h:1365-1388) --------------- */
@@ -119,11 +120,21 @@ int vie_solve(vie_ctx* ctx, int method, double tol, int maxit, const double* b, 
 int vie_set_rhs(vie_ctx* ctx, const double* b);
 int vie_solve_resident(vie_ctx* ctx, int method, double tol, int maxit, vie_info* info);
 int vie_get_solution(vie_ctx* ctx, double* x);
-/* FP32-complex inner BiCGSTAB wrapped in FP64 iterative refinement: the FP64
-   residual b - A64 x and the correction update stay in FP64.  Needs a
-   context created with VIE_FP64; inner_tol <= 0 picks a default. */
-int vie_solve_mixed_resident(vie_ctx* ctx, double tol, double inner_tol, int maxit,
-                             vie_info* info);
+/* Mixed precision: the Krylov recurrences, FFTs and spectral product run in
+   FP32 complex while the solution accumulates in FP64 and the residual that
+   decides convergence is the true FP64 residual b - A64 x (stop rule
+   ||b - A64 x|| / ||b|| < tol).  Needs a context created with VIE_FP64.
+     VIE_MIXED_RELIABLE (BiCGSTAB only): one FP32 BiCGSTAB whose residual is
+       replaced by the FP64 residual whenever it has fallen by inner_param
+       (<= 0: 0.1) since the last replacement, without restarting;
+     VIE_MIXED_REFINE: classic iterative refinement, each step an FP32 solve
+       (BiCGSTAB, CORS or BiCORSTAB) of A e = r to inner_param (<= 0: 1e-4)
+       followed by x += e in FP64.
+   maxit bounds the FP32 iterations; info->outer counts residual replacements
+   or refinement steps. */
+typedef enum { VIE_MIXED_REFINE = 0, VIE_MIXED_RELIABLE = 1 } vie_mixed_strategy;
+int vie_solve_mixed_resident(vie_ctx* ctx, int strategy, int inner_method, double tol, double inner_param,
+                             int maxit, vie_info* info);
 
 /* ---- post-processing ----------------------------------------------------- */
 /* Forward_RCS of FarField1 (vie_fft/NearFar.h:16-109) of the resident

@@ -17,7 +17,10 @@ from vie_fft_amd import viefft as VF
 pytestmark = pytest.mark.gpu
 
 GRIDS = [((3, 4, 5), (0.02, 0.02, 0.02)), ((4, 6, 10), (0.02, 0.03, 0.025)), ((8, 8, 8), (0.02, 0.02, 0.02)),
-         ((16, 12, 20), (0.015, 0.015, 0.015)), ((24, 16, 10), (0.02, 0.02, 0.02)), ((32, 32, 32), (0.02, 0.02, 0.02))]
+         ((16, 12, 20), (0.015, 0.015, 0.015)), ((24, 16, 10), (0.02, 0.02, 0.02)), ((32, 32, 32), (0.02, 0.02, 0.02)),
+         # x extents with the register-resident x pass (Nx = 64, 96, 128, 256)
+         ((64, 8, 10), (0.02, 0.025, 0.02)), ((96, 6, 8), (0.02, 0.02, 0.02)), ((128, 16, 12), (0.02, 0.02, 0.02)),
+         ((256, 4, 6), (0.01, 0.02, 0.02))]
 
 
 def test_tiny_grid_matches_dense_assembly():

@@ -120,13 +120,14 @@ template <class R> struct Op {
     double d[3], k0;
     T* tw[3] = {};
     T* twist[3] = {};
+    T* twP[3] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
     T* U[3] = {};
     T* Vv[3] = {};
     T* Ua[3] = {};
     T* Va[3] = {};
     int* fold[3] = {};
     T* G = nullptr;
-    T* dg = nullptr;
+    T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
     T* W1 = nullptr;
     T* W2 = nullptr;
@@ -185,9 +186,18 @@ template <class R> struct Op {
             copy_sync(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
             copy_sync(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice, st);
         }
+        for (int a = 0; a < 3; ++a) {
+            const int P = 2 * n[a];
+            std::vector<T> h(P);
+            for (int e = 0; e < P; ++e) {
+                const long double ang = -2.0L * (long double)kPi * e / P;
+                h[e] = hc<R>(std::complex<double>((double)cosl(ang), (double)sinl(ang)));
+            }
+            twP[a] = dalloc<T>(P);
+            copy_sync(twP[a], h.data(), P * sizeof(T), hipMemcpyHostToDevice, st);
+        }
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
-        dg = dalloc<T>(3 * g.V);
         flags = dalloc<uint8_t>(3 * g.V);
         if (w1) {
             W1 = w1;
@@ -225,6 +235,7 @@ template <class R> struct Op {
             dfree(Va[a]);
             dfree(fold[a]);
         }
+        for (int a = 0; a < 3; ++a) dfree(twP[a]);
         dfree(G);
         dfree(dg);
         dfree(flags);
@@ -243,19 +254,22 @@ template <class R> struct Op {
         const int off = sizeof(R) == 8 ? 0 : 10;  // timing classes 1-5 FP64, 11-15 FP32
         auto t0 = [&](int c) { if (tm && tm->cls == c + off) tm->mark(st); };
         t0(1);
-        HIPCHK(run_zfwd<R>(g, lin, W1, tw[2], twist[2], done, st));
+        HIPCHK(run_zfwd<R>(g, lin, W1, tw[2], twist[2], twP[2], done, st));
         t0(1);
         t0(2);
-        HIPCHK(run_yfwd<R>(g, W1, W2, tw[1], twist[1], done, st));
+        HIPCHK(run_yfwd<R>(g, W1, W2, tw[1], twist[1], twP[1], done, st));
         t0(2);
         t0(3);
-        HIPCHK(run_xmix<R>(g, W2, tw[0], twist[0], adj ? ma : mf, done, st));
+        HIPCHK(run_xmix<R>(g, W2, tw[0], twist[0], twP[0], adj ? ma : mf, done, st));
         t0(3);
         t0(4);
-        HIPCHK(run_yinv<R>(g, W2, W1, tw[1], twist[1], done, st));
+        HIPCHK(run_yinv<R>(g, W2, W1, tw[1], twist[1], twP[1], done, st));
         t0(4);
+        Diag<R> dd{};
+        dd.d = dg;
+        dd.conj = adj ? 1 : 0;
         t0(5);
-        HIPCHK(run_zinv<R>(g, W1, src, y, dg, adj ? 1 : 0, flags, dots, tw[2], twist[2], done, st, &nblk));
+        HIPCHK(run_zinv<R>(g, W1, src, y, dd, flags, dots, tw[2], twist[2], twP[2], done, st, &nblk));
         t0(5);
         return nblk;
     }
@@ -288,7 +302,6 @@ struct vie_ctx {
     bool green_ready = false;
     bool medium_ready = false;
     std::vector<double> eps_h;     // per-voxel eps, 2V doubles
-    std::vector<uint8_t> mat_h;    // material ids (if given)
     std::vector<uint8_t> flags_h;  // 3V
     double2* eps_dev = nullptr;
     uint8_t* mat_dev = nullptr;
@@ -352,7 +365,7 @@ template <class R> void fill0(vie_ctx* c, cx<R>* x, long n)
 
 template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 {
-    const unsigned gsz = grid1(L.n, NT, 1024);
+    const unsigned gsz = grid1((L.n + 1) / 2, NT, 2048);
     L.dots.partial = c->partial;
     hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz), dim3(NT), 0, c->stream, L, done);
     HIPCHK(hipGetLastError());
@@ -473,6 +486,7 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
     auto dg = diag_host(c);
     std::vector<cx<R>> h(dg.size());
     for (size_t i = 0; i < dg.size(); ++i) h[i] = hc<R>(dg[i]);
+    if (!op->dg) op->dg = dalloc<cx<R>>(3 * c->V);
     copy_sync(op->dg, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice, c->stream);
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
 }
@@ -814,6 +828,130 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
     return drive(c, st, iter);
 }
 
+// Mixed-precision BiCGSTAB with reliable updates: the BiCGSTAB recurrences of
+// solve_bicgstab run in FP32 (operator, FFTs, vectors), the solution is
+// accumulated in FP64, and whenever the recursive residual has fallen by
+// `delta` since the last replacement (and when it meets the tolerance) the
+// FP32 residual is replaced by the true FP64 residual b - A64 x64 without
+// restarting the Krylov process.  The stop rule is the reference's
+// ||r|| / ||b|| < tol (vie_fft/CG_Iteration.h:727-740), checked on the true
+// FP64 residual.  Returns the FP32 state; *true_err and *replacements report
+// the final FP64 residual and the number of replacements.
+KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, double* true_err,
+                               int* replacements)
+{
+    using R = float;
+    Op<float>* op = make_op<float>(c);
+    Op<double>* op64 = make_op<double>(c);
+    KState* st = c->state;
+    KState* ax = c->aux;
+    const long n = 3 * c->V;
+    float2 *b = vec<R>(c, VB), *x = vec<R>(c, VX), *r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ),
+           *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
+    const int* done = &st->done;
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->b64, b);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_zero<double>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->x64);
+    HIPCHK(hipGetLastError());
+    // FP64 ||b||^2 for the true residual
+    state_init(c, ax, 0.0, 1);
+    {
+        LinComb<double> L{};
+        L.n = n;
+        L.x[0] = c->b64;
+        L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+        long nb = lincomb<double>(c, L, nullptr);
+        scalar(c, ax, SC_BNRM, nb, 1);
+    }
+    state_init(c, st, tol, maxit);
+    fill0<R>(c, p, n);
+    fill0<R>(c, Ap, n);
+    {
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {1, 1}, b, nullptr);
+        long nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_INIT, nb, 1);
+    }
+    auto iter = [&]() {
+        Lin<R> lp{};
+        lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
+        lp.coef = &st->cf[CF_BS_P];
+        lp.out = p;
+        long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
+        Lin<R> lq{};
+        lq.x[0] = r; lq.x[1] = Ap;
+        lq.coef = &st->cf[CF_BS_Q];
+        lq.out = q;
+        nb = op->apply(lq, q, Aq, dots_with<R>(c, 2, {0, 1, 0, 0}, q, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_OMEGA, nb, 2);
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = x; L.x[1] = p; L.x[2] = q; L.x[3] = Aq;
+        L.coef = &st->cf[CF_BS_U];
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 2, {3, 3, 1, 3}, b, nullptr);
+        nb = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_END, nb, 2);
+    };
+    double err_ref = 1.0, terr = 1.0;
+    int nrep = 0;
+    KState h{};
+    for (;;) {
+        for (int k = 0; k < c->ahead; ++k) iter();
+        h = read_state(c, st);
+        const bool stop = h.status != KS_OK || h.iter >= h.maxit;
+        if (!(h.done || h.err < delta * err_ref || stop)) continue;
+        // reliable update: x64 += x32 ; x32 = 0 ; r = b - A64 x64 (in FP64, then FP32)
+        hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
+        HIPCHK(hipGetLastError());
+        fill0<R>(c, x, n);
+        op64->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, &c->timer);
+        LinComb<double> M{};
+        M.n = n;
+        M.x[0] = c->b64;
+        M.x[1] = c->t64;
+        M.hc[0][0] = 1.0;
+        M.hc[1][0] = -1.0;
+        M.out = c->r64;
+        M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+        long nb = lincomb<double>(c, M, nullptr);
+        scalar(c, ax, SC_RELRES, nb, 1);
+        ++nrep;
+        terr = read_state(c, ax).err;
+        if (terr < tol || stop || !std::isfinite(terr)) break;
+        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->r64, r);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_state_resume, dim3(1), dim3(1), 0, c->stream, st);
+        HIPCHK(hipGetLastError());
+        LinComb<R> Rr{};  // b^H r, r^H r of the replaced residual
+        Rr.n = n;
+        Rr.x[0] = r;
+        Rr.hc[0][0] = 1.0;
+        Rr.dots = dots_with<R>(c, 2, {1, 0, 0, 0}, b, nullptr);
+        nb = lincomb<R>(c, Rr, nullptr);
+        scalar(c, st, SC_BICGSTAB_REPLACE, nb, 2);
+        err_ref = terr;
+    }
+    h = read_state(c, st);
+    if (terr < tol) {
+        h.converged = 1;
+        h.status = KS_OK;
+    } else if (h.status == KS_OK) {
+        h.converged = 0;
+    }
+    h.err = terr;
+    *true_err = terr;
+    *replacements = nrep;
+    return h;
+}
+
 template <class R> KState solve_method(vie_ctx* c, int method, const cx<R>* b, cx<R>* x, double tol, int maxit)
 {
     Op<R>* op = make_op<R>(c);
@@ -853,10 +991,18 @@ double true_relres(vie_ctx* c)
     return read_state(c, s).err;
 }
 
-int status_of(const KState& h)
+int status_of(vie_ctx* c, const KState& h)
 {
-    if (h.status == KS_BREAKDOWN) return VIE_BREAKDOWN;
-    if (h.status == KS_NAN) return VIE_ERR_NAN;
+    if (h.status == KS_BREAKDOWN) {
+        c->err = "Krylov breakdown (zero or non-finite denominator) in scalar step " + std::to_string(h.fail_op) +
+                 " at iteration " + std::to_string(h.iter);
+        return VIE_BREAKDOWN;
+    }
+    if (h.status == KS_NAN) {
+        c->err = "non-finite residual at iteration " + std::to_string(h.iter);
+        return VIE_ERR_NAN;
+    }
+    if (!h.converged) c->err = "maxit reached after " + std::to_string(h.iter) + " iterations";
     return h.converged ? VIE_OK : VIE_NOT_CONVERGED;
 }
 
@@ -1009,7 +1155,7 @@ int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_
             c->eps_h[2 * v] = eps_table[2 * m];
             c->eps_h[2 * v + 1] = eps_table[2 * m + 1];
         }
-        c->mat_h.assign(material, material + c->V);
+        require(c, n_mat <= 256, "at most 256 materials");
         dfree(c->mat_dev);
         c->mat_dev = dalloc<uint8_t>(c->V);
         copy_sync(c->mat_dev, material, c->V, hipMemcpyHostToDevice, c->stream);
@@ -1024,7 +1170,6 @@ int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags
     if (!c || !eps || !flags) return VIE_ERR_ARG;
     try {
         c->eps_h.assign(eps, eps + 2 * c->V);
-        c->mat_h.clear();
         dfree(c->mat_dev);
         return set_medium_common(c, flags);
     } catch (const Err& e) {
@@ -1081,6 +1226,10 @@ int vie_incident(vie_ctx* c, double theta, double phi, double psi, double* E)
 int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* stream)
 {
     if (!c || !x || !y) return VIE_ERR_ARG;
+    if (((uintptr_t)x | (uintptr_t)y) % 16) {
+        c->err = "device buffers must be 16-byte aligned";
+        return VIE_ERR_ARG;
+    }
     try {
         check_ready(c);
         hipStream_t st = stream ? (hipStream_t)stream : c->stream;
@@ -1173,22 +1322,58 @@ int vie_solve_resident(vie_ctx* c, int method, double tol, int maxit, vie_info* 
             info->seconds = std::chrono::duration<double>(t1 - t0).count();
             info->true_relres = true_relres(c);
         }
-        return status_of(h);
+        return status_of(c, h);
     } catch (const Err& e) {
         return fail(c, e);
     }
 }
 
-int vie_solve_mixed_resident(vie_ctx* c, double tol, double inner_tol, int maxit, vie_info* info)
+int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double tol, double inner_param,
+                             int maxit, vie_info* info)
 {
     if (!c) return VIE_ERR_ARG;
+    if (strategy == VIE_MIXED_RELIABLE) {
+        try {
+            check_ready(c);
+            require(c, c->rhs_ready, "vie_set_rhs has not been called");
+            require(c, c->desc.precision == VIE_FP64, "mixed solve needs an FP64 context");
+            require(c, inner_method == VIE_BICGSTAB, "reliable updates are implemented for BiCGSTAB");
+            require(c, tol > 0, "tol must be positive");
+            if (maxit <= 0) maxit = (int)std::min<long>(c->V, 1L << 30);
+            const double delta = inner_param > 0 ? inner_param : 0.1;
+            timing_begin(c);
+            auto t0 = std::chrono::steady_clock::now();
+            double terr = 1.0;
+            int nrep = 0;
+            KState h = solve_bicgstab_reliable(c, tol, maxit, delta, &terr, &nrep);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            auto t1 = std::chrono::steady_clock::now();
+            timing_end(c);
+            c->sol_ready = true;
+            if (info) {
+                info->iterations = h.iter;
+                info->converged = h.converged;
+                info->err = terr;
+                info->outer = nrep;
+                info->seconds = std::chrono::duration<double>(t1 - t0).count();
+                info->true_relres = terr;
+            }
+            return status_of(c, h);
+        } catch (const Err& e) {
+            return fail(c, e);
+        }
+    }
+    const double inner_tol = inner_param;
     try {
+        require(c, strategy == VIE_MIXED_REFINE, "unknown mixed-precision strategy");
+        require(c, inner_method == VIE_BICGSTAB || inner_method == VIE_CORS || inner_method == VIE_BICORSTAB,
+                "inner method must be BiCGSTAB, CORS or BiCORSTAB");
         check_ready(c);
         require(c, c->rhs_ready, "vie_set_rhs has not been called");
         require(c, c->desc.precision == VIE_FP64, "mixed solve needs an FP64 context");
         require(c, tol > 0, "tol must be positive");
         if (maxit <= 0) maxit = (int)std::min<long>(c->V, 1L << 30);
-        if (inner_tol <= 0) inner_tol = 1e-4;
+        const double itol0 = inner_tol > 0 ? inner_tol : 1e-4;
         const long n = 3 * c->V;
         Op<double>* op64 = make_op<double>(c);
         make_op<float>(c);
@@ -1207,8 +1392,12 @@ int vie_solve_mixed_resident(vie_ctx* c, double tol, double inner_tol, int maxit
         long nb = lincomb<double>(c, L, nullptr);
         scalar(c, s, SC_BNRM, nb, 1);
         int total = 0, outer = 0, status = VIE_OK;
-        double err = 1.0, rnorm = 0.0;
+        double err = 1.0, rnorm = 0.0, prev_err = 0.0, prev_scale = 0.0;
         KState last{};
+        // Safeguards: one inner solve runs at most inner_cap iterations, and a
+        // correction that raises the FP64 residual is taken back (x -= s e)
+        // before the loop stops, so the result is never worse than before.
+        const int inner_cap = std::max(200, (int)std::min<long>(maxit, 4000));
         for (;;) {
             // r64 = b - A64 x64 (FP64 residual), ||r64||
             if (outer == 0) {
@@ -1234,21 +1423,40 @@ int vie_solve_mixed_resident(vie_ctx* c, double tol, double inner_tol, int maxit
                 err = h.err;
                 rnorm = std::sqrt(h.sums[0].x);
             }
-            if (!std::isfinite(err)) { status = VIE_ERR_NAN; break; }
+            if (!std::isfinite(err) || (outer > 0 && err >= prev_err)) {
+                // the last correction did not help: take it back
+                hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, -prev_scale, e32,
+                                   c->x64);
+                HIPCHK(hipGetLastError());
+                err = prev_err;
+                status = VIE_NOT_CONVERGED;
+                c->err = "FP32 inner solve stopped reducing the FP64 residual (" + std::to_string(err) + ")";
+                break;
+            }
             if (err < tol) break;
-            if (total >= maxit) { status = VIE_NOT_CONVERGED; break; }
+            if (total >= maxit) {
+                status = VIE_NOT_CONVERGED;
+                c->err = "maxit reached";
+                break;
+            }
             // inner FP32 solve of A e = r / ||r||
             const double sc = 1.0 / rnorm;
             hipLaunchKernelGGL(k_scale_from64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, sc, c->r64, b32);
             HIPCHK(hipGetLastError());
-            const double itol = std::max(inner_tol, 0.5 * tol / err);
-            last = solve_method<float>(c, VIE_BICGSTAB, b32, e32, std::min(itol, 0.5), maxit - total);
+            const double itol = std::max(itol0, 0.5 * tol / err);
+            last = solve_method<float>(c, inner_method, b32, e32, std::min(itol, 0.5),
+                                       std::min(maxit - total, inner_cap));
             total += last.iter;
             ++outer;
-            if (last.status == KS_NAN) { status = VIE_ERR_NAN; break; }
+            if (last.status == KS_NAN || (last.status == KS_BREAKDOWN && last.iter == 0)) {
+                status = last.status == KS_NAN ? VIE_ERR_NAN : VIE_BREAKDOWN;
+                c->err = "FP32 inner solve failed at its start";
+                break;
+            }
             hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, rnorm, e32, c->x64);
             HIPCHK(hipGetLastError());
-            if (last.status == KS_BREAKDOWN && last.iter == 0) { status = VIE_BREAKDOWN; break; }
+            prev_err = err;
+            prev_scale = rnorm;
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         auto t1 = std::chrono::steady_clock::now();

@@ -152,6 +152,48 @@ template <int D, class T> struct dft_impl<5, D, T> {
     }
 };
 
+// radix 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2,
+// X[k] = sum_n2 w4^{n2 k2} (w16^{n2 k1} sum_n1 w4^{n1 k1} x[4 n1 + n2])
+template <int D, class T> struct dft_impl<16, D, T> {
+    static __device__ __forceinline__ void run(T* v)
+    {
+        using R = decltype(v[0].x);
+        const R c1 = (R)0.92387953251128675612818318939679;  // cos(pi/8)
+        const R s1 = (R)0.38268343236508977172845998403040;  // sin(pi/8)
+        const R h = (R)0.70710678118654752440084436210485;
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            T a[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+            dft_impl<4, D, T>::run(a);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) v[n2 + 4 * k1] = a[k1];
+        }
+        // twiddles w16^{n2 k1}: w = c + D j s (D = -1: exp(-j ...), D = +1: its conjugate),
+        // x w = (x.x c - D x.y s, x.y c + D x.x s)
+#define VIE_TW(idx, c, s) v[idx] = mkc(v[idx].x * (c) - (R)D * v[idx].y * (s), v[idx].y * (c) + (R)D * v[idx].x * (s))
+        VIE_TW(1 + 4 * 1, c1, s1);   // w^1
+        VIE_TW(1 + 4 * 2, h, h);     // w^2
+        VIE_TW(1 + 4 * 3, s1, c1);   // w^3
+        VIE_TW(2 + 4 * 1, h, h);     // w^2
+        v[2 + 4 * 2] = mulj<D>(v[2 + 4 * 2]);  // w^4 = D j
+        VIE_TW(2 + 4 * 3, -h, h);    // w^6
+        VIE_TW(3 + 4 * 1, s1, c1);   // w^3
+        VIE_TW(3 + 4 * 2, -h, h);    // w^6
+        VIE_TW(3 + 4 * 3, -c1, -s1); // w^9
+#undef VIE_TW
+        T o[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            T a[4] = {v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]};
+            dft_impl<4, D, T>::run(a);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = a[k2];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = o[k];
+    }
+};
+
 // One Stockham radix-R stage over COLS independent columns held in LDS as
 // buf[element * LD + column].  S is the product of the radices already done.
 // Butterfly i reads elements i + q*N/R, applies w_{S R}^{q (i mod S)} and

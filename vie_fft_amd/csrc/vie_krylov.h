@@ -26,7 +26,7 @@ struct KState {
     double2 sums[4];
     double bnrm2, err, tol;
     int iter, maxit, done, converged, status;
-    int pad;
+    int fail_op;          // scalar program that stopped the solve, if any
 };
 
 enum { KS_OK = 0, KS_BREAKDOWN = 5, KS_NAN = 6 };
@@ -72,8 +72,38 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
         }
     }
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    auto dots_at = [&](long i, T v, T v2) {
+        for (int d = 0; d < L.dots.n; ++d) {
+            const int sa = L.dots.a[d], sb = L.dots.b[d];
+            const T a = sa == 0 ? v : sa == 1 ? L.dots.e1[i] : sa == 2 ? L.dots.e2[i] : v2;
+            const T b = sb == 0 ? v : sb == 1 ? L.dots.e1[i] : sb == 2 ? L.dots.e2[i] : v2;
+            dot_acc(acc[d], a, b, L.dots.nc[d]);
+        }
+    };
     const long stride = (long)gridDim.x * NT;
-    for (long i = (long)blockIdx.x * NT + threadIdx.x; i < L.n; i += stride) {
+    // pairs of entries (16-byte accesses for FP32), then a possible odd tail
+    const long npair = L.n / 2;
+    for (long i2 = (long)blockIdx.x * NT + threadIdx.x; i2 < npair; i2 += stride) {
+        const long i = 2 * i2;
+        T v = czero<T>(), v2 = czero<T>(), w = czero<T>(), w2 = czero<T>();
+        for (int q = 0; q < 4; ++q)
+            if (L.x[q]) {
+                T a0, a1;
+                ld2(L.x[q] + i, a0, a1);
+                v = cadd(v, cmul(c[q], a0));
+                w = cadd(w, cmul(c[q], a1));
+                if (L.out2) {
+                    v2 = cadd(v2, cmul(c2[q], a0));
+                    w2 = cadd(w2, cmul(c2[q], a1));
+                }
+            }
+        dots_at(i, v, v2);
+        dots_at(i + 1, w, w2);
+        if (L.out) st2(L.out + i, v, w);
+        if (L.out2) st2(L.out2 + i, v2, w2);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (L.n & 1)) {
+        const long i = L.n - 1;
         T v = czero<T>(), v2 = czero<T>();
         for (int q = 0; q < 4; ++q)
             if (L.x[q]) {
@@ -81,12 +111,7 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
                 v = cadd(v, cmul(c[q], xv));
                 if (L.out2) v2 = cadd(v2, cmul(c2[q], xv));
             }
-        for (int d = 0; d < L.dots.n; ++d) {
-            const int sa = L.dots.a[d], sb = L.dots.b[d];
-            const T a = sa == 0 ? v : sa == 1 ? L.dots.e1[i] : sa == 2 ? L.dots.e2[i] : v2;
-            const T b = sb == 0 ? v : sb == 1 ? L.dots.e1[i] : sb == 2 ? L.dots.e2[i] : v2;
-            dot_acc(acc[d], a, b, L.dots.nc[d]);
-        }
+        dots_at(i, v, v2);
         if (L.out) L.out[i] = v;
         if (L.out2) L.out2[i] = v2;
     }
@@ -155,13 +180,19 @@ enum ScalarOp {
     SC_RES_END,           // v0 = r^H r: err, iteration end, rho1 = rho
     SC_BICOR_RHO,         // v0 = b^H Ap
     SC_BICOR_ALPHA,       // v0 = b^H Aq
-    SC_BICOR_OMEGA        // v0 = qc^H pc, v1 = qc^H qc
+    SC_BICOR_OMEGA,       // v0 = qc^H pc, v1 = qc^H qc
+    SC_BICGSTAB_REPLACE   // v0 = b^H r, v1 = r^H r after a residual replacement
 };
 
 __device__ __forceinline__ void ks_fail(KState* s, int status)
 {
     s->status = status;
     s->done = 1;
+}
+__device__ __forceinline__ void ks_fail_at(KState* s, int status, int op)
+{
+    s->fail_op = op;
+    ks_fail(s, status);
 }
 
 // End-of-iteration bookkeeping common to all methods.
@@ -214,25 +245,40 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
         if (!(v[0].x > 0.0) || !isfinite(v[0].x)) ks_fail(s, v[0].x == 0.0 ? KS_BREAKDOWN : KS_NAN);
         break;
     case SC_BICGSTAB_ALPHA:
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->alpha = zdiv(s->rho, v[0]);
         set3(cf + CF_BS_Q, one, zneg(s->alpha), O);  // q = r - alpha Ap
         break;
     case SC_BICGSTAB_OMEGA:
-        if (zbad(v[1])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[1])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->omega = zdiv(v[0], v[1]);
         // over {x, p, q, Aq}: x = x + alpha p + omega q ; r = q - omega Aq
         set8(cf + CF_BS_U, one, s->alpha, s->omega, O, O, O, one, zneg(s->omega));
         break;
     case SC_BICGSTAB_END: {
-        ks_finish(s, sqrt(v[0].x / s->bnrm2), hist);
-        if (s->done) break;
+        // rho_1 = rho and the next p-update coefficients are formed even on
+        // the last iteration, so a residual replacement can resume from here
         s->rho1 = s->rho;
         s->rho = v[1];
-        if (zbad(s->rho1) || zbad(s->omega)) { ks_fail(s, KS_BREAKDOWN); break; }
+        const bool bad = zbad(s->rho1) || zbad(s->omega);
+        if (!bad) {
+            s->beta = zmul(zdiv(s->rho, s->rho1), zdiv(s->alpha, s->omega));
+            // over {r, p, Ap}: p = r + beta p - beta omega Ap
+            set3(cf + CF_BS_P, one, s->beta, zneg(zmul(s->beta, s->omega)));
+        }
+        ks_finish(s, sqrt(v[0].x / s->bnrm2), hist);
+        if (!s->done && bad) ks_fail_at(s, KS_BREAKDOWN, op);
+        break;
+    }
+    case SC_BICGSTAB_REPLACE: {
+        // the recursive residual was replaced by the true one: v0 = b^H r,
+        // v1 = r^H r (reliable update, mixed precision)
+        s->rho = v[0];
+        if (zbad(s->rho1) || zbad(s->omega)) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->beta = zmul(zdiv(s->rho, s->rho1), zdiv(s->alpha, s->omega));
-        // over {r, p, Ap}: p = r + beta p - beta omega Ap
         set3(cf + CF_BS_P, one, s->beta, zneg(zmul(s->beta, s->omega)));
+        s->err = sqrt(v[1].x / s->bnrm2);
+        if (s->err < s->tol) { s->converged = 1; s->done = 1; }
         break;
     }
     case SC_RELRES:
@@ -246,7 +292,7 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
         if (!(v[1].x > 0.0) || !isfinite(v[1].x)) ks_fail(s, v[1].x == 0.0 ? KS_BREAKDOWN : KS_NAN);
         break;
     case SC_CG_ALPHA: {
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         const double a = s->sums[2].x / v[0].x;  // alpha = v1 / v2, v1 = v3
         s->alpha = zc(a, 0);
         s->sums[3] = s->sums[2];
@@ -269,11 +315,11 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
     case SC_BICG_INIT:
         s->sums[2] = v[0];  // vx = r^T r
         s->iter = 0; s->converged = 0; s->status = 0;
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         set8(cf + CF_BI_S, zdiv(one, v[0]), O, O, O, O, O, O, O);  // p = r / v1
         break;
     case SC_BICG_ALPHA:
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->alpha = zdiv(one, v[0]);
         // over {x, p, r, Ap}: x = x + a p ; r = r - a Ap
         set8(cf + CF_BI_U, one, s->alpha, O, O, O, O, one, zneg(s->alpha));
@@ -283,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
         const double err = sqrt((v[0].x * v[0].x + v[0].y * v[0].y) / (vx.x * vx.x + vx.y * vx.y));
         ks_finish(s, err, hist);
         if (s->done) break;
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->beta = zdiv(one, v[0]);
         set8(cf + CF_BI_P, one, s->beta, O, O, O, O, O, O);  // p = p + beta r
         break;
@@ -292,7 +338,7 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
     case SC_CORS_RHO: {
         s->rho = v[0];
         if (s->iter > 0) {
-            if (zbad(s->rho1)) { ks_fail(s, KS_BREAKDOWN); break; }
+            if (zbad(s->rho1)) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
             s->beta = zdiv(s->rho, s->rho1);
         } else {
             s->beta = O;
@@ -305,7 +351,7 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
         break;
     }
     case SC_CORS_ALPHA: {
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->alpha = zdiv(s->rho, v[0]);
         const double2 a = s->alpha, a2 = zmul(a, a);
         // over {p, q, qc, Aq}: pc = p - a q ; rc = qc - a Aq
@@ -324,7 +370,7 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
     case SC_BICOR_RHO: {
         s->rho = v[0];
         if (s->iter > 0) {
-            if (zbad(s->rho1) || zbad(s->omega)) { ks_fail(s, KS_BREAKDOWN); break; }
+            if (zbad(s->rho1) || zbad(s->omega)) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
             s->beta = zmul(zdiv(s->rho, s->rho1), zdiv(s->alpha, s->omega));
         } else {
             s->beta = O;
@@ -337,13 +383,13 @@ __global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2*
         break;
     }
     case SC_BICOR_ALPHA:
-        if (zbad(v[0])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[0])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->alpha = zdiv(s->rho, v[0]);
         // over {r, q, Ap, Aq}: pc = r - a q ; qc = Ap - a Aq
         set8(cf + CF_BC_L2, one, zneg(s->alpha), O, O, O, O, one, zneg(s->alpha));
         break;
     case SC_BICOR_OMEGA:
-        if (zbad(v[1])) { ks_fail(s, KS_BREAKDOWN); break; }
+        if (zbad(v[1])) { ks_fail_at(s, KS_BREAKDOWN, op); break; }
         s->omega = zdiv(v[0], v[1]);
         // over {x, p, pc}: x = x + a p + w pc
         set8(cf + CF_BC_X, one, s->alpha, s->omega, O, O, O, O, O);
@@ -380,6 +426,16 @@ __global__ void k_scale_from64(long n, double s, const double2* in, cx<R>* out)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
         out[t] = mkc((R)(s * in[t].x), (R)(s * in[t].y));
+}
+
+// Let a solve that stopped on its recursive residual continue (after a
+// residual replacement); a breakdown or the iteration limit stays final.
+__global__ void k_state_resume(KState* s)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && s->status == KS_OK && s->iter < s->maxit) {
+        s->done = 0;
+        s->converged = 0;
+    }
 }
 
 // KState initialisation from the host-side tolerance and limit.

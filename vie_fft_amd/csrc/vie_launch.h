@@ -11,23 +11,26 @@ template <class R> struct LinComb;
 // axis lengths with compiled pass kernels
 bool size_supported(int n);
 
+// tw/twist: length-N tables of the LDS-staged passes; twP: exp(-2 pi j e / 2N),
+// e < 2N, of the register-resident passes (all along the pass's own axis).
 template <class R>
 hipError_t run_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
-                    const int* done, hipStream_t st);
+                    const cx<R>* twP, const int* done, hipStream_t st);
 template <class R>
 hipError_t run_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx<R>* tw, const cx<R>* twist,
-                    const int* done, hipStream_t st);
+                    const cx<R>* twP, const int* done, hipStream_t st);
+// twP: exp(-2 pi j e / (2 Nx)), e < 2 Nx (register x pass)
 template <class R>
-hipError_t run_xmix(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const MixTabs<R>& mt,
-                    const int* done, hipStream_t st);
+hipError_t run_xmix(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const cx<R>* twP,
+                    const MixTabs<R>& mt, const int* done, hipStream_t st);
 template <class R>
 hipError_t run_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
-                    const int* done, hipStream_t st);
+                    const cx<R>* twP, const int* done, hipStream_t st);
 // *nblk receives the number of partial sums written per dot product
 template <class R>
-hipError_t run_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, cx<R>* y, const cx<R>* dg,
-                    int dconj, const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw,
-                    const cx<R>* twist, const int* done, hipStream_t st, long* nblk);
+hipError_t run_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, cx<R>* y, const Diag<R>& dg,
+                    const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw,
+                    const cx<R>* twist, const cx<R>* twP, const int* done, hipStream_t st, long* nblk);
 // partial-sum count a zinv launch produces for these sizes
 template <class R> long zinv_blocks(const Geo& g);
 

@@ -1,6 +1,10 @@
 // vie_launch_impl.h -- launcher bodies; included once per precision with
 // VIE_REAL defined as double or float.
 #include "vie_launch.h"
+#include "vie_xpass.h"
+
+#include <algorithm>
+#include <cstdlib>
 
 // Axis lengths (products of 2, 3, 5) with compiled pass kernels.
 #define VIE_SIZES(X) \
@@ -11,99 +15,244 @@ namespace vie {
 
 using R_ = VIE_REAL;
 
-template <>
-hipError_t run_zfwd<R_>(const Geo& g, const Lin<R_>& lin, cx<R_>* W1, const cx<R_>* tw,
-                        const cx<R_>* twist, const int* done, hipStream_t st)
+// Register-resident y/z pass configurations (one component per thread).
+template <class R, int N> struct YZReg { static constexpr bool ok = false; };
+template <> struct YZReg<float, 32> { static constexpr bool ok = true; static constexpr int T = 4; using PL = XPlan<16, 4>; };
+template <> struct YZReg<float, 64> { static constexpr bool ok = true; static constexpr int T = 8; using PL = XPlan<16, 8>; };
+template <> struct YZReg<float, 96> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct YZReg<float, 128> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<16, 16>; };
+template <> struct YZReg<float, 256> { static constexpr bool ok = true; static constexpr int T = 32; using PL = XPlan<16, 16, 2>; };
+template <> struct YZReg<double, 32> { static constexpr bool ok = true; static constexpr int T = 8; using PL = XPlan<8, 8>; };
+template <> struct YZReg<double, 64> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<8, 4, 4>; };
+template <> struct YZReg<double, 96> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct YZReg<double, 128> { static constexpr bool ok = true; static constexpr int T = 32; using PL = XPlan<8, 8, 4>; };
+template <> struct YZReg<double, 256> { static constexpr bool ok = true; static constexpr int T = 64; using PL = XPlan<8, 8, 8>; };
+
+// VIE_PASSES_LDS=1 selects the LDS-staged y/z passes for every size (A/B timing).
+static bool yz_lds_forced()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("VIE_PASSES_LDS");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
+// The z forward pass defaults to the LDS version: its loads of the three
+// Krylov vectors and the fused p update stream at near copy bandwidth there
+// (128^3: 52 us FP32 / 96 us FP64 against 63 / 100 us for the register
+// version).  VIE_ZFWD_REG=1 selects the register version.
+static bool zfwd_reg()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("VIE_ZFWD_REG");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
+template <class R, int n>
+static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
+                              const cx<R>* twP, const int* done, hipStream_t st)
 {
     const long rows = (long)g.nx * g.ny;
-    switch (g.nz) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_z<R_>(n);                                                         \
-        dim3 grid((unsigned)((rows + B - 1) / B), 3);                                         \
-        hipLaunchKernelGGL((k_zfwd<R_, n, B>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done); \
-        break;                                                                                \
+    if constexpr (YZReg<R, n>::ok) {
+        if (zfwd_reg() && !yz_lds_forced()) {
+            using X = YZReg<R, n>;
+            constexpr int B = NT / X::T;
+            dim3 grid((unsigned)((rows + B - 1) / B), 3);
+            hipLaunchKernelGGL((k_zfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, lin, W1, twP, done);
+            return hipGetLastError();
+        }
     }
+    constexpr int B = b_z<R>(n);
+    dim3 grid((unsigned)((rows + B - 1) / B), 3);
+    hipLaunchKernelGGL((k_zfwd<R, n, B>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done);
+    return hipGetLastError();
+}
+
+template <class R, int n>
+static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx<R>* tw, const cx<R>* twist,
+                              const cx<R>* twP, const int* done, hipStream_t st)
+{
+    if constexpr (YZReg<R, n>::ok) {
+        if (!yz_lds_forced()) {
+            using X = YZReg<R, n>;
+            constexpr int B = NT / X::T;
+            const long nzt = (2L * g.nz + B - 1) / B;
+            dim3 grid((unsigned)(g.nx * nzt), 3);
+            hipLaunchKernelGGL((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, W2, twP, done);
+            return hipGetLastError();
+        }
+    }
+    constexpr int B = b_y<R>(n);
+    const long nzt = (2L * g.nz + B - 1) / B;
+    dim3 grid((unsigned)(g.nx * nzt), 3);
+    hipLaunchKernelGGL((k_yfwd<R, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done);
+    return hipGetLastError();
+}
+
+template <class R, int n>
+static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
+                              const cx<R>* twP, const int* done, hipStream_t st)
+{
+    if constexpr (YZReg<R, n>::ok) {
+        if (!yz_lds_forced()) {
+            using X = YZReg<R, n>;
+            constexpr int B = NT / X::T;
+            const long nzt = (2L * g.nz + B - 1) / B;
+            dim3 grid((unsigned)(g.nx * nzt), 3);
+            hipLaunchKernelGGL((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, W1, twP, done);
+            return hipGetLastError();
+        }
+    }
+    constexpr int B = b_y<R>(n);
+    const long nzt = (2L * g.nz + B - 1) / B;
+    dim3 grid((unsigned)(g.nx * nzt), 3);
+    hipLaunchKernelGGL((k_yinv<R, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done);
+    return hipGetLastError();
+}
+
+template <class R, int n>
+static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, cx<R>* y, const Diag<R>& dg,
+                              const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw, const cx<R>* twist,
+                              const cx<R>* twP, const int* done, hipStream_t st, long* nblk)
+{
+    const long rows = (long)g.nx * g.ny;
+    if constexpr (YZReg<R, n>::ok) {
+        if (!yz_lds_forced() && dg.d) {
+            using X = YZReg<R, n>;
+            constexpr int B = NT / X::T;
+            dim3 grid((unsigned)((rows + B - 1) / B), 3);
+            if (nblk) *nblk = (long)grid.x * grid.y;
+            hipLaunchKernelGGL((k_zinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, src, y, dg,
+                               flags, dots, twP, done);
+            return hipGetLastError();
+        }
+    }
+    constexpr int B = b_z<R>(n);
+    dim3 grid((unsigned)((rows + B - 1) / B), 3);
+    if (nblk) *nblk = (long)grid.x * grid.y;
+    hipLaunchKernelGGL((k_zinv<R, n, B>), grid, dim3(NT), 0, st, g, W1, src, y, dg, flags, dots, tw, twist, done);
+    return hipGetLastError();
+}
+
+template <class R, int n> static long zinv_blocks_n(const Geo& g)
+{
+    const long rows = (long)g.nx * g.ny;
+    long r = 3 * ((rows + b_z<R>(n) - 1) / b_z<R>(n));
+    if constexpr (YZReg<R, n>::ok) {
+        constexpr int B = NT / YZReg<R, n>::T;
+        r = std::max(r, 3 * ((rows + B - 1) / B));
+    }
+    return r;
+}
+
+template <>
+hipError_t run_zfwd<R_>(const Geo& g, const Lin<R_>& lin, cx<R_>* W1, const cx<R_>* tw,
+                        const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
+{
+    switch (g.nz) {
+#define CASE(n) \
+    case n: return launch_zfwd<R_, n>(g, lin, W1, tw, twist, twP, done, st);
         VIE_SIZES(CASE)
 #undef CASE
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template <>
 hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>* tw,
-                        const cx<R_>* twist, const int* done, hipStream_t st)
+                        const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
 {
     switch (g.ny) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_y<R_>(n);                                                         \
-        const long nzt = (2L * g.nz + B - 1) / B;                                             \
-        dim3 grid((unsigned)(g.nx * nzt), 3);                                                 \
-        hipLaunchKernelGGL((k_yfwd<R_, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done); \
-        break;                                                                                \
-    }
+#define CASE(n) \
+    case n: return launch_yfwd<R_, n>(g, W1, W2, tw, twist, twP, done, st);
         VIE_SIZES(CASE)
 #undef CASE
     default:
         return hipErrorInvalidValue;
     }
+}
+
+// Register-resident x-pass configurations: T threads per column, radix plan.
+template <class R, int N> struct XReg { static constexpr bool ok = false; };
+template <> struct XReg<float, 32> { static constexpr bool ok = true; static constexpr int T = 8; using PL = XPlan<8, 8>; };
+template <> struct XReg<float, 64> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<8, 4, 4>; };
+template <> struct XReg<float, 96> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct XReg<float, 128> { static constexpr bool ok = true; static constexpr int T = 32; using PL = XPlan<8, 8, 4>; };
+template <> struct XReg<float, 256> { static constexpr bool ok = true; static constexpr int T = 32; using PL = XPlan<16, 16, 2>; };
+template <> struct XReg<double, 32> { static constexpr bool ok = true; static constexpr int T = 8; using PL = XPlan<8, 8>; };
+template <> struct XReg<double, 64> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<8, 4, 4>; };
+template <> struct XReg<double, 96> { static constexpr bool ok = true; static constexpr int T = 16; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct XReg<double, 128> { static constexpr bool ok = true; static constexpr int T = 32; using PL = XPlan<8, 8, 4>; };
+template <> struct XReg<double, 256> { static constexpr bool ok = true; static constexpr int T = 64; using PL = XPlan<8, 8, 8>; };
+
+// VIE_XPASS_LDS=1 selects the LDS-staged x pass for every size (A/B timing).
+static bool xpass_lds_forced()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("VIE_XPASS_LDS");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
+template <class R, int n>
+static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const cx<R>* twP,
+                           const MixTabs<R>& mt, const int* done, hipStream_t st)
+{
+    if constexpr (XReg<R, n>::ok) {
+        if (!xpass_lds_forced()) {
+            using X = XReg<R, n>;
+            constexpr int B = NT / X::T;
+            const long nzt = (2L * g.nz + B - 1) / B;
+            dim3 grid((unsigned)(2L * g.ny * nzt));
+            hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, twP, mt, done);
+            return hipGetLastError();
+        }
+    }
+    constexpr int B = b_x<R>(n);
+    const long nzt = (2L * g.nz + B - 1) / B;
+    dim3 grid((unsigned)(2L * g.ny * nzt));
+    hipLaunchKernelGGL((k_xmix<R, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done);
     return hipGetLastError();
 }
 
 template <>
-hipError_t run_xmix<R_>(const Geo& g, cx<R_>* W2, const cx<R_>* tw, const cx<R_>* twist,
+hipError_t run_xmix<R_>(const Geo& g, cx<R_>* W2, const cx<R_>* tw, const cx<R_>* twist, const cx<R_>* twP,
                         const MixTabs<R_>& mt, const int* done, hipStream_t st)
 {
     switch (g.nx) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_x<R_>(n);                                                         \
-        const long nzt = (2L * g.nz + B - 1) / B;                                             \
-        dim3 grid((unsigned)(2L * g.ny * nzt));                                               \
-        hipLaunchKernelGGL((k_xmix<R_, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done); \
-        break;                                                                                \
-    }
+#define CASE(n) \
+    case n: return launch_x<R_, n>(g, W2, tw, twist, twP, mt, done, st);
         VIE_SIZES(CASE)
 #undef CASE
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template <>
 hipError_t run_yinv<R_>(const Geo& g, const cx<R_>* W2, cx<R_>* W1, const cx<R_>* tw,
-                        const cx<R_>* twist, const int* done, hipStream_t st)
+                        const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
 {
     switch (g.ny) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_y<R_>(n);                                                         \
-        const long nzt = (2L * g.nz + B - 1) / B;                                             \
-        dim3 grid((unsigned)(g.nx * nzt), 3);                                                 \
-        hipLaunchKernelGGL((k_yinv<R_, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done); \
-        break;                                                                                \
-    }
+#define CASE(n) \
+    case n: return launch_yinv<R_, n>(g, W2, W1, tw, twist, twP, done, st);
         VIE_SIZES(CASE)
 #undef CASE
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template <> long zinv_blocks<R_>(const Geo& g)
 {
-    const long rows = (long)g.nx * g.ny;
     switch (g.nz) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_z<R_>(n);                                                         \
-        return 3 * ((rows + B - 1) / B);                                                      \
-    }
+#define CASE(n) \
+    case n: return zinv_blocks_n<R_, n>(g);
         VIE_SIZES(CASE)
 #undef CASE
     default:
@@ -113,27 +262,18 @@ template <> long zinv_blocks<R_>(const Geo& g)
 
 template <>
 hipError_t run_zinv<R_>(const Geo& g, const cx<R_>* W1, const cx<R_>* src, cx<R_>* y,
-                        const cx<R_>* dg, int dconj, const uint8_t* flags, const Dots<R_>& dots,
-                        const cx<R_>* tw, const cx<R_>* twist, const int* done, hipStream_t st,
-                        long* nblk)
+                        const Diag<R_>& dg, const uint8_t* flags, const Dots<R_>& dots,
+                        const cx<R_>* tw, const cx<R_>* twist, const cx<R_>* twP, const int* done,
+                        hipStream_t st, long* nblk)
 {
-    const long rows = (long)g.nx * g.ny;
     switch (g.nz) {
-#define CASE(n)                                                                               \
-    case n: {                                                                                 \
-        constexpr int B = b_z<R_>(n);                                                         \
-        dim3 grid((unsigned)((rows + B - 1) / B), 3);                                         \
-        if (nblk) *nblk = (long)grid.x * grid.y;                                              \
-        hipLaunchKernelGGL((k_zinv<R_, n, B>), grid, dim3(NT), 0, st, g, W1, src, y, dg, dconj, \
-                           flags, dots, tw, twist, done);                                     \
-        break;                                                                                \
-    }
+#define CASE(n) \
+    case n: return launch_zinv<R_, n>(g, W1, src, y, dg, flags, dots, tw, twist, twP, done, st, nblk);
         VIE_SIZES(CASE)
 #undef CASE
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace vie

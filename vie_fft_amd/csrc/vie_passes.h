@@ -36,6 +36,14 @@ template <class R> struct MixTabs {
 
 template <class T> __device__ __forceinline__ T czero() { return mkc(decltype(T().x)(0), decltype(T().x)(0)); }
 
+// Diagonal face term d_f = c_f (chi[i + e_f] + chi[i]) for i_f < N_f - 1
+// (vie_fft/fft.h:405-588), chi = 1/(eps - 1), c_f = -j (delta_f/2)(Z0/k0),
+// precomputed on the host into a 3V array (component-major like the field).
+template <class R> struct Diag {
+    const cx<R>* d;       // per-face coefficients (3V)
+    int conj;             // adjoint: use conj(d_f)
+};
+
 // Block-wide sum of a double2; sh holds NT/64 entries.  Result valid in thread 0.
 __device__ __forceinline__ double2 block_sum(double2 v, double2* sh)
 {
@@ -76,6 +84,29 @@ __device__ __forceinline__ void load_coef(const double2* coef, cx<R>* c)
     }
 }
 
+// Two consecutive complex values per lane: one 16-byte access for FP32,
+// two for FP64 (already 16 bytes each).
+__device__ __forceinline__ void ld2(const float2* p, float2& a, float2& b)
+{
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    a = make_float2(v.x, v.y);
+    b = make_float2(v.z, v.w);
+}
+__device__ __forceinline__ void ld2(const double2* p, double2& a, double2& b)
+{
+    a = p[0];
+    b = p[1];
+}
+__device__ __forceinline__ void st2(float2* p, float2 a, float2 b)
+{
+    *reinterpret_cast<float4*>(p) = make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ void st2(double2* p, double2 a, double2 b)
+{
+    p[0] = a;
+    p[1] = b;
+}
+
 // ------------------------------------------------------------------ zfwd
 // grid (ceil(Nx*Ny / B), 3 components); N == Nz.
 template <class R, int N, int B>
@@ -99,31 +130,87 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     T c[3];
     load_coef<R>(lin.coef, c);
     __syncthreads();
-    for (int e = threadIdx.x; e < B * N; e += NT) {
-        const int col = e / N, n = e % N;
-        const long row = row0 + col;
-        T v = czero<T>();
-        if (row < rows) {
-            const long idx = s * g.V + row * N + n;
-            v = cmul(c[0], lin.x[0][idx]);
-            if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
-            if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
-            if (lin.out) lin.out[idx] = v;
-            const bool drop = (s == 2 && n == N - 1) || (s == 0 && row >= rows - g.ny) ||
-                              (s == 1 && (row % g.ny) == g.ny - 1);
-            if (drop) v = czero<T>();
+    if constexpr (N % 2 == 0) {
+        // all loads first, then the combination, stores and LDS writes
+        constexpr int IT = (B * N / 2 + NT - 1) / NT;
+        T x0[IT][2], x1[IT][2], x2[IT][2];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int col = e / (N / 2), n = 2 * (e % (N / 2));
+            const long row = row0 + col;
+            if (e < B * N / 2 && row < rows) {
+                const long idx = s * g.V + row * N + n;
+                ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
+                if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
+                if (lin.x[2]) ld2(lin.x[2] + idx, x2[it][0], x2[it][1]);
+            }
         }
-        buf[n * LD + col] = v;
-        buf[n * LD + B + col] = cmul(v, tws[n]);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e >= B * N / 2) break;
+            const int col = e / (N / 2), n = 2 * (e % (N / 2));
+            const long row = row0 + col;
+            T v0 = czero<T>(), v1 = czero<T>();
+            if (row < rows) {
+                const long idx = s * g.V + row * N + n;
+                v0 = cmul(c[0], x0[it][0]);
+                v1 = cmul(c[0], x0[it][1]);
+                if (lin.x[1]) {
+                    v0 = cadd(v0, cmul(c[1], x1[it][0]));
+                    v1 = cadd(v1, cmul(c[1], x1[it][1]));
+                }
+                if (lin.x[2]) {
+                    v0 = cadd(v0, cmul(c[2], x2[it][0]));
+                    v1 = cadd(v1, cmul(c[2], x2[it][1]));
+                }
+                if (lin.out) st2(lin.out + idx, v0, v1);
+                const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
+                if (dropall) v0 = v1 = czero<T>();
+                if (s == 2 && n + 1 == N - 1) v1 = czero<T>();
+            }
+            buf[n * LD + col] = v0;
+            buf[n * LD + B + col] = cmul(v0, tws[n]);
+            buf[(n + 1) * LD + col] = v1;
+            buf[(n + 1) * LD + B + col] = cmul(v1, tws[n + 1]);
+        }
+    } else {
+        for (int e = threadIdx.x; e < B * N; e += NT) {
+            const int col = e / N, n = e % N;
+            const long row = row0 + col;
+            T v = czero<T>();
+            if (row < rows) {
+                const long idx = s * g.V + row * N + n;
+                v = cmul(c[0], lin.x[0][idx]);
+                if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
+                if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
+                if (lin.out) lin.out[idx] = v;
+                const bool drop = (s == 2 && n == N - 1) || (s == 0 && row >= rows - g.ny) ||
+                                  (s == 1 && (row % g.ny) == g.ny - 1);
+                if (drop) v = czero<T>();
+            }
+            buf[n * LD + col] = v;
+            buf[n * LD + B + col] = cmul(v, tws[n]);
+        }
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, -1>(buf, tw);
-    for (int e = threadIdx.x; e < B * 2 * N; e += NT) {
-        const int col = e / (2 * N), t = e % (2 * N);
+    // t and t+1 share the half e when N is even (pairs never straddle N)
+    for (int e = threadIdx.x; e < B * N; e += NT) {
+        const int col = e / N, t = 2 * (e % N);
         const long row = row0 + col;
         if (row < rows) {
             const int ep = t >= N, m = t - ep * N;
-            W1[(s * rows + row) * (2 * N) + t] = buf[m * LD + ep * B + col];
+            T* dst = W1 + (s * rows + row) * (2 * N) + t;
+            if constexpr (N % 2 == 0)
+                st2(dst, buf[m * LD + ep * B + col], buf[(m + 1) * LD + ep * B + col]);
+            else {
+                // odd N: t+1 may fall in the other half
+                dst[0] = buf[m * LD + ep * B + col];
+                const int t1 = t + 1, ep1 = t1 >= N, m1 = t1 - ep1 * N;
+                dst[1] = buf[m1 * LD + ep1 * B + col];
+            }
         }
     }
 }
@@ -138,6 +225,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
 {
     using T = cx<R>;
     constexpr int COLS = 2 * B, LD = COLS;
+    static_assert(B % 2 == 0, "column pairs");
     __shared__ T buf[N * LD];
     __shared__ T tw[N];
     __shared__ T tws[N];
@@ -153,20 +241,24 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
     }
     __syncthreads();
     const long in0 = ((long)s * g.nx + i) * N * P2 + z0;
-    for (int e = threadIdx.x; e < N * B; e += NT) {
-        const int j = e / B, col = e % B;
-        T v = (z0 + col < P2) ? W1[in0 + (long)j * P2 + col] : czero<T>();
-        buf[j * LD + col] = v;
-        buf[j * LD + B + col] = cmul(v, tws[j]);
+    // column pairs (col, col+1): P2 and z0 are even, so a pair is in or out together
+    for (int e = threadIdx.x; e < N * B / 2; e += NT) {
+        const int j = e / (B / 2), col = 2 * (e % (B / 2));
+        T v0 = czero<T>(), v1 = czero<T>();
+        if (z0 + col < P2) ld2(W1 + in0 + (long)j * P2 + col, v0, v1);
+        buf[j * LD + col] = v0;
+        buf[j * LD + col + 1] = v1;
+        buf[j * LD + B + col] = cmul(v0, tws[j]);
+        buf[j * LD + B + col + 1] = cmul(v1, tws[j]);
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, -1>(buf, tw);
     const long out0 = ((long)s * g.nx + i) * (2 * N) * P2 + z0;
-    for (int e = threadIdx.x; e < 2 * N * B; e += NT) {
-        const int t = e / B, col = e % B;
+    for (int e = threadIdx.x; e < N * B; e += NT) {
+        const int t = e / (B / 2), col = 2 * (e % (B / 2));
         if (z0 + col < P2) {
             const int ep = t >= N, m = t - ep * N;
-            W2[out0 + (long)t * P2 + col] = buf[m * LD + ep * B + col];
+            st2(W2 + out0 + (long)t * P2 + col, buf[m * LD + ep * B + col], buf[m * LD + ep * B + col + 1]);
         }
     }
 }
@@ -264,6 +356,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
 {
     using T = cx<R>;
     constexpr int COLS = 2 * B, LD = COLS;
+    static_assert(B % 2 == 0, "column pairs");
     __shared__ T buf[N * LD];
     __shared__ T tw[N];
     __shared__ T tws[N];
@@ -278,18 +371,22 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
         tws[t] = twistg[t];
     }
     const long in0 = ((long)s * g.nx + i) * (2 * N) * P2 + z0;
-    for (int e = threadIdx.x; e < 2 * N * B; e += NT) {
-        const int t = e / B, col = e % B;
+    for (int e = threadIdx.x; e < N * B; e += NT) {
+        const int t = e / (B / 2), col = 2 * (e % (B / 2));
         const int ep = t >= N, m = t - ep * N;
-        buf[m * LD + ep * B + col] = (z0 + col < P2) ? W2[in0 + (long)t * P2 + col] : czero<T>();
+        T v0 = czero<T>(), v1 = czero<T>();
+        if (z0 + col < P2) ld2(W2 + in0 + (long)t * P2 + col, v0, v1);
+        buf[m * LD + ep * B + col] = v0;
+        buf[m * LD + ep * B + col + 1] = v1;
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
     const long out0 = ((long)s * g.nx + i) * N * P2 + z0;
-    for (int e = threadIdx.x; e < N * B; e += NT) {
-        const int j = e / B, col = e % B;
+    for (int e = threadIdx.x; e < N * B / 2; e += NT) {
+        const int j = e / (B / 2), col = 2 * (e % (B / 2));
         if (z0 + col < P2)
-            W1[out0 + (long)j * P2 + col] = cadd(buf[j * LD + col], cmulc(buf[j * LD + B + col], tws[j]));
+            st2(W1 + out0 + (long)j * P2 + col, cadd(buf[j * LD + col], cmulc(buf[j * LD + B + col], tws[j])),
+                cadd(buf[j * LD + col + 1], cmulc(buf[j * LD + B + col + 1], tws[j])));
     }
 }
 
@@ -298,8 +395,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
 template <class R, int N, int B>
 __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1,
                                              const cx<R>* __restrict__ src, cx<R>* __restrict__ y,
-                                             const cx<R>* __restrict__ dg, int dconj,
-                                             const uint8_t* __restrict__ flags, Dots<R> dots,
+                                             Diag<R> dg, const uint8_t* __restrict__ flags, Dots<R> dots,
                                              const cx<R>* __restrict__ twg,
                                              const cx<R>* __restrict__ twistg, const int* done)
 {
@@ -317,31 +413,70 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
-    for (int e = threadIdx.x; e < B * 2 * N; e += NT) {
-        const int col = e / (2 * N), t = e % (2 * N);
+    auto dval = [&](long idx) -> T { return dg.conj ? cconj(dg.d[idx]) : dg.d[idx]; };
+    for (int e = threadIdx.x; e < B * N; e += NT) {
+        const int col = e / N, t = 2 * (e % N);
         const long row = row0 + col;
+        T v0 = czero<T>(), v1 = czero<T>();
+        if (row < rows) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
         const int ep = t >= N, m = t - ep * N;
-        buf[m * LD + ep * B + col] = row < rows ? W1[(f * rows + row) * (2 * N) + t] : czero<T>();
+        buf[m * LD + ep * B + col] = v0;
+        const int t1 = t + 1, ep1 = t1 >= N, m1 = t1 - ep1 * N;
+        buf[m1 * LD + ep1 * B + col] = v1;
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    for (int e = threadIdx.x; e < B * N; e += NT) {
-        const int col = e / N, k = e % N;
-        const long row = row0 + col;
-        if (row < rows) {
-            const long idx = f * g.V + row * N + k;
-            const T out = cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k]));
-            T yv = czero<T>();
-            if (flags[idx]) {
-                const T dv = dconj ? cconj(dg[idx]) : dg[idx];
-                yv = cadd(out, cmul(dv, src[idx]));
+    auto epi = [&](long idx, T out, T sv, T dv, uint8_t fl, T& yv) {
+        yv = czero<T>();
+        if (fl) yv = cadd(out, cmul(dv, sv));
+    };
+    auto dot1 = [&](long idx, T yv) {
+        for (int d = 0; d < dots.n; ++d) {
+            const T a = dots.a[d] == 0 ? yv : dots.a[d] == 1 ? dots.e1[idx] : dots.e2[idx];
+            const T b = dots.b[d] == 0 ? yv : dots.b[d] == 1 ? dots.e1[idx] : dots.e2[idx];
+            dot_acc(acc[d], a, b, dots.nc[d]);
+        }
+    };
+    if constexpr (N % 2 == 0) {
+        for (int e = threadIdx.x; e < B * N / 2; e += NT) {
+            const int col = e / (N / 2), k = 2 * (e % (N / 2));
+            const long row = row0 + col;
+            if (row < rows) {
+                const long idx = f * g.V + row * N + k;
+                const T o0 = cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k]));
+                const T o1 = cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1]));
+                T s0, s1, d0, d1, y0, y1;
+                ld2(src + idx, s0, s1);
+                if (dg.d) {
+                    ld2(dg.d + idx, d0, d1);
+                    if (dg.conj) {
+                        d0 = cconj(d0);
+                        d1 = cconj(d1);
+                    }
+                } else {
+                    const long vox = row * N + k;
+                    d0 = dval(idx);
+                    d1 = dval(idx + 1);
+                }
+                epi(idx, o0, s0, d0, flags[idx], y0);
+                epi(idx + 1, o1, s1, d1, flags[idx + 1], y1);
+                st2(y + idx, y0, y1);
+                dot1(idx, y0);
+                dot1(idx + 1, y1);
             }
-            y[idx] = yv;
-            for (int d = 0; d < dots.n; ++d) {
-                const T a = dots.a[d] == 0 ? yv : dots.a[d] == 1 ? dots.e1[idx] : dots.e2[idx];
-                const T b = dots.b[d] == 0 ? yv : dots.b[d] == 1 ? dots.e1[idx] : dots.e2[idx];
-                dot_acc(acc[d], a, b, dots.nc[d]);
+        }
+    } else {
+        for (int e = threadIdx.x; e < B * N; e += NT) {
+            const int col = e / N, k = e % N;
+            const long row = row0 + col;
+            if (row < rows) {
+                const long idx = f * g.V + row * N + k;
+                const T out = cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k]));
+                T yv;
+                epi(idx, out, src[idx], dval(idx), flags[idx], yv);
+                y[idx] = yv;
+                dot1(idx, yv);
             }
         }
     }

@@ -1,0 +1,532 @@
+// vie_xpass.h -- register-resident x pass: forward transform along x, the
+// 3x3 spectral product with the Green's function, inverse transform and
+// truncation, for B k-columns of the three components at once.
+//
+// Each column is one full padded transform of length P = 2N (input zero on
+// [N, P), only rows n < N kept on output) spread over T threads that hold
+// E = P/T values each in registers.  A Stockham stage with radix r runs E/r
+// butterflies per thread (butterfly i = t + T b reads positions i + q P/r and
+// writes (i - i mod S) r + i mod S + q S, S the product of earlier radices);
+// stages exchange through LDS, one component at a time.  After the last
+// forward stage thread t holds frequencies k = i + q P/r, applies the product
+// there, and runs the inverse as the reverse dataflow of the forward (conjugate
+// DFT, then conjugate twiddle, read and written at swapped positions), so every
+// thread ends with the rows n = i + q P/r1 it loaded.  This replaces the
+// L_DC_Dq / L_Dq spectral loops and the x-direction part of their transforms
+// (vie_fft/fft.h:41-397, paths relative to the reference root).
+#pragma once
+
+#include "vie_passes.h"
+
+namespace vie {
+
+template <int... Rs> struct XPlan {
+    static constexpr int n = sizeof...(Rs);
+    static constexpr int r[n] = {Rs...};
+    static constexpr int stride(int s)
+    {
+        int v = 1;
+        for (int u = 0; u < s; ++u) v *= r[u];
+        return v;
+    }
+};
+
+template <class R, int N, int T, class PL> struct XCfg {
+    static constexpr int P = 2 * N;
+    static constexpr int E = P / T;
+    static constexpr int B = NT / T;     // columns per workgroup
+    static constexpr int LD = B + 1;     // padded LDS row (bank spread)
+    static_assert(P % T == 0, "T must divide P");
+    static_assert(B * T == NT, "T must divide the workgroup");
+};
+
+// forward stage s on one component's registers
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xf_stage(T_* v, const T_* tw, int t)
+{
+    constexpr int r = PL::r[s], S = PL::stride(s), P = C::P, E = C::E, T = P / E;
+#pragma unroll
+    for (int b = 0; b < E / r; ++b) {
+        const int i = t + T * b;
+        const int k = i % S;
+        T_* a = v + b * r;
+        if (S > 1) {
+#pragma unroll
+            for (int q = 1; q < r; ++q) a[q] = cmul(a[q], tw[q * k * (P / (S * r))]);
+        }
+        dft_impl<r, -1, T_>::run(a);
+    }
+}
+
+// inverse stage s (reverse dataflow of xf_stage)
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xi_stage(T_* v, const T_* tw, int t)
+{
+    constexpr int r = PL::r[s], S = PL::stride(s), P = C::P, E = C::E, T = P / E;
+#pragma unroll
+    for (int b = 0; b < E / r; ++b) {
+        const int i = t + T * b;
+        const int k = i % S;
+        T_* a = v + b * r;
+        dft_impl<r, +1, T_>::run(a);
+        if (S > 1) {
+#pragma unroll
+            for (int q = 1; q < r; ++q) a[q] = cmulc(a[q], tw[q * k * (P / (S * r))]);
+        }
+    }
+}
+
+// exchange after forward stage s: write its outputs, read stage s+1 inputs
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
+{
+    constexpr int r = PL::r[s], S = PL::stride(s), r2 = PL::r[s + 1], P = C::P, E = C::E, T = P / E,
+                  LD = C::LD;
+#pragma unroll
+    for (int b = 0; b < E / r; ++b) {
+        const int i = t + T * b, k = i % S, j = (i - k) * r + k;
+#pragma unroll
+        for (int q = 0; q < r; ++q) lds[(j + q * S) * LD + col] = v[b * r + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < E / r2; ++b) {
+        const int i = t + T * b;
+#pragma unroll
+        for (int q = 0; q < r2; ++q) v[b * r2 + q] = lds[(i + q * (P / r2)) * LD + col];
+    }
+    __syncthreads();
+}
+
+// exchange after inverse stage s (s > 0): write at stage-s input positions,
+// read at stage s-1 output positions
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
+{
+    constexpr int r = PL::r[s], rp = PL::r[s - 1], Sp = PL::stride(s - 1), P = C::P, E = C::E,
+                  T = P / E, LD = C::LD;
+#pragma unroll
+    for (int b = 0; b < E / r; ++b) {
+        const int i = t + T * b;
+#pragma unroll
+        for (int q = 0; q < r; ++q) lds[(i + q * (P / r)) * LD + col] = v[b * r + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < E / rp; ++b) {
+        const int i = t + T * b, k = i % Sp, j = (i - k) * rp + k;
+#pragma unroll
+        for (int q = 0; q < rp; ++q) v[b * rp + q] = lds[(j + q * Sp) * LD + col];
+    }
+    __syncthreads();
+}
+
+template <class T_, class C, class PL, int s = 0>
+__device__ __forceinline__ void x_forward(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
+{
+    if constexpr (s < PL::n) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xf_stage<T_, C, PL, s>(v[c], tw, t);
+            if constexpr (s + 1 < PL::n) xf_xchg<T_, C, PL, s>(v[c], lds, t, col);
+        }
+        x_forward<T_, C, PL, s + 1>(v, lds, tw, t, col);
+    }
+}
+
+template <class T_, class C, class PL, int s = PL::n - 1>
+__device__ __forceinline__ void x_inverse(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
+{
+    if constexpr (s >= 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xi_stage<T_, C, PL, s>(v[c], tw, t);
+            if constexpr (s > 0) xi_xchg<T_, C, PL, s>(v[c], lds, t, col);
+        }
+        x_inverse<T_, C, PL, s - 1>(v, lds, tw, t, col);
+    }
+}
+
+// Parity-order position of min(k, 2n - k) for the split-order index t of an
+// axis of length n (frequency k = 2 (t mod n) + t / n): evens 0, 2, .., then odds.
+__device__ __forceinline__ int fold_split(int t, int n)
+{
+    const int ep = t >= n ? 1 : 0;
+    const int k = 2 * (t - ep * n) + ep;
+    const int kf = k < 2 * n - k ? k : 2 * n - k;
+    return (kf & 1) ? (n / 2 + 1) + (kf >> 1) : (kf >> 1);
+}
+
+// grid (2Ny * ceil(2Nz / B)); N == Nx; in place on W2.
+// twP[e] = exp(-2 pi j e / P), e < P.
+template <class R, int N, int T, class PL>
+__global__ __launch_bounds__(NT) void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP,
+                                             MixTabs<R> mt, const int* done)
+{
+    using T_ = cx<R>;
+    using C = XCfg<R, N, T, PL>;
+    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
+    __shared__ T_ lds[P * LD];
+    __shared__ T_ tw[P];
+    __shared__ T_ uv[2 * P];          // U and Vv of the x axis by frequency
+    __shared__ T_ gsl[(N + 1) * B];  // G octant slice [fx][col] of this tile
+    if (done && *done) return;
+    const int P1 = 2 * g.ny, P2 = 2 * g.nz;
+    const int nzt = (P2 + B - 1) / B;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / nzt, z0 = (tile % nzt) * B;
+    const int col = threadIdx.x % B, t = threadIdx.x / B;
+    const bool colok = z0 + col < P2;
+    const long rowst = (long)P1 * P2;  // stride between x rows
+    T_* base = W2 + (long)ty * P2 + z0 + col;
+    const long cst = (long)g.nx * rowst;  // stride between components
+
+    // issue the pencil loads first, then the tables, so their latencies overlap
+    T_ v[3][E];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < E / r1; ++b) {
+            const int i = t + (P / E) * b;
+#pragma unroll
+            for (int q = 0; q < r1; ++q) {
+                const int n = i + q * (P / r1);
+                v[c][b * r1 + q] = (n < N && colok) ? base[c * cst + n * rowst] : czero<T_>();
+            }
+        }
+    const long gs = (long)(g.ny + 1) * (g.nz + 1);
+    const long gy = (long)fold_split(ty, g.ny) * (g.nz + 1);
+    for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
+        const int fxe = e / B, ce = e % B;
+        const int tz = z0 + ce < P2 ? z0 + ce : z0;
+        T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tz, g.nz)];
+        gsl[e] = mt.gconj ? cconj(gv) : gv;
+    }
+    for (int e = threadIdx.x; e < P; e += NT) {
+        tw[e] = twP[e];
+        const int ts = (e & 1) * N + (e >> 1);  // split-order index of frequency e
+        uv[e] = mt.U[0][ts];
+        uv[P + e] = mt.Vv[0][ts];
+    }
+    const T_ uy = mt.U[1][ty], vy = mt.Vv[1][ty];
+    const int tz = colok ? z0 + col : z0;
+    const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
+    __syncthreads();
+    x_forward<T_, C, PL>(v, lds, tw, t, col);
+
+    const T_ D0 = mkc(mt.D[0][0], mt.D[0][1]);
+    const T_ D1 = mkc(mt.D[1][0], mt.D[1][1]);
+    const T_ D2 = mkc(mt.D[2][0], mt.D[2][1]);
+    // last forward stage: register e = b rL + q holds frequency i + q P/rL, i = t + T b
+#pragma unroll
+    for (int b = 0; b < E / rL; ++b) {
+#pragma unroll
+        for (int q = 0; q < rL; ++q) {
+            const int kx = t + (P / E) * b + q * (P / rL);
+            const int kf = kx < P - kx ? kx : P - kx;
+            const int fx = (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
+            const T_ Gv = gsl[fx * B + col];
+            const int e = b * rL + q;
+            const T_ X0 = v[0][e], X1 = v[1][e], X2 = v[2][e];
+            const T_ ux = uv[kx], vx = uv[P + kx];
+            const T_ sig = cadd(cadd(cmul(vx, X0), cmul(vy, X1)), cmul(vz, X2));
+            v[0][e] = cmul(Gv, cadd(cmul(D0, X0), cmul(ux, sig)));
+            v[1][e] = cmul(Gv, cadd(cmul(D1, X1), cmul(uy, sig)));
+            v[2][e] = cmul(Gv, cadd(cmul(D2, X2), cmul(uz, sig)));
+        }
+    }
+    x_inverse<T_, C, PL>(v, lds, tw, t, col);
+    if (colok) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < E / r1; ++b) {
+                const int i = t + (P / E) * b;
+#pragma unroll
+                for (int q = 0; q < r1; ++q) {
+                    const int n = i + q * (P / r1);
+                    if (n < N) base[c * cst + n * rowst] = v[c][b * r1 + q];
+                }
+            }
+    }
+}
+
+}  // namespace vie
+
+namespace vie {
+
+// ---------------------------------------------------------------------------
+// Register-resident y and z passes (one component per thread).  Same engine as
+// the x pass: a full padded transform of length P = 2N over T threads holding
+// E = P/T values each.  Spectral data in HBM keeps the split order of the
+// LDS-staged passes (t = (k mod 2) N + k / 2), so any pass variant can follow
+// any other; the conversion is folded into the addressing.
+// ---------------------------------------------------------------------------
+
+template <class T_, class C, class PL, int s = 0>
+__device__ __forceinline__ void r_forward(T_* v, T_* lds, const T_* tw, int t, int col)
+{
+    if constexpr (s < PL::n) {
+        xf_stage<T_, C, PL, s>(v, tw, t);
+        if constexpr (s + 1 < PL::n) xf_xchg<T_, C, PL, s>(v, lds, t, col);
+        r_forward<T_, C, PL, s + 1>(v, lds, tw, t, col);
+    }
+}
+
+template <class T_, class C, class PL, int s = PL::n - 1>
+__device__ __forceinline__ void r_inverse(T_* v, T_* lds, const T_* tw, int t, int col)
+{
+    if constexpr (s >= 0) {
+        xi_stage<T_, C, PL, s>(v, tw, t);
+        if constexpr (s > 0) xi_xchg<T_, C, PL, s>(v, lds, t, col);
+        r_inverse<T_, C, PL, s - 1>(v, lds, tw, t, col);
+    }
+}
+
+template <int N> __device__ __forceinline__ int split_of(int k) { return (k & 1) * N + (k >> 1); }
+
+// z forward: rows of Nz cells (optionally the Lin combination) -> W1 rows of
+// 2Nz in split order.  grid (ceil(Nx*Ny / B), 3); N == Nz.
+template <class R, int N, int T, class PL>
+__global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restrict__ W1,
+                                               const cx<R>* __restrict__ twP, const int* done)
+{
+    using T_ = cx<R>;
+    using C = XCfg<R, N, T, PL>;
+    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
+    __shared__ T_ lds[P * LD];
+    __shared__ T_ tw[P];
+    if (done && *done) return;
+    const int s = blockIdx.y;
+    const long rows = (long)g.nx * g.ny;
+    const int col = threadIdx.x / T, t = threadIdx.x % T;  // consecutive lanes: consecutive cells
+    const long row = (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = row < rows;
+    T_ c[3];
+    load_coef<R>(lin.coef, c);
+    const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
+    const long base = s * g.V + row * N;
+    // all loads before any store (lin.out may alias an input)
+    T_ v[E], w1[E], w2[E];
+#pragma unroll
+    for (int b = 0; b < E / r1; ++b) {
+        const int i = t + T * b;
+#pragma unroll
+        for (int q = 0; q < r1; ++q) {
+            const int n = i + q * (P / r1), e = b * r1 + q;
+            v[e] = w1[e] = w2[e] = czero<T_>();
+            if (n < N && rowok) {
+                v[e] = lin.x[0][base + n];
+                if (lin.x[1]) w1[e] = lin.x[1][base + n];
+                if (lin.x[2]) w2[e] = lin.x[2][base + n];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < E / r1; ++b) {
+        const int i = t + T * b;
+#pragma unroll
+        for (int q = 0; q < r1; ++q) {
+            const int n = i + q * (P / r1), e = b * r1 + q;
+            T_ z = cadd(cadd(cmul(c[0], v[e]), cmul(c[1], w1[e])), cmul(c[2], w2[e]));
+            if (n < N && rowok) {
+                if (lin.out) lin.out[base + n] = z;
+                if (dropall || (s == 2 && n == N - 1)) z = czero<T_>();
+            } else {
+                z = czero<T_>();
+            }
+            v[e] = z;
+        }
+    }
+    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
+    __syncthreads();
+    r_forward<T_, C, PL>(v, lds, tw, t, col);
+    if (rowok) {
+        T_* out = W1 + (s * rows + row) * (2 * N);
+#pragma unroll
+        for (int b = 0; b < E / rL; ++b)
+#pragma unroll
+            for (int q = 0; q < rL; ++q) out[split_of<N>(t + T * b + q * (P / rL))] = v[b * rL + q];
+    }
+}
+
+// z inverse: W1 rows (split order) -> y = mask (trunc(IDFT) + d x), dots.
+// grid (ceil(Nx*Ny / B), 3); N == Nz.
+template <class R, int N, int T, class PL>
+__global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ W1, const cx<R>* __restrict__ src,
+                                               cx<R>* __restrict__ y, Diag<R> dg, const uint8_t* __restrict__ flags,
+                                               Dots<R> dots, const cx<R>* __restrict__ twP, const int* done)
+{
+    using T_ = cx<R>;
+    using C = XCfg<R, N, T, PL>;
+    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
+    __shared__ T_ lds[P * LD];
+    __shared__ T_ tw[P];
+    __shared__ double2 red[NT / 64];
+    if (done && *done) return;
+    const int f = blockIdx.y;
+    const long rows = (long)g.nx * g.ny;
+    const int col = threadIdx.x / T, t = threadIdx.x % T;
+    const long row = (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = row < rows;
+    T_ v[E];
+    {
+        const T_* in = W1 + (f * rows + (rowok ? row : 0)) * (2 * N);
+#pragma unroll
+        for (int b = 0; b < E / rL; ++b)
+#pragma unroll
+            for (int q = 0; q < rL; ++q)
+                v[b * rL + q] = rowok ? in[split_of<N>(t + T * b + q * (P / rL))] : czero<T_>();
+    }
+    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
+    __syncthreads();
+    r_inverse<T_, C, PL>(v, lds, tw, t, col);
+    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    if (rowok) {
+        const long base = f * g.V + row * N;
+        // epilogue operands of the kept rows n < N: all loads first
+        T_ sv[E], dv[E], e1v[E], e2v[E];
+        uint8_t fl[E];
+        const bool need1 = dots.n > 0 && (dots.a[0] == 1 || dots.b[0] == 1 || (dots.n > 1 && (dots.a[1] == 1 || dots.b[1] == 1)) ||
+                                          (dots.n > 2 && (dots.a[2] == 1 || dots.b[2] == 1)));
+        const bool need2 = dots.n > 0 && (dots.a[0] == 2 || dots.b[0] == 2 || (dots.n > 1 && (dots.a[1] == 2 || dots.b[1] == 2)) ||
+                                          (dots.n > 2 && (dots.a[2] == 2 || dots.b[2] == 2)));
+        const bool e1src = dots.e1 == src;
+#pragma unroll
+        for (int b = 0; b < E / r1; ++b) {
+            const int i = t + T * b;
+#pragma unroll
+            for (int q = 0; q < r1; ++q) {
+                const int n = i + q * (P / r1), e = b * r1 + q;
+                if (n < N) {
+                    const long idx = base + n;
+                    sv[e] = src[idx];
+                    dv[e] = dg.d[idx];
+                    fl[e] = flags[idx];
+                    if (need1 && !e1src) e1v[e] = dots.e1[idx];
+                    if (need2) e2v[e] = dots.e2[idx];
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < E / r1; ++b) {
+            const int i = t + T * b;
+#pragma unroll
+            for (int q = 0; q < r1; ++q) {
+                const int n = i + q * (P / r1), e = b * r1 + q;
+                if (n < N) {
+                    const long idx = base + n;
+                    T_ yv = czero<T_>();
+                    if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
+                    y[idx] = yv;
+                    const T_ o1 = e1src ? sv[e] : e1v[e];
+                    for (int d = 0; d < dots.n; ++d) {
+                        const T_ a = dots.a[d] == 0 ? yv : dots.a[d] == 1 ? o1 : e2v[e];
+                        const T_ bb = dots.b[d] == 0 ? yv : dots.b[d] == 1 ? o1 : e2v[e];
+                        dot_acc(acc[d], a, bb, dots.nc[d]);
+                    }
+                }
+            }
+        }
+    }
+    if (dots.n > 0) {
+        const long nblk = (long)gridDim.x * gridDim.y;
+        const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        for (int d = 0; d < dots.n; ++d) {
+            double2 sm = block_sum(acc[d], red);
+            if (threadIdx.x == 0) dots.partial[d * nblk + blk] = sm;
+        }
+    }
+}
+
+// y forward: W1 [Nx][Ny][2Nz] -> W2 [Nx][2Ny][2Nz], B k-columns per workgroup.
+// grid (Nx * ceil(2Nz / B), 3); N == Ny.
+template <class R, int N, int T, class PL>
+__global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ W1, cx<R>* __restrict__ W2,
+                                               const cx<R>* __restrict__ twP, const int* done)
+{
+    using T_ = cx<R>;
+    using C = XCfg<R, N, T, PL>;
+    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
+    __shared__ T_ lds[P * LD];
+    __shared__ T_ tw[P];
+    if (done && *done) return;
+    const int s = blockIdx.y;
+    const int P2 = 2 * g.nz;
+    const int nzt = (P2 + B - 1) / B;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int col = threadIdx.x % B, t = threadIdx.x / B;
+    const bool colok = z0 + col < P2;
+    const T_* in = W1 + ((long)s * g.nx + i) * N * P2 + z0 + col;
+    T_ v[E];
+#pragma unroll
+    for (int b = 0; b < E / r1; ++b) {
+        const int ii = t + T * b;
+#pragma unroll
+        for (int q = 0; q < r1; ++q) {
+            const int n = ii + q * (P / r1);
+            v[b * r1 + q] = (n < N && colok) ? in[(long)n * P2] : czero<T_>();
+        }
+    }
+    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
+    __syncthreads();
+    r_forward<T_, C, PL>(v, lds, tw, t, col);
+    if (colok) {
+        T_* out = W2 + ((long)s * g.nx + i) * (2 * N) * P2 + z0 + col;
+#pragma unroll
+        for (int b = 0; b < E / rL; ++b)
+#pragma unroll
+            for (int q = 0; q < rL; ++q)
+                out[(long)split_of<N>(t + T * b + q * (P / rL)) * P2] = v[b * rL + q];
+    }
+}
+
+// y inverse: W2 -> W1 rows j < Ny.  grid (Nx * ceil(2Nz / B), 3); N == Ny.
+template <class R, int N, int T, class PL>
+__global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ W2, cx<R>* __restrict__ W1,
+                                               const cx<R>* __restrict__ twP, const int* done)
+{
+    using T_ = cx<R>;
+    using C = XCfg<R, N, T, PL>;
+    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
+    __shared__ T_ lds[P * LD];
+    __shared__ T_ tw[P];
+    if (done && *done) return;
+    const int s = blockIdx.y;
+    const int P2 = 2 * g.nz;
+    const int nzt = (P2 + B - 1) / B;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int col = threadIdx.x % B, t = threadIdx.x / B;
+    const bool colok = z0 + col < P2;
+    const T_* in = W2 + ((long)s * g.nx + i) * (2 * N) * P2 + z0 + col;
+    T_ v[E];
+#pragma unroll
+    for (int b = 0; b < E / rL; ++b)
+#pragma unroll
+        for (int q = 0; q < rL; ++q)
+            v[b * rL + q] = colok ? in[(long)split_of<N>(t + T * b + q * (P / rL)) * P2] : czero<T_>();
+    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
+    __syncthreads();
+    r_inverse<T_, C, PL>(v, lds, tw, t, col);
+    if (colok) {
+        T_* out = W1 + ((long)s * g.nx + i) * N * P2 + z0 + col;
+#pragma unroll
+        for (int b = 0; b < E / r1; ++b) {
+            const int ii = t + T * b;
+#pragma unroll
+            for (int q = 0; q < r1; ++q) {
+                const int n = ii + q * (P / r1);
+                if (n < N) out[(long)n * P2] = v[b * r1 + q];
+            }
+        }
+    }
+}
+
+}  // namespace vie

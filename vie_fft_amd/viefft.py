@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libviefft.so")
 
 FP64, FP32 = 0, 1
+MIXED_REFINE, MIXED_RELIABLE = 0, 1
 CG, BICG, BICGSTAB, CORS, BICORSTAB = 1, 2, 3, 4, 5
 METHODS = {"cg": CG, "bicg": BICG, "bicgstab": BICGSTAB, "cors": CORS, "bicorstab": BICORSTAB}
 OK, ERR_ARG, ERR_DEVICE, ERR_OOM, NOT_CONVERGED, BREAKDOWN, ERR_NAN, ERR_UNSUPPORTED = range(8)
@@ -81,7 +82,8 @@ def load():
     L.vie_set_rhs.argtypes = [vp, dp]
     L.vie_solve_resident.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(Info)]
     L.vie_get_solution.argtypes = [vp, dp]
-    L.vie_solve_mixed_resident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_int, ct.POINTER(Info)]
+    L.vie_solve_mixed_resident.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
+                                           ct.POINTER(Info)]
     L.vie_forward_scatter.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double, dp]
     L.vie_absorbed.argtypes = [vp, ct.c_int, dp]
     L.vie_time_kernel.argtypes = [vp, ct.c_int]
@@ -203,10 +205,15 @@ class Solver:
                     allow=(OK, NOT_CONVERGED))
         return info.as_dict()
 
-    def solve_mixed(self, tol=1e-6, inner_tol=0.0, maxit=0):
+    def solve_mixed(self, tol=1e-6, inner_param=0.0, maxit=0, inner_method=BICGSTAB, strategy=MIXED_RELIABLE):
+        """FP32 Krylov recurrences with FP64 solution and FP64 true-residual stop rule
+        (resident b -> x).  strategy MIXED_RELIABLE: residual replacement every
+        inner_param reduction; MIXED_REFINE: restarted refinement, inner tolerance inner_param."""
+        inner_method = METHODS.get(inner_method, inner_method) if isinstance(inner_method, str) else inner_method
         info = Info()
-        self._check(self.lib.vie_solve_mixed_resident(self.h, float(tol), float(inner_tol), int(maxit),
-                                                      ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        self._check(self.lib.vie_solve_mixed_resident(self.h, int(strategy), int(inner_method), float(tol),
+                                                      float(inner_param), int(maxit), ct.byref(info)),
+                    allow=(OK, NOT_CONVERGED))
         return info.as_dict()
 
     def solution(self):
