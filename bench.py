@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s to rel. residual 1e-6 @128³ grid, mixed vs FP64; % of HBM roofline"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# HBM bytes per launch of the FP32 x pass at 128^3, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of known size):
+# profiles/k_xreg_f32_128cube_hbm_traffic.md
+XPASS_F32_128_TRAFFIC = 443.1e6
 
 
 def model_bytes(V, oct_, s):
@@ -147,7 +151,8 @@ def main():
                  if ms3_64 > 0 else None},
         "speedup_mixed_vs_fp64": t64 / tm,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": XPASS_F32_128_TRAFFIC if n == (128, 128, 128) else None,
                      "kernel": "k_xreg<float,128,32,XPlan<8,8,4>> (x-axis forward + spectral product + inverse)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }

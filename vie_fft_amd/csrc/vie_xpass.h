@@ -31,11 +31,36 @@ template <int... Rs> struct XPlan {
     }
 };
 
-template <class R, int N, int T, class PL> struct XCfg {
+// CS: column-slow thread map (col = tid / T, the z passes, whose lanes walk
+// along a contiguous row); otherwise col = tid % B (lanes walk across
+// columns).  Exchange slots are padded by one every M positions, M = E for
+// power-of-two E (else 4): with that padding every exchange of the plans in
+// use is free of LDS bank conflicts in the column-fast map and within 1.75x
+// of conflict-free in the column-slow map (64 lanes, 32 or 64 banks, 8- and
+// 16-byte elements, checked by enumeration).
+// ROW1 (column-fast only): rows of B + 1 slots instead, which costs the FP32
+// x pass fewer registers (one more wave per SIMD) than it loses to the
+// 2-way conflicts left on two of its exchanges.
+template <class R, int N, int T, class PL, bool CS = false, bool ROW1 = false> struct XCfg {
     static constexpr int P = 2 * N;
     static constexpr int E = P / T;
     static constexpr int B = NT / T;     // columns per workgroup
-    static constexpr int LD = B + 1;     // padded LDS row (bank spread)
+    static constexpr int M = (E & (E - 1)) == 0 ? E : 4;
+    static constexpr int LM = M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : M == 32 ? 5 : 6;
+    static constexpr int PP = P + P / M;  // padded positions per column
+    static constexpr int LDS = ROW1 ? P * (B + 1) : PP * B;  // exchange buffer elements
+    // padded slot of position p, and the LDS index of slot l in column col
+    __device__ static constexpr unsigned pad(unsigned p) { return ROW1 ? p : p + (p >> LM); }
+    __device__ static constexpr unsigned idx(unsigned l, unsigned col)
+    {
+        return CS ? col * PP + l : l * (ROW1 ? B + 1 : B) + col;
+    }
+    __device__ static constexpr unsigned at(unsigned p, unsigned col) { return idx(pad(p), col); }
+    // pad(p0 + d) == pad(p0) + pad(d) whenever p0 is a multiple of M: then the
+    // slots of one butterfly are a common base plus compile-time offsets
+    static constexpr int step(int d) { return ROW1 ? d * (B + 1) : CS ? d + d / M : (d + d / M) * B; }
+    static_assert(!(CS && ROW1), "row padding applies to the column-fast map");
+    static_assert((1 << LM) == M, "pad period must be a power of two");
     static_assert(P % T == 0, "T must divide P");
     static_assert(B * T == NT, "T must divide the workgroup");
 };
@@ -80,20 +105,33 @@ __device__ __forceinline__ void xi_stage(T_* v, const T_* tw, int t)
 template <class T_, class C, class PL, int s>
 __device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
 {
-    constexpr int r = PL::r[s], S = PL::stride(s), r2 = PL::r[s + 1], P = C::P, E = C::E, T = P / E,
-                  LD = C::LD;
+    constexpr int r = PL::r[s], S = PL::stride(s), r2 = PL::r[s + 1], P = C::P, E = C::E, T = P / E;
+    constexpr bool wsep = (S * r) % C::M == 0 && (S % C::M == 0 || C::M % S == 0);
+    constexpr bool rsep = (P / r2) % C::M == 0;
 #pragma unroll
     for (int b = 0; b < E / r; ++b) {
         const int i = t + T * b, k = i % S, j = (i - k) * r + k;
+        if constexpr (wsep) {
+            T_* w = lds + C::idx(C::pad(j), col);
 #pragma unroll
-        for (int q = 0; q < r; ++q) lds[(j + q * S) * LD + col] = v[b * r + q];
+            for (int q = 0; q < r; ++q) w[C::step(q * S)] = v[b * r + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < r; ++q) lds[C::at(j + q * S, col)] = v[b * r + q];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < E / r2; ++b) {
         const int i = t + T * b;
+        if constexpr (rsep) {
+            const T_* w = lds + C::idx(C::pad(i), col);
 #pragma unroll
-        for (int q = 0; q < r2; ++q) v[b * r2 + q] = lds[(i + q * (P / r2)) * LD + col];
+            for (int q = 0; q < r2; ++q) v[b * r2 + q] = w[C::step(q * (P / r2))];
+        } else {
+#pragma unroll
+            for (int q = 0; q < r2; ++q) v[b * r2 + q] = lds[C::at(i + q * (P / r2), col)];
+        }
     }
     __syncthreads();
 }
@@ -104,19 +142,33 @@ template <class T_, class C, class PL, int s>
 __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
 {
     constexpr int r = PL::r[s], rp = PL::r[s - 1], Sp = PL::stride(s - 1), P = C::P, E = C::E,
-                  T = P / E, LD = C::LD;
+                  T = P / E;
+    constexpr bool wsep = (P / r) % C::M == 0;
+    constexpr bool rsep = (Sp * rp) % C::M == 0 && (Sp % C::M == 0 || C::M % Sp == 0);
 #pragma unroll
     for (int b = 0; b < E / r; ++b) {
         const int i = t + T * b;
+        if constexpr (wsep) {
+            T_* w = lds + C::idx(C::pad(i), col);
 #pragma unroll
-        for (int q = 0; q < r; ++q) lds[(i + q * (P / r)) * LD + col] = v[b * r + q];
+            for (int q = 0; q < r; ++q) w[C::step(q * (P / r))] = v[b * r + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < r; ++q) lds[C::at(i + q * (P / r), col)] = v[b * r + q];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < E / rp; ++b) {
         const int i = t + T * b, k = i % Sp, j = (i - k) * rp + k;
+        if constexpr (rsep) {
+            const T_* w = lds + C::idx(C::pad(j), col);
 #pragma unroll
-        for (int q = 0; q < rp; ++q) v[b * rp + q] = lds[(j + q * Sp) * LD + col];
+            for (int q = 0; q < rp; ++q) v[b * rp + q] = w[C::step(q * Sp)];
+        } else {
+#pragma unroll
+            for (int q = 0; q < rp; ++q) v[b * rp + q] = lds[C::at(j + q * Sp, col)];
+        }
     }
     __syncthreads();
 }
@@ -158,19 +210,24 @@ __device__ __forceinline__ int fold_split(int t, int n)
 }
 
 // grid (2Ny * ceil(2Nz / B)); N == Nx; in place on W2.
-// twP[e] = exp(-2 pi j e / P), e < P.
-template <class R, int N, int T, class PL>
-__global__ __launch_bounds__(NT) void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP,
-                                             MixTabs<R> mt, const int* done)
+// twP[e] = exp(-2 pi j e / P), e < P.  WPE: waves per SIMD the register
+// allocation aims for (0: compiler default).
+template <class R, int N, int T, class PL, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTabs<R> mt, const int* done)
 {
     using T_ = cx<R>;
-    using C = XCfg<R, N, T, PL>;
-    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    using C = XCfg<R, N, T, PL, false, sizeof(R) == 4>;
+    constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    __shared__ T_ lds[P * LD];
+    // FP64: the G octant slice of the tile is staged in LDS (occupancy is
+    // register-bound there); FP32 reads G per frequency, which keeps LDS
+    // small enough for the register-bound occupancy.
+    constexpr bool GSL = sizeof(R) == 8;
+    __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
-    __shared__ T_ uv[2 * P];          // U and Vv of the x axis by frequency
-    __shared__ T_ gsl[(N + 1) * B];  // G octant slice [fx][col] of this tile
+    __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
+    __shared__ T_ gsl[GSL ? (N + 1) * B : 1];    // G octant slice [fx][col]
     if (done && *done) return;
     const int P1 = 2 * g.ny, P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
@@ -195,13 +252,15 @@ __global__ __launch_bounds__(NT) void k_xreg(Geo g, cx<R>* __restrict__ W2, cons
                 v[c][b * r1 + q] = (n < N && colok) ? base[c * cst + n * rowst] : czero<T_>();
             }
         }
-    const long gs = (long)(g.ny + 1) * (g.nz + 1);
-    const long gy = (long)fold_split(ty, g.ny) * (g.nz + 1);
-    for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
-        const int fxe = e / B, ce = e % B;
-        const int tz = z0 + ce < P2 ? z0 + ce : z0;
-        T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tz, g.nz)];
-        gsl[e] = mt.gconj ? cconj(gv) : gv;
+    if constexpr (GSL) {
+        const long gs = (long)(g.ny + 1) * (g.nz + 1);
+        const long gy = (long)fold_split(ty, g.ny) * (g.nz + 1);
+        for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
+            const int fxe = e / B, ce = e % B;
+            const int tze = z0 + ce < P2 ? z0 + ce : z0;
+            T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tze, g.nz)];
+            gsl[e] = mt.gconj ? cconj(gv) : gv;
+        }
     }
     for (int e = threadIdx.x; e < P; e += NT) {
         tw[e] = twP[e];
@@ -212,12 +271,14 @@ __global__ __launch_bounds__(NT) void k_xreg(Geo g, cx<R>* __restrict__ W2, cons
     const T_ uy = mt.U[1][ty], vy = mt.Vv[1][ty];
     const int tz = colok ? z0 + col : z0;
     const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
+    // G octant column of this thread: G[fx][fold(ky)][fold(kz)], read in the product
+    const T_* gcol = mt.G + (long)fold_split(ty, g.ny) * (g.nz + 1) + fold_split(tz, g.nz);
+    const long gs = (long)(g.ny + 1) * (g.nz + 1);
     __syncthreads();
     x_forward<T_, C, PL>(v, lds, tw, t, col);
 
-    const T_ D0 = mkc(mt.D[0][0], mt.D[0][1]);
-    const T_ D1 = mkc(mt.D[1][0], mt.D[1][1]);
-    const T_ D2 = mkc(mt.D[2][0], mt.D[2][1]);
+    // D_f = j k0 Z0 delta_f / NP is imaginary: D_f X = (-Im D_f X.y, Im D_f X.x)
+    const R D0 = mt.D[0][1], D1 = mt.D[1][1], D2 = mt.D[2][1];
     // last forward stage: register e = b rL + q holds frequency i + q P/rL, i = t + T b
 #pragma unroll
     for (int b = 0; b < E / rL; ++b) {
@@ -226,14 +287,20 @@ __global__ __launch_bounds__(NT) void k_xreg(Geo g, cx<R>* __restrict__ W2, cons
             const int kx = t + (P / E) * b + q * (P / rL);
             const int kf = kx < P - kx ? kx : P - kx;
             const int fx = (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
-            const T_ Gv = gsl[fx * B + col];
+            T_ Gv;
+            if constexpr (GSL) {
+                Gv = gsl[fx * B + col];
+            } else {
+                Gv = gcol[fx * gs];
+                if (mt.gconj) Gv = cconj(Gv);
+            }
             const int e = b * rL + q;
             const T_ X0 = v[0][e], X1 = v[1][e], X2 = v[2][e];
             const T_ ux = uv[kx], vx = uv[P + kx];
             const T_ sig = cadd(cadd(cmul(vx, X0), cmul(vy, X1)), cmul(vz, X2));
-            v[0][e] = cmul(Gv, cadd(cmul(D0, X0), cmul(ux, sig)));
-            v[1][e] = cmul(Gv, cadd(cmul(D1, X1), cmul(uy, sig)));
-            v[2][e] = cmul(Gv, cadd(cmul(D2, X2), cmul(uz, sig)));
+            v[0][e] = cmul(Gv, cadd(mkc(-D0 * X0.y, D0 * X0.x), cmul(ux, sig)));
+            v[1][e] = cmul(Gv, cadd(mkc(-D1 * X1.y, D1 * X1.x), cmul(uy, sig)));
+            v[2][e] = cmul(Gv, cadd(mkc(-D2 * X2.y, D2 * X2.x), cmul(uz, sig)));
         }
     }
     x_inverse<T_, C, PL>(v, lds, tw, t, col);
@@ -293,10 +360,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
                                                const cx<R>* __restrict__ twP, const int* done)
 {
     using T_ = cx<R>;
-    using C = XCfg<R, N, T, PL>;
-    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    using C = XCfg<R, N, T, PL, true>;
+    constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    __shared__ T_ lds[P * LD];
+    __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
@@ -360,10 +427,10 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                                                Dots<R> dots, const cx<R>* __restrict__ twP, const int* done)
 {
     using T_ = cx<R>;
-    using C = XCfg<R, N, T, PL>;
-    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    using C = XCfg<R, N, T, PL, true>;
+    constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    __shared__ T_ lds[P * LD];
+    __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     __shared__ double2 red[NT / 64];
     if (done && *done) return;
@@ -450,9 +517,9 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
 {
     using T_ = cx<R>;
     using C = XCfg<R, N, T, PL>;
-    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    __shared__ T_ lds[P * LD];
+    __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
@@ -493,9 +560,9 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
 {
     using T_ = cx<R>;
     using C = XCfg<R, N, T, PL>;
-    constexpr int P = C::P, E = C::E, B = C::B, LD = C::LD;
+    constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    __shared__ T_ lds[P * LD];
+    __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
