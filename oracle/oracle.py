@@ -64,6 +64,12 @@ def lib():
         L.orc_absorbed.argtypes = [ct.POINTER(Grid), dp, dp, ct.POINTER(ct.c_ubyte), ct.c_int]
         L.orc_absorbed.restype = ct.c_double
         L.orc_rcs.argtypes = [ct.POINTER(Grid), dp, ct.c_double, ct.c_double, dp, dp]
+        ub = ct.POINTER(ct.c_ubyte)
+        L.orc_absorption.argtypes = [ct.POINTER(Grid), dp, dp, ub]
+        L.orc_absorption.restype = ct.c_double
+        L.orc_absorption_grad.argtypes = [ct.POINTER(Grid), ct.c_void_p, dp, dp, ub, ct.c_int, ct.c_double,
+                                          ct.c_int, dp, dp, ct.POINTER(Info)]
+        L.orc_absorption_grad.restype = ct.c_double
         _lib = L
     return _lib
 
@@ -206,3 +212,25 @@ def rcs(n, d, k0, J, theta=0.0, phi=0.0):
     rp = np.zeros(181)
     lib().orc_rcs(ct.byref(grid(n, d, k0)), _dp(J), theta, phi, _dp(rt), _dp(rp))
     return rt, rp
+
+
+def absorption(n, d, k0, J, eps, flags):
+    """Absorbed power over all active faces (``orc_absorption``)."""
+    J = _c128(J)
+    eps = _c128(eps)
+    f = np.ascontiguousarray(flags, dtype=np.uint8)
+    return lib().orc_absorption(ct.byref(grid(n, d, k0)), _dp(J), _dp(eps),
+                                f.ctypes.data_as(ct.POINTER(ct.c_ubyte)))
+
+
+def absorption_grad(op, J, method=3, tol=1e-12, maxit=None):
+    """(P, dP/dRe eps, dP/dIm eps, info) for the solution J of op (``orc_absorption_grad``)."""
+    J = _c128(J).reshape((3,) + op.n)
+    gre = np.zeros(op.n)
+    gim = np.zeros(op.n)
+    info = Info()
+    P = lib().orc_absorption_grad(ct.byref(op.g), op.h, _dp(J), _dp(op.eps),
+                                  op.flags.ctypes.data_as(ct.POINTER(ct.c_ubyte)), int(method), float(tol),
+                                  int(maxit if maxit else 3 * int(np.prod(op.n))), _dp(gre), _dp(gim),
+                                  ct.byref(info))
+    return P, gre, gim, {"iterations": info.iterations, "converged": bool(info.converged), "err": info.err}

@@ -881,3 +881,99 @@ void orc_rcs(const orc_grid* g, const double* Jd, double theta, double phi,
         rcs_phi[a] = 10.0 * log10(r / lam / lam);
     }
 }
+
+/* ---- absorbed power over active faces and its adjoint gradient ----------
+ * Per face (f, i) with neighbour i + e_f, S = 1/(eps_i - 1) + 1/(eps_{i+e_f} - 1),
+ * NearField's absorbed power (vie_fft/NearFar.h:192-205) is
+ *   (k0/Z0) Im(-eps_eff) |J|^2 |(-j Z0/(2 k0)) S|^2 dV,  eps_eff = 2/S + 1,
+ * and Im(-eps_eff) |S|^2 = -Im(2 conj(S)) = 2 Im(S), so the weight reduces to
+ * w = dV Z0/(2 k0) Im(S).
+ *
+ * Gradient: the product is A = conv + diag(d), d_f[i] = c_f S_f[i] with
+ * c_f = -j (delta_f/2)(Z0/k0) (vie_fft/fft.h:419-434).  With A J = b and
+ * g = w conj(J),  dP = sum dw |J|^2 + 2 Re(g^T dJ),  dJ = -A^{-1} (dd J), so
+ *   dP/da = sum (dw/da) |J|^2 - 2 Re(mu^T (dd/da J)),  mu = A^{-T} g.
+ * diag(1/delta_f) A is complex symmetric (masked rows and columns included),
+ * hence A^{-T} g = diag(1/delta) A^{-1} diag(delta) g: one solve of A nu =
+ * delta g with the same operator, mu = nu / delta.  For a = Re(eps_v):
+ * dS/da = -1/(eps_v - 1)^2 on every face touching v; for a = Im(eps_v) the
+ * same times j (d is holomorphic in eps; w is not). */
+static int face_active(const orc_grid* g, const unsigned char* flags, int f, int i, int j, int k)
+{
+    const int c[3] = {i, j, k};
+    if (c[f] >= g->n[f] - 1) return 0;
+    return flags[(long)f * vol(g) + ((long)i * g->n[1] + j) * g->n[2] + k] != 0;
+}
+
+double orc_absorption(const orc_grid* g, const double* Jd, const double* epsd, const unsigned char* flags)
+{
+    const cplx* J = (const cplx*)Jd;
+    const cplx* eps = (const cplx*)epsd;
+    const long V = vol(g);
+    const long st[3] = {(long)g->n[1] * g->n[2], g->n[2], 1};
+    const double K = g->d[0] * g->d[1] * g->d[2] * ORC_Z0 / (2.0 * g->k0);
+    double P = 0;
+    for (int i = 0; i < g->n[0]; ++i)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int k = 0; k < g->n[2]; ++k)
+                for (int f = 0; f < 3; ++f) {
+                    if (!face_active(g, flags, f, i, j, k)) continue;
+                    const long v = ((long)i * g->n[1] + j) * g->n[2] + k;
+                    const cplx S = 1.0 / (eps[v] - 1.0) + 1.0 / (eps[v + st[f]] - 1.0);
+                    const double a = cabs(J[f * V + v]);
+                    P += K * cimag(S) * a * a;
+                }
+    return P;
+}
+
+double orc_absorption_grad(const orc_grid* g, orc_op* op, const double* Jd, const double* epsd,
+                           const unsigned char* flags, int method, double tol, int maxit,
+                           double* grad_re, double* grad_im, orc_info* info)
+{
+    const cplx* J = (const cplx*)Jd;
+    const cplx* eps = (const cplx*)epsd;
+    const long V = vol(g);
+    const int N1 = g->n[1], N2 = g->n[2];
+    const long st[3] = {(long)N1 * N2, N2, 1};
+    const double K = g->d[0] * g->d[1] * g->d[2] * ORC_Z0 / (2.0 * g->k0);
+    cplx* rhs = calloc(3 * V, sizeof(cplx));
+    cplx* nu = calloc(3 * V, sizeof(cplx));
+    double P = 0;
+    for (int i = 0; i < g->n[0]; ++i)
+        for (int j = 0; j < N1; ++j)
+            for (int k = 0; k < N2; ++k)
+                for (int f = 0; f < 3; ++f) {
+                    if (!face_active(g, flags, f, i, j, k)) continue;
+                    const long v = ((long)i * N1 + j) * N2 + k;
+                    const cplx S = 1.0 / (eps[v] - 1.0) + 1.0 / (eps[v + st[f]] - 1.0);
+                    const double w = K * cimag(S);
+                    P += w * creal(J[f * V + v] * conj(J[f * V + v]));
+                    rhs[f * V + v] = g->d[f] * w * conj(J[f * V + v]);
+                }
+    orc_info inf;
+    orc_solve(method, orc_op_matvec, op, 3 * V, (const double*)rhs, (double*)nu, tol, maxit, NULL, &inf);
+    if (info) *info = inf;
+    for (long v = 0; v < V; ++v) grad_re[v] = grad_im[v] = 0;
+    for (int i = 0; i < g->n[0]; ++i)
+        for (int j = 0; j < N1; ++j)
+            for (int k = 0; k < N2; ++k)
+                for (int f = 0; f < 3; ++f) {
+                    if (!face_active(g, flags, f, i, j, k)) continue;
+                    const long v = ((long)i * N1 + j) * N2 + k;
+                    const long F = f * V + v;
+                    const cplx cf = -I * (g->d[f] / 2.0) * ORC_Z0 / g->k0;
+                    const cplx mu = nu[F] / g->d[f];
+                    const double aJ2 = creal(J[F] * conj(J[F]));
+                    const long cells[2] = {v, v + st[f]};
+                    for (int s = 0; s < 2; ++s) {
+                        const long u = cells[s];
+                        const cplx dS = -1.0 / ((eps[u] - 1.0) * (eps[u] - 1.0)); /* dS/de_u */
+                        /* a = Re(eps_u): dS/da = dS; a = Im(eps_u): dS/da = j dS */
+                        grad_re[u] += K * cimag(dS) * aJ2 - 2.0 * creal(mu * cf * dS * J[F]);
+                        grad_im[u] += K * cimag(I * dS) * aJ2 - 2.0 * creal(mu * cf * I * dS * J[F]);
+                    }
+                }
+    free(rhs);
+    free(nu);
+    return P;
+}

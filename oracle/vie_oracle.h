@@ -89,6 +89,21 @@ double orc_absorbed(const orc_grid* g, const double* J, const double* eps,
 void orc_rcs(const orc_grid* g, const double* J, double theta, double phi,
              double* rcs_theta, double* rcs_phi);
 
+/* ---- absorbed power over all active faces and its eps gradient ----------
+   P = sum over faces (f, i) with flags_f[i] = 1 and i_f < N_f - 1 of
+   w_f[i] |J_f[i]|^2, w_f[i] = (dV Z0 / (2 k0)) Im(1/(eps[i]-1) + 1/(eps[i+e_f]-1)):
+   the per-face term of NearField's Ene_Metal (vie_fft/NearFar.h:186-250)
+   without its material and i,j,k < N-1 selection. */
+double orc_absorption(const orc_grid* g, const double* J, const double* eps,
+                      const unsigned char* flags);
+/* dP/dRe(eps_v) and dP/dIm(eps_v) for every voxel, J the solution of A J = b
+   for this operator: one more solve, of A nu = delta_f w conj(J) with the same
+   operator (diag(1/delta_f) A is complex symmetric), by `method` to `tol`.
+   Returns P; info (may be NULL) describes the extra solve. */
+double orc_absorption_grad(const orc_grid* g, orc_op* op, const double* J, const double* eps,
+                           const unsigned char* flags, int method, double tol, int maxit,
+                           double* grad_re, double* grad_im, orc_info* info);
+
 #ifdef __cplusplus
 }
 #endif
