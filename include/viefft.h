@@ -144,6 +144,25 @@ int vie_forward_scatter(vie_ctx* ctx, double theta, double phi, double psi, doub
    (needs vie_set_medium); times Z0 / area = Q_abs. */
 int vie_absorbed(vie_ctx* ctx, int mat, double* value);
 
+/* Absorbed power of the resident solution over every active face (flag set,
+   own-axis index < N_f - 1): P = sum w_f |J_f|^2, w_f = dV Z0/(2 k0)
+   Im(1/(eps_a - 1) + 1/(eps_b - 1)) for the two cells a, b of the face.  This
+   is the per-face term of NearField's Ene_Metal (vie_fft/NearFar.h:186-250)
+   without its material and i,j,k < N-1 selection.  Needs eps and flags that
+   are 0 on own-axis last planes (the 3-point flag rule gives that). */
+int vie_absorption(vie_ctx* ctx, double* value);
+
+/* Adjoint gradient of vie_absorption with respect to the permittivity of every
+   cell (no reference counterpart; the design-gradient workload): one more
+   solve with the same operator, A nu = delta_f w conj(J), by `method` to
+   `tol` (diag(1/delta_f) A is complex symmetric), then one gather over the six
+   faces of each cell.  grad_re / grad_im (host, V doubles each; grad_im may be
+   NULL) receive dP/dRe(eps) and dP/dIm(eps); *value (may be NULL) receives P;
+   info describes the adjoint solve.  The resident solution must solve A J = b
+   for the current medium (vie_solve_resident or vie_solve before). */
+int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double* value, double* grad_re,
+                        double* grad_im, vie_info* info);
+
 /* ---- instrumentation ----------------------------------------------------- */
 /* Per-launch HIP-event timing of one kernel class while solving:
    0 off, 1 z-forward, 2 y-forward, 3 x spectral pass, 4 y-inverse,

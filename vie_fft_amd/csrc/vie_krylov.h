@@ -595,4 +595,92 @@ __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+
+// ---------------------------------------------------- absorbed power, gradient
+// Face (f, v) is active when flags_f[v] != 0 and v_f < N_f - 1; its absorbed
+// power is w |J|^2 with w = K Im(S), K = dV Z0 / (2 k0),
+// S = 1/(eps_v - 1) + 1/(eps_{v+e_f} - 1): NearField's per-face Ene_Metal term
+// (vie_fft/NearFar.h:192-205) with Im(-eps_eff)|S|^2 = 2 Im(S) folded in.
+__device__ __forceinline__ double2 zinv1(double2 e)  // 1 / (e - 1)
+{
+    return zdiv(make_double2(1, 0), make_double2(e.x - 1.0, e.y));
+}
+
+// P partial sums per block; rhs (optional) = delta_f w conj(J) on active
+// faces, 0 elsewhere: the right-hand side of the adjoint solve.
+__global__ __launch_bounds__(NT) void k_abs_rhs(int n0, int n1, int n2, double K, double d0, double d1, double d2,
+                                                const double2* __restrict__ J, const double2* __restrict__ eps,
+                                                const uint8_t* __restrict__ flags, double2* __restrict__ rhs,
+                                                double2* partial)
+{
+    __shared__ double2 red[NT / 64];
+    const long V = (long)n0 * n1 * n2;
+    const long st[3] = {(long)n1 * n2, n2, 1};
+    const int nn[3] = {n0, n1, n2};
+    const double dd[3] = {d0, d1, d2};
+    double acc = 0;
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int c[3] = {(int)(v / ((long)n1 * n2)), (int)((v / n2) % n1), (int)(v % n2)};
+        const double2 iv = zinv1(eps[v]);
+        for (int f = 0; f < 3; ++f) {
+            const long F = f * V + v;
+            double2 r = make_double2(0, 0);
+            if (c[f] < nn[f] - 1 && flags[F]) {
+                const double2 iw = zinv1(eps[v + st[f]]);
+                const double w = K * (iv.y + iw.y);
+                const double2 j = J[F];
+                acc += w * (j.x * j.x + j.y * j.y);
+                r = make_double2(dd[f] * w * j.x, -dd[f] * w * j.y);
+            }
+            if (rhs) rhs[F] = r;
+        }
+    }
+    double2 s = block_sum(make_double2(acc, 0), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// dP/dRe(eps_u), dP/dIm(eps_u) for every cell u, gathering the (up to six)
+// active faces that touch u.  nu solves A nu = rhs (k_abs_rhs); the adjoint
+// field is mu_f = nu_f / delta_f.  With dS = -1/(eps_u - 1)^2 and
+// c_f = -j (delta_f/2)(Z0/k0) (the diagonal's factor, vie_fft/fft.h:419-434):
+//   d/dRe: K Im(dS) |J|^2 - 2 Re(mu c_f dS J);  d/dIm: same with dS -> j dS.
+__global__ __launch_bounds__(NT) void k_abs_grad(int n0, int n1, int n2, double K, double z0k0, double d0,
+                                                 double d1, double d2, const double2* __restrict__ J,
+                                                 const double2* __restrict__ nu, const double2* __restrict__ eps,
+                                                 const uint8_t* __restrict__ flags, double* __restrict__ gre,
+                                                 double* __restrict__ gim)
+{
+    const long V = (long)n0 * n1 * n2;
+    const long st[3] = {(long)n1 * n2, n2, 1};
+    const int nn[3] = {n0, n1, n2};
+    const double dd[3] = {d0, d1, d2};
+    for (long u = (long)blockIdx.x * NT + threadIdx.x; u < V; u += (long)gridDim.x * NT) {
+        const int c[3] = {(int)(u / ((long)n1 * n2)), (int)((u / n2) % n1), (int)(u % n2)};
+        const double2 iu = zinv1(eps[u]);
+        const double2 dS = zneg(zmul(iu, iu));
+        double ar = 0, ai = 0;
+        for (int f = 0; f < 3; ++f) {
+            const double2 cf = make_double2(0, -0.5 * dd[f] * z0k0);
+            for (int side = 0; side < 2; ++side) {
+                // side 0: face (f, u) (u is its lower cell); side 1: face (f, u - e_f)
+                const int cf_idx = c[f] - side;
+                if (cf_idx < 0 || cf_idx >= nn[f] - 1) continue;
+                const long F = f * V + u - side * st[f];
+                if (!flags[F]) continue;
+                const double2 j = J[F];
+                const double2 m = nu[F];
+                const double2 mu = make_double2(m.x / dd[f], m.y / dd[f]);
+                const double aJ2 = j.x * j.x + j.y * j.y;
+                const double2 t = zmul(zmul(mu, cf), j);  // mu c_f J
+                const double2 tr = zmul(t, dS);           // times dS
+                ar += K * dS.y * aJ2 - 2.0 * tr.x;
+                // j dS: Im(j dS) = Re(dS); Re(t j dS) = -Im(t dS)
+                ai += K * dS.x * aJ2 + 2.0 * tr.y;
+            }
+        }
+        gre[u] = ar;
+        if (gim) gim[u] = ai;
+    }
+}
+
 }  // namespace vie

@@ -109,24 +109,61 @@ class Scene:
             done |= sel
         return out
 
+    def material_grid(self, xs, ys, zs):
+        """material_at on the tensor grid xs x ys x zs (sorted 1-D coordinate
+        arrays in cell units): each shape is evaluated on its bounding block only."""
+        xs, ys, zs = (np.asarray(a, dtype=np.float64) for a in (xs, ys, zs))
+        out = np.zeros((xs.size, ys.size, zs.size), dtype=np.uint8)
+        done = np.zeros(out.shape, dtype=bool)
+
+        def rng(a, lo, hi, nmax):  # index range of a within [lo, hi] and inside [0, nmax]
+            lo, hi = max(lo, 0.0), min(hi, float(nmax))
+            return np.searchsorted(a, lo, "left"), np.searchsorted(a, hi, "right")
+
+        for s in self.shapes:
+            if isinstance(s, Sphere):
+                cx, cy, cz = s.center
+                r = s.radius
+                i0, i1 = rng(xs, cx - r, cx + r, self.n[0])
+                j0, j1 = rng(ys, cy - r, cy + r, self.n[1])
+                k0, k1 = rng(zs, cz - r, cz + r, self.n[2])
+                if i0 >= i1 or j0 >= j1 or k0 >= k1:
+                    continue
+                hit = ((xs[i0:i1, None, None] - cx) ** 2 + (ys[None, j0:j1, None] - cy) ** 2
+                       + (zs[None, None, k0:k1] - cz) ** 2) <= r * r
+            elif isinstance(s, Band):
+                i0, i1 = rng(xs, 0.0, self.n[0], self.n[0])
+                j0, j1 = rng(ys, 0.0, self.n[1], self.n[1])
+                k0 = np.searchsorted(zs, max(s.z0, 0.0), "left")
+                k1 = np.searchsorted(zs, s.z1, "left")  # z < z1
+                k1 = min(k1, np.searchsorted(zs, float(self.n[2]), "right"))
+                if i0 >= i1 or j0 >= j1 or k0 >= k1:
+                    continue
+                hit = np.ones((i1 - i0, j1 - j0, k1 - k0), dtype=bool)
+            else:
+                raise TypeError(s)
+            blk = (slice(i0, i1), slice(j0, j1), slice(k0, k1))
+            sel = hit & ~done[blk]
+            out[blk][sel] = s.mat
+            done[blk] |= sel
+        return out
+
     def material(self):
-        i, j, k = np.meshgrid(*[np.arange(v, dtype=np.float64) for v in self.n], indexing="ij")
-        return self.material_at(i + 0.5, j + 0.5, k + 0.5)
+        return self.material_grid(*[np.arange(v, dtype=np.float64) + 0.5 for v in self.n])
 
     def flags(self):
-        """Flag_X, Flag_Y, Flag_Z with the 3-point rule (vie_fft/Model.h:345-394)."""
-        i, j, k = np.meshgrid(*[np.arange(v, dtype=np.float64) for v in self.n], indexing="ij")
-        c = [i + 0.5, j + 0.5, k + 0.5]
+        """Flag_X, Flag_Y, Flag_Z with the 3-point rule (vie_fft/Model.h:345-394):
+        a face is active when the two cell centres it separates and the face
+        centre are all non-background."""
+        c = [np.arange(v, dtype=np.float64) + 0.5 for v in self.n]
         out = np.zeros((3,) + tuple(self.n), dtype=np.uint8)
         for f in range(3):
-            p = [a.copy() for a in c]
-            p[f] = p[f] + 0.5                      # face centre
-            m1 = self.material_at(*p)
-            p[f] = p[f] - 0.5
-            m2 = self.material_at(*p)
-            p[f] = p[f] + 1.0
-            m3 = self.material_at(*p)
-            out[f] = ((m1 != 0) & (m2 != 0) & (m3 != 0)).astype(np.uint8)
+            m = []
+            for shift in (0.5, 0.0, 1.0):   # face centre, lower cell, upper cell
+                p = list(c)
+                p[f] = c[f] + shift
+                m.append(self.material_grid(*p) != 0)
+            out[f] = (m[0] & m[1] & m[2]).astype(np.uint8)
         return out
 
     def eps(self):

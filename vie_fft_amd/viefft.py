@@ -31,7 +31,7 @@ EXPORTS = [
     "vie_set_medium_eps", "vie_green_compute", "vie_incident", "vie_apply", "vie_apply_async",
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
-    "vie_model_bytes_matvec",
+    "vie_model_bytes_matvec", "vie_absorption", "vie_absorption_grad",
 ]
 
 
@@ -90,6 +90,8 @@ def load():
     L.vie_kernel_time.argtypes = [vp, ct.POINTER(ct.c_int), dp]
     L.vie_model_bytes_matvec.argtypes = [vp]
     L.vie_model_bytes_matvec.restype = ct.c_double
+    L.vie_absorption.argtypes = [vp, dp]
+    L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
     _lib = L
     return L
 
@@ -233,6 +235,23 @@ class Solver:
         v = ct.c_double()
         self._check(self.lib.vie_absorbed(self.h, int(mat), ct.byref(v)))
         return v.value
+
+    def absorption(self):
+        """Absorbed power of the resident solution over all active faces."""
+        v = ct.c_double()
+        self._check(self.lib.vie_absorption(self.h, ct.byref(v)))
+        return v.value
+
+    def absorption_grad(self, method=BICGSTAB, tol=1e-10, maxit=0):
+        """(P, dP/dRe eps, dP/dIm eps, info of the adjoint solve) for the resident solution."""
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        v = ct.c_double()
+        gre = np.zeros(self.n)
+        gim = np.zeros(self.n)
+        info = Info()
+        self._check(self.lib.vie_absorption_grad(self.h, int(method), float(tol), int(maxit), ct.byref(v),
+                                                 _dp(gre), _dp(gim), ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        return v.value, gre, gim, info.as_dict()
 
     # -- instrumentation -----------------------------------------------------
     def time_kernel(self, cls):
