@@ -95,6 +95,14 @@ int vie_set_medium_eps(vie_ctx* ctx, const double* eps_voxel, const unsigned cha
 /* Green_Function_Compute (vie_fft/GreenFunction.h:8-141): computed on the
    device; only the octant that the even extension makes unique is kept. */
 int vie_green_compute(vie_ctx* ctx);
+
+/* Change the free-space wavenumber of an existing context (one grid swept over
+   wavelengths, as the reference's frequency loop does by re-running the
+   program per wavelength, vie_fft/main.cpp:27-36): the spectral tables and the
+   diagonal term follow; G~ must be recomputed (vie_green_compute) and the
+   right-hand side set again.  The medium stays; call vie_set_medium again for
+   dispersive materials. */
+int vie_set_k0(vie_ctx* ctx, double k0);
 /* Incident_Wave (vie_fft/GreenFunction.h:144-248); angles in degrees;
    E: host, 3V complex, written unmasked like Ex/Ey/Ez_inc. */
 int vie_incident(vie_ctx* ctx, double theta, double phi, double psi, double* E);
@@ -118,6 +126,10 @@ int vie_solve(vie_ctx* ctx, int method, double tol, int maxit, const double* b, 
 /* resident variant: b is uploaded once by vie_set_rhs and the solution stays
    on the device (vie_get_solution copies it out); nothing crosses PCIe inside. */
 int vie_set_rhs(vie_ctx* ctx, const double* b);
+/* Resident right-hand side computed on the device: Incident_Wave
+   (vie_fft/GreenFunction.h:144-248) for angles theta, phi, psi (degrees)
+   masked by the flags (vie_fft/CG_Iteration.h:559-562); no host transfer. */
+int vie_set_rhs_incident(vie_ctx* ctx, double theta, double phi, double psi);
 int vie_solve_resident(vie_ctx* ctx, int method, double tol, int maxit, vie_info* info);
 int vie_get_solution(vie_ctx* ctx, double* x);
 /* Mixed precision: the Krylov recurrences, FFTs and spectral product run in

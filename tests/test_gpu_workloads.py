@@ -1,0 +1,79 @@
+"""Full-size workloads: the 128x128x64 layered case (5 materials, 550 nm)
+against the oracle at its real size, and the 256^3 random-spheres case
+(60 spheres, tol 1e-8) mixed precision against FP64."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import rel
+from vie_fft_amd import geometry as G
+from vie_fft_amd import viefft as VF
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    sc = G.layered_case()
+    eps, flags, mat = sc.eps(), sc.flags(), sc.material()
+    op = O.Operator(sc.n, sc.d, sc.k0, eps, flags, O.SCHED_FUSED)
+    s = VF.Solver(sc.n, sc.d, sc.k0)
+    s.set_medium(mat, sc.eps_table, flags)
+    s.green_compute()
+    b = s.incident(*sc.rhs_angles()) * flags
+    yield sc, op, s, b, flags
+    s.close()
+
+
+def test_c2_operator_full_size(c2):
+    sc, op, s, b, flags = c2
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((3,) + sc.n) + 1j * rng.standard_normal((3,) + sc.n)) * flags
+    ref = op.apply(x)
+    assert rel(s.apply(x), ref) <= 1e-12
+    assert rel(s.apply(x, adjoint=True), op.apply(x, adjoint=True)) <= 1e-12
+
+
+@pytest.mark.parametrize("method", [VF.BICGSTAB, VF.CORS])
+def test_c2_iterates_match_oracle(c2, method):
+    """Three iterations from x = 0 at full size: the GPU iterate equals the
+    oracle's (CORS included, although CORS does not converge on this case)."""
+    sc, op, s, b, flags = c2
+    ref = op.solve(b, method=method, tol=1e-30, maxit=3)
+    x, info = s.solve(b, method=method, tol=1e-30, maxit=3)
+    assert info["iterations"] == ref["iterations"] == 3
+    assert rel(x, ref["x"]) <= 1e-10
+
+
+@pytest.mark.parametrize("method", [VF.BICGSTAB, VF.BICORSTAB])
+def test_c2_converges(c2, method):
+    sc, op, s, b, flags = c2
+    s.set_rhs(b)
+    info = s.solve_resident(method, tol=1e-6, maxit=5000)
+    assert info["converged"] and info["true_relres"] < 1.01e-6
+    qe = s.forward_scatter() / sc.area
+    mixed = s.solve_mixed(tol=1e-6, maxit=5000)
+    assert mixed["converged"] and mixed["true_relres"] < 1e-6
+    assert abs(s.forward_scatter() / sc.area - qe) <= 1e-5 * abs(qe)
+
+
+def test_c3_mixed_matches_fp64():
+    sc = G.random_spheres_case()
+    flags = sc.flags()
+    with VF.Solver(sc.n, sc.d, sc.k0) as s:
+        s.set_medium(sc.material(), sc.eps_table, flags)
+        s.green_compute()
+        s.set_rhs(s.incident(*sc.rhs_angles()))
+        f64 = s.solve_resident(VF.BICGSTAB, tol=1e-8, maxit=20000)
+        q64 = s.forward_scatter()
+        a64 = s.absorbed(1)
+        x64 = s.solution()
+        mixed = s.solve_mixed(tol=1e-8, maxit=20000)
+        qm = s.forward_scatter()
+        am = s.absorbed(1)
+        xm = s.solution()
+    assert f64["converged"] and f64["true_relres"] < 1e-8
+    assert mixed["converged"] and mixed["true_relres"] < 1e-8
+    assert abs(qm - q64) <= 1e-7 * abs(q64)
+    assert abs(am - a64) <= 1e-7 * abs(a64)
+    assert rel(xm, x64) <= 1e-6

@@ -148,7 +148,7 @@ template <class R> struct Op {
         const double NP = 8.0 * (double)g.V;
         for (int a = 0; a < 3; ++a) {
             const int N = n[a];
-            std::vector<T> h_tw(N), h_twist(N), h_u(2 * N), h_v(2 * N), h_ua(2 * N), h_va(2 * N);
+            std::vector<T> h_tw(N), h_twist(N), h_u(2 * N), h_va(2 * N);
             std::vector<int> h_fold(2 * N);
             for (int t = 0; t < N; ++t) {
                 const long double a1 = -2.0L * (long double)kPi * t / N;
@@ -156,17 +156,13 @@ template <class R> struct Op {
                 h_tw[t] = hc<R>(std::complex<double>((double)cosl(a1), (double)sinl(a1)));
                 h_twist[t] = hc<R>(std::complex<double>((double)cosl(a2), (double)sinl(a2)));
             }
-            const std::complex<double> J(0, 1);
             for (int t = 0; t < 2 * N; ++t) {
                 const int k = 2 * (t % N) + t / N;
                 const long double th = (long double)kPi * k / N;
                 const double s = (double)sinl(th), h = (double)sinl(th / 2);
-                // e^{j th} - 1 = -2 sin^2(th/2) + j sin th ; 1 - e^{-j th} = 2 sin^2(th/2) + j sin th
-                const std::complex<double> u(-2.0 * h * h, s), w(2.0 * h * h, s);
-                const std::complex<double> vv = J * kZ0 / (k0 * d[a] * NP) * w;
+                // e^{j th} - 1 = -2 sin^2(th/2) + j sin th
+                const std::complex<double> u(-2.0 * h * h, s);
                 h_u[t] = hc<R>(u);
-                h_v[t] = hc<R>(vv);
-                h_ua[t] = hc<R>(std::conj(vv));
                 h_va[t] = hc<R>(std::conj(u));
                 const int kf = std::min(k, 2 * N - k);
                 h_fold[t] = (kf % 2 == 0) ? kf / 2 : (N / 2 + 1) + (kf - 1) / 2;
@@ -181,8 +177,6 @@ template <class R> struct Op {
             copy_sync(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
             copy_sync(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
             copy_sync(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
             copy_sync(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
             copy_sync(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice, st);
         }
@@ -214,15 +208,41 @@ template <class R> struct Op {
             ma.U[a] = Ua[a];
             ma.Vv[a] = Va[a];
             ma.fold[a] = fold[a];
-            const std::complex<double> D = std::complex<double>(0, 1) * k0 * kZ0 * d[a] / NP;
+        }
+        mf.G = ma.G = G;
+        mf.gconj = 0;
+        ma.gconj = 1;
+        set_k0(kk, st);
+    }
+
+    // The wavenumber enters the tables through v_s = (j Z0/(k0 delta_s NP))
+    // (1 - e^{-j theta_s}) (its conjugate is the adjoint's u) and D_f =
+    // j k0 Z0 delta_f / NP (vie_fft/fft.h:76-100, 256-281).
+    void set_k0(double kk, hipStream_t st)
+    {
+        k0 = kk;
+        const double NP = 8.0 * (double)g.V;
+        const std::complex<double> J(0, 1);
+        for (int a = 0; a < 3; ++a) {
+            const int N = n[a];
+            std::vector<T> h_v(2 * N), h_ua(2 * N);
+            for (int t = 0; t < 2 * N; ++t) {
+                const int k = 2 * (t % N) + t / N;
+                const long double th = (long double)kPi * k / N;
+                const double s = (double)sinl(th), h = (double)sinl(th / 2);
+                const std::complex<double> w(2.0 * h * h, s);  // 1 - e^{-j th}
+                const std::complex<double> vv = J * kZ0 / (k0 * d[a] * NP) * w;
+                h_v[t] = hc<R>(vv);
+                h_ua[t] = hc<R>(std::conj(vv));
+            }
+            copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            const std::complex<double> D = J * k0 * kZ0 * d[a] / NP;
             mf.D[a][0] = (R)D.real();
             mf.D[a][1] = (R)D.imag();
             ma.D[a][0] = (R)D.real();
             ma.D[a][1] = (R)-D.imag();
         }
-        mf.G = ma.G = G;
-        mf.gconj = 0;
-        ma.gconj = 1;
     }
     ~Op()
     {
@@ -299,6 +319,8 @@ struct vie_ctx {
     Op<double>* op64 = nullptr;
     Op<float>* op32 = nullptr;
     double2* Goff = nullptr;       // FP64 octant of G~ (parity order)
+    double2* gwork = nullptr;      // second octant buffer of green_compute
+    double* gdct = nullptr;        // cosine tables of green_compute
     bool green_ready = false;
     bool medium_ready = false;
     std::vector<double> eps_h;     // per-voxel eps, 2V doubles
@@ -417,40 +439,27 @@ void green_compute(vie_ctx* c)
     const int* n = c->desc.n;
     const double* d = c->desc.d;
     const long L0 = n[0] + 1, L1 = n[1] + 1, L2 = n[2] + 1, tot = L0 * L1 * L2;
-    // ping-pong buffers: offsets -> x cosine sums -> y -> z
-    double2* buf[2] = {nullptr, nullptr};
-    double* dct = nullptr;
-    try {
-        buf[0] = dalloc<double2>(tot);
-        buf[1] = dalloc<double2>(tot);
-        hipLaunchKernelGGL(k_green_offsets, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, n[0],
-                           n[1], n[2], d[0], d[1], d[2], c->desc.k0, buf[0]);
+    // ping-pong between gwork and Goff; both kept for later calls
+    if (!c->Goff) c->Goff = dalloc<double2>(tot);
+    if (!c->gwork) c->gwork = dalloc<double2>(tot);
+    const int nmax = std::max(n[0], std::max(n[1], n[2]));
+    if (!c->gdct) c->gdct = dalloc<double>(3 * 2 * nmax);
+    std::vector<double> ctab(3 * 2 * nmax);
+    for (int ax = 0; ax < 3; ++ax)
+        for (int j = 0; j < 2 * n[ax]; ++j) ctab[ax * 2 * nmax + j] = (double)cosl((long double)kPi * j / n[ax]);
+    copy_sync(c->gdct, ctab.data(), ctab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    double2* src = c->gwork;
+    double2* dst = c->Goff;
+    hipLaunchKernelGGL(k_green_offsets, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, n[0], n[1], n[2],
+                       d[0], d[1], d[2], c->desc.k0, src);
+    HIPCHK(hipGetLastError());
+    for (int ax = 0; ax < 3; ++ax) {
+        hipLaunchKernelGGL(k_green_cos, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, (int)L0, (int)L1,
+                           (int)L2, ax, c->gdct + ax * 2 * nmax, src, dst);
         HIPCHK(hipGetLastError());
-        const int nmax = std::max(n[0], std::max(n[1], n[2]));
-        dct = dalloc<double>(2 * nmax);
-        int cur = 0;
-        for (int ax = 0; ax < 3; ++ax) {
-            const int N = n[ax];
-            std::vector<double> ctab(2 * N);
-            for (int j = 0; j < 2 * N; ++j) ctab[j] = (double)cosl((long double)kPi * j / N);
-            HIPCHK(hipMemcpyAsync(dct, ctab.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_green_cos, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, (int)L0,
-                               (int)L1, (int)L2, ax, dct, buf[cur], buf[1 - cur]);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(c->stream));
-            cur = 1 - cur;
-        }
-        dfree(dct);
-        dfree(c->Goff);
-        c->Goff = buf[cur];
-        buf[cur] = nullptr;
-        dfree(buf[1 - cur]);
-    } catch (...) {
-        dfree(dct);
-        dfree(buf[0]);
-        dfree(buf[1]);
-        throw;
+        std::swap(src, dst);
     }
+    // offsets in gwork; x: gwork -> Goff, y: Goff -> gwork, z: gwork -> Goff
     c->green_ready = true;
     if (c->op64) HIPCHK(hipMemcpyAsync(c->op64->G, c->Goff, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
     if (c->op32) {
@@ -461,33 +470,16 @@ void green_compute(vie_ctx* c)
     HIPCHK(hipStreamSynchronize(c->stream));
 }
 
-// d_f (vie_fft/fft.h:405-588) from per-voxel eps, on the host.
-std::vector<std::complex<double>> diag_host(vie_ctx* c)
-{
-    const int* n = c->desc.n;
-    const long V = c->V;
-    const long st[3] = {(long)n[1] * n[2], n[2], 1};
-    std::vector<std::complex<double>> dg(3 * V);
-    const std::complex<double>* eps = (const std::complex<double>*)c->eps_h.data();
-    for (int f = 0; f < 3; ++f) {
-        const std::complex<double> cf = std::complex<double>(0, -1) * (c->desc.d[f] / 2.0) * kZ0 / c->desc.k0;
-        for (long v = 0; v < V; ++v) {
-            const int idx[3] = {(int)(v / st[0]), (int)((v / n[2]) % n[1]), (int)(v % n[2])};
-            std::complex<double> val = 0;
-            if (idx[f] < n[f] - 1) val = cf * (1.0 / (eps[v + st[f]] - 1.0) + 1.0 / (eps[v] - 1.0));
-            dg[f * V + v] = val;
-        }
-    }
-    return dg;
-}
-
 template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
 {
-    auto dg = diag_host(c);
-    std::vector<cx<R>> h(dg.size());
-    for (size_t i = 0; i < dg.size(); ++i) h[i] = hc<R>(dg[i]);
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
     if (!op->dg) op->dg = dalloc<cx<R>>(3 * c->V);
-    copy_sync(op->dg, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice, c->stream);
+    double cim[3];
+    for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / c->desc.k0;
+    hipLaunchKernelGGL(k_diag<R>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                       cim[2], c->eps_dev, op->dg);
+    HIPCHK(hipGetLastError());
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
 }
 
@@ -1096,6 +1088,8 @@ int vie_destroy(vie_ctx* c)
     delete c->op32;   // shares buffers owned by op64: delete first
     delete c->op64;
     dfree(c->Goff);
+    dfree(c->gwork);
+    dfree(c->gdct);
     dfree(c->eps_dev);
     dfree(c->mat_dev);
     dfree(c->state);
@@ -1134,8 +1128,7 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
                     throw Err(VIE_ERR_ARG, "eps = 1 on an active face (1/(eps-1) is infinite)");
             }
         }
-    dfree(c->eps_dev);
-    c->eps_dev = dalloc<double2>(c->V);
+    if (!c->eps_dev) c->eps_dev = dalloc<double2>(c->V);
     copy_sync(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     c->medium_ready = true;
     if (c->op64) upload_medium<double>(c, c->op64);
@@ -1172,6 +1165,28 @@ int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags
         c->eps_h.assign(eps, eps + 2 * c->V);
         dfree(c->mat_dev);
         return set_medium_common(c, flags);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_k0(vie_ctx* c, double k0)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        require(c, k0 > 0 && std::isfinite(k0), "k0 must be positive");
+        c->desc.k0 = k0;
+        if (c->op64) c->op64->set_k0(k0, c->stream);
+        if (c->op32) c->op32->set_k0(k0, c->stream);
+        c->green_ready = false;
+        c->rhs_ready = false;
+        c->sol_ready = false;
+        // the diagonal term's factor (Z0 / k0) changed: rebuild it
+        if (c->medium_ready) {
+            if (c->op64) upload_medium<double>(c, c->op64);
+            if (c->op32) upload_medium<float>(c, c->op32);
+        }
+        return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
     }
@@ -1260,6 +1275,37 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
         }
         copy_sync(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_rhs_incident(vie_ctx* c, double theta, double phi, double psi)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        require(c, c->medium_ready, "vie_set_medium has not been called");
+        const int* n = c->desc.n;
+        const double* d = c->desc.d;
+        const double k0 = c->desc.k0;
+        const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+        const double kv[3] = {k0 * std::sin(th) * std::cos(ph), k0 * std::sin(th) * std::sin(ph), k0 * std::cos(th)};
+        const double pol[3] = {std::cos(th) * std::cos(ph) * std::cos(ps) - std::sin(ph) * std::sin(ps),
+                               std::cos(th) * std::sin(ph) * std::cos(ps) + std::cos(ph) * std::sin(ps),
+                               -std::sin(th) * std::cos(ps)};
+        double2 seg[3];
+        for (int cpt = 0; cpt < 3; ++cpt) {
+            const double s = std::fabs(kv[cpt]) < kEps ? d[cpt] : 2.0 * std::sin(kv[cpt] * d[cpt] / 2.0) / kv[cpt];
+            seg[cpt] = make_double2(s * pol[cpt], 0.0);
+        }
+        Op<double>* op = make_op<double>(c);
+        hipLaunchKernelGGL(k_incident, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2],
+                           kv[0], kv[1], kv[2], seg[0], seg[1], seg[2], op->flags, c->b64);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->rhs_ready = true;
+        c->sol_ready = false;
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

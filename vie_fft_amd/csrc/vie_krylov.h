@@ -596,6 +596,62 @@ __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV
 }
 
 
+// ------------------------------------------------------------ set-up kernels
+// Diagonal face term d_f[v] = c_f (1/(eps[v + e_f] - 1) + 1/(eps[v] - 1)) for
+// v_f < N_f - 1, else 0 (LXX_DO/LYY_DO/LZZ_DO, vie_fft/fft.h:405-588);
+// c_f = j cim[f].
+template <class R>
+__global__ __launch_bounds__(NT) void k_diag(int n0, int n1, int n2, double c0, double c1, double c2,
+                                             const double2* __restrict__ eps, cx<R>* __restrict__ d)
+{
+    const long V = (long)n0 * n1 * n2;
+    const long st[3] = {(long)n1 * n2, n2, 1};
+    const int nn[3] = {n0, n1, n2};
+    const double cim[3] = {c0, c1, c2};
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int c[3] = {(int)(v / ((long)n1 * n2)), (int)((v / n2) % n1), (int)(v % n2)};
+        const double2 a = zdiv(make_double2(1, 0), make_double2(eps[v].x - 1.0, eps[v].y));
+        for (int f = 0; f < 3; ++f) {
+            double2 r = make_double2(0, 0);
+            if (c[f] < nn[f] - 1) {
+                const double2 e = eps[v + st[f]];
+                const double2 b = zdiv(make_double2(1, 0), make_double2(e.x - 1.0, e.y));
+                const double2 sm = make_double2(a.x + b.x, a.y + b.y);
+                r = make_double2(-cim[f] * sm.y, cim[f] * sm.x);  // j cim * sm
+            }
+            d[f * V + v] = mkc((R)r.x, (R)r.y);
+        }
+    }
+}
+
+// Incident_Wave (vie_fft/GreenFunction.h:144-248) masked by the flags
+// (vie_fft/CG_Iteration.h:559-562): b_c = Flag_c exp(-j k.r_c) seg_c, r_c the
+// face centre of component c, seg_c its pulse integral times the polarisation.
+__global__ __launch_bounds__(NT) void k_incident(int n0, int n1, int n2, double d0, double d1, double d2,
+                                                 double k0x, double k0y, double k0z, double2 s0, double2 s1,
+                                                 double2 s2, const uint8_t* __restrict__ flags,
+                                                 double2* __restrict__ b)
+{
+    const long V = (long)n0 * n1 * n2;
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), k = (int)(v % n2);
+        const double px = (i + 0.5) * d0, py = (j + 0.5) * d1, pz = (k + 0.5) * d2;
+        const double base = k0x * px + k0y * py + k0z * pz;
+        const double2 seg[3] = {s0, s1, s2};
+        const double shift[3] = {0.5 * d0 * k0x, 0.5 * d1 * k0y, 0.5 * d2 * k0z};
+        for (int c = 0; c < 3; ++c) {
+            const long F = c * V + v;
+            double2 r = make_double2(0, 0);
+            if (flags[F]) {
+                double sn, cs;
+                sincos(-(base + shift[c]), &sn, &cs);
+                r = zmul(make_double2(cs, sn), seg[c]);
+            }
+            b[F] = r;
+        }
+    }
+}
+
 // ---------------------------------------------------- absorbed power, gradient
 // Face (f, v) is active when flags_f[v] != 0 and v_f < N_f - 1; its absorbed
 // power is w |J|^2 with w = K Im(S), K = dV Z0 / (2 k0),

@@ -32,6 +32,7 @@ EXPORTS = [
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
     "vie_model_bytes_matvec", "vie_absorption", "vie_absorption_grad",
+    "vie_set_k0", "vie_set_rhs_incident",
 ]
 
 
@@ -91,6 +92,8 @@ def load():
     L.vie_model_bytes_matvec.argtypes = [vp]
     L.vie_model_bytes_matvec.restype = ct.c_double
     L.vie_absorption.argtypes = [vp, dp]
+    L.vie_set_k0.argtypes = [vp, ct.c_double]
+    L.vie_set_rhs_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
     L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
     _lib = L
     return L
@@ -162,6 +165,11 @@ class Solver:
         f = np.ascontiguousarray(flags, dtype=np.uint8).reshape((3,) + self.n)
         self._check(self.lib.vie_set_medium_eps(self.h, _dp(e), _ub(f)))
 
+    def set_k0(self, k0):
+        """Move the context to another wavenumber (then green_compute, set_rhs)."""
+        self._check(self.lib.vie_set_k0(self.h, float(k0)))
+        self.k0 = float(k0)
+
     def green_compute(self):
         """Green_Function_Compute (vie_fft/GreenFunction.h:8-141)."""
         self._check(self.lib.vie_green_compute(self.h))
@@ -199,6 +207,10 @@ class Solver:
     def set_rhs(self, b):
         b = self._field(b)
         self._check(self.lib.vie_set_rhs(self.h, _dp(b)))
+
+    def set_rhs_incident(self, theta=0.0, phi=0.0, psi=0.0):
+        """Resident right-hand side = masked incident field, computed on the device."""
+        self._check(self.lib.vie_set_rhs_incident(self.h, float(theta), float(phi), float(psi)))
 
     def solve_resident(self, method=BICGSTAB, tol=1e-3, maxit=0):
         method = METHODS.get(method, method) if isinstance(method, str) else method
