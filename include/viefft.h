@@ -15,6 +15,7 @@
  *   FFT_Program              vie_fft/CG_Iteration.h:1365-1388-> vie_solve
  *   FarField1                vie_fft/NearFar.h:16-109        -> vie_forward_scatter
  *   NearField (Ene_Metal)    vie_fft/NearFar.h:180-250       -> vie_absorbed
+ *   FarField2 (RCS cuts)     vie_fft/NearFar.h:302-456       -> vie_rcs, vie_far_field
  *
  * Data layout is the reference's: complex numbers are interleaved
  * {double re, double im} (MKL_Complex16, vie_fft/common.h:68-94), 3-D arrays
@@ -155,6 +156,18 @@ int vie_forward_scatter(vie_ctx* ctx, double theta, double phi, double psi, doub
 /* Ene_Metal of NearField (vie_fft/NearFar.h:180-250) for material id `mat`
    (needs vie_set_medium); times Z0 / area = Q_abs. */
 int vie_absorbed(vie_ctx* ctx, int mat, double* value);
+
+/* Far-field amplitudes of the resident solution for ndir directions (degrees):
+   S[3q + c] = dV exp(j k0 delta_c v_c / 2) sum_cells J_c exp(j k0 r.v), r the
+   cell centre, v the unit direction -- the sums of FarField1 and FarField2
+   (vie_fft/NearFar.h:59-96, 321-380); S holds ndir x 3 complex (interleaved
+   doubles).  One pass over J serves up to 1024 directions. */
+int vie_far_field(vie_ctx* ctx, int ndir, const double* theta_deg, const double* phi_deg, double* S);
+
+/* FarField2 (vie_fft/NearFar.h:302-456): RCS in dB relative to lambda^2 of the
+   E plane (phi_s = phi, theta component) and the H plane (phi_s = phi + 90,
+   phi component) for scattering angles 0..180 degrees (181 values each). */
+int vie_rcs(vie_ctx* ctx, double phi_deg, double* rcs_theta, double* rcs_phi);
 
 /* Absorbed power of the resident solution over every active face (flag set,
    own-axis index < N_f - 1): P = sum w_f |J_f|^2, w_f = dV Z0/(2 k0)

@@ -1578,6 +1578,103 @@ int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double
     }
 }
 
+// Far-field amplitudes S_c(v) = dV exp(j k0 delta_c v_c / 2) sum_cells J_c
+// exp(j k0 r.v) of the resident solution (FarField1/FarField2 sums,
+// vie_fft/NearFar.h:59-96, 321-380) for ndir directions; S holds ndir x 3 complex.
+static void far_field_dev(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg,
+                          std::complex<double>* S)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const double k0 = c->desc.k0;
+    std::vector<double> vh(3 * ndir);
+    for (int q = 0; q < ndir; ++q) {
+        const double th = kPi * theta_deg[q] / 180.0, ph = kPi * phi_deg[q] / 180.0;
+        vh[3 * q] = std::sin(th) * std::cos(ph);
+        vh[3 * q + 1] = std::sin(th) * std::sin(ph);
+        vh[3 * q + 2] = std::cos(th);
+    }
+    const long npen = (long)n[0] * n[1];
+    const unsigned nblk = (unsigned)std::min<long>(npen, 1024);
+    double* vdev = dalloc<double>(3 * ndir);
+    double2* part = nullptr;
+    double2* sums = nullptr;
+    try {
+        part = dalloc<double2>((long)nblk * 3 * ndir);
+        sums = dalloc<double2>(3 * ndir);
+        copy_sync(vdev, vh.data(), vh.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_farfield, dim3(nblk), dim3(NT), 3 * n[2] * sizeof(double2), c->stream, n[0], n[1], n[2],
+                           d[0], d[1], d[2], k0, ndir, vdev, c->x64, part);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_colsum, dim3((3 * ndir + NT - 1) / NT), dim3(NT), 0, c->stream, (long)nblk, 3 * ndir,
+                           part, sums);
+        HIPCHK(hipGetLastError());
+        std::vector<double2> h(3 * ndir);
+        copy_sync(h.data(), sums, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        const double dV = d[0] * d[1] * d[2];
+        for (int q = 0; q < ndir; ++q)
+            for (int cpt = 0; cpt < 3; ++cpt)
+                S[3 * q + cpt] = std::polar(1.0, k0 * 0.5 * d[cpt] * vh[3 * q + cpt]) *
+                                 std::complex<double>(h[cpt * ndir + q].x, h[cpt * ndir + q].y) * dV;
+    } catch (...) {
+        dfree(vdev);
+        dfree(part);
+        dfree(sums);
+        throw;
+    }
+    dfree(vdev);
+    dfree(part);
+    dfree(sums);
+}
+
+int vie_far_field(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg, double* S)
+{
+    if (!c || ndir < 0 || (ndir > 0 && (!theta_deg || !phi_deg || !S))) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, c->desc.n[2] <= 1024, "z extent too large for the far-field kernel");
+        for (int q0 = 0; q0 < ndir; q0 += FF_MAXD * NT) {
+            const int m = std::min(ndir - q0, FF_MAXD * NT);
+            far_field_dev(c, m, theta_deg + q0, phi_deg + q0, (std::complex<double>*)S + 3 * (long)q0);
+        }
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_rcs(vie_ctx* c, double phi_deg, double* rcs_theta, double* rcs_phi)
+{
+    if (!c || !rcs_theta || !rcs_phi) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        std::vector<double> th(362), ph(362);
+        for (int a = 0; a <= 180; ++a) {
+            th[a] = th[181 + a] = a;
+            ph[a] = phi_deg;          // E plane
+            ph[181 + a] = phi_deg + 90.0;  // H plane
+        }
+        std::vector<std::complex<double>> S(3 * 362);
+        far_field_dev(c, 362, th.data(), ph.data(), S.data());
+        const double k0 = c->desc.k0, lam = 2.0 * kPi / k0;
+        for (int a = 0; a <= 180; ++a) {
+            const double t = kPi * a / 180.0;
+            double p = kPi * phi_deg / 180.0;
+            const std::complex<double>* e = &S[3 * a];
+            const std::complex<double> mt = e[0] * (std::cos(t) * std::cos(p)) + e[1] * (std::cos(t) * std::sin(p)) -
+                                            e[2] * std::sin(t);
+            rcs_theta[a] = 10.0 * std::log10(k0 * k0 * kZ0 * kZ0 * std::norm(mt) / (4.0 * kPi) / lam / lam);
+            p = kPi * (phi_deg + 90.0) / 180.0;
+            const std::complex<double>* h = &S[3 * (181 + a)];
+            const std::complex<double> mp = h[0] * std::sin(p) - h[1] * std::cos(p);
+            rcs_phi[a] = 10.0 * std::log10(k0 * k0 * kZ0 * kZ0 * std::norm(mp) / (4.0 * kPi) / lam / lam);
+        }
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
 int vie_absorbed(vie_ctx* c, int mat, double* value)
 {
     if (!c || !value) return VIE_ERR_ARG;

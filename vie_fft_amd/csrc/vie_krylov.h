@@ -564,6 +564,88 @@ __global__ __launch_bounds__(NT) void k_farsum(int n0, int n1, int n2, double d0
     }
 }
 
+// Far-field sums for many directions in one pass over J (FarField2,
+// vie_fft/NearFar.h:302-456): per direction v, sum_cells J_c exp(j k0 r.v)
+// with r the cell centre.  Each block walks z-pencils (i, j); each thread owns
+// up to FF_MAXD directions and evaluates a pencil by Horner's rule in
+// w = exp(j k0 dz vz), then applies the pencil's phase.  Partials are laid out
+// [block][component][direction].
+constexpr int FF_MAXD = 4;
+__global__ __launch_bounds__(NT) void k_farfield(int n0, int n1, int n2, double d0, double d1, double d2, double k0,
+                                                 int ndir, const double* __restrict__ vdir,
+                                                 const double2* __restrict__ J, double2* __restrict__ partial)
+{
+    extern __shared__ double2 pen[];  // 3 * n2
+    const long V = (long)n0 * n1 * n2;
+    double2 acc[FF_MAXD][3];
+    double2 w[FF_MAXD];
+    double vx[FF_MAXD], vy[FF_MAXD], vz[FF_MAXD];
+#pragma unroll
+    for (int q = 0; q < FF_MAXD; ++q) {
+        const int dir = threadIdx.x + q * NT;
+        vx[q] = vy[q] = vz[q] = 0;
+        if (dir < ndir) {
+            vx[q] = vdir[3 * dir];
+            vy[q] = vdir[3 * dir + 1];
+            vz[q] = vdir[3 * dir + 2];
+        }
+        double sn, cs;
+        sincos(k0 * d2 * vz[q], &sn, &cs);
+        w[q] = make_double2(cs, sn);
+        for (int c = 0; c < 3; ++c) acc[q][c] = make_double2(0, 0);
+    }
+    const long npen = (long)n0 * n1;
+    for (long p = blockIdx.x; p < npen; p += gridDim.x) {
+        const int i = (int)(p / n1), j = (int)(p % n1);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 3 * n2; e += NT) {
+            const int c = e / n2, k = e % n2;
+            pen[e] = J[c * V + p * n2 + k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < FF_MAXD; ++q) {
+            if (threadIdx.x + q * NT >= ndir) break;
+            double2 s[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+            for (int k = n2 - 1; k >= 0; --k)
+                for (int c = 0; c < 3; ++c) {
+                    const double2 t = zmul(s[c], w[q]);
+                    const double2 jv = pen[c * n2 + k];
+                    s[c] = make_double2(t.x + jv.x, t.y + jv.y);
+                }
+            double sn, cs;
+            sincos(k0 * ((i + 0.5) * d0 * vx[q] + (j + 0.5) * d1 * vy[q] + 0.5 * d2 * vz[q]), &sn, &cs);
+            const double2 ph = make_double2(cs, sn);
+            for (int c = 0; c < 3; ++c) {
+                const double2 t = zmul(s[c], ph);
+                acc[q][c].x += t.x;
+                acc[q][c].y += t.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FF_MAXD; ++q) {
+        const int dir = threadIdx.x + q * NT;
+        if (dir >= ndir) break;
+        for (int c = 0; c < 3; ++c) partial[((long)blockIdx.x * 3 + c) * ndir + dir] = acc[q][c];
+    }
+}
+
+// Column sums of a [rows][cols] double2 array in row order (deterministic).
+__global__ __launch_bounds__(NT) void k_colsum(long rows, int cols, const double2* __restrict__ in,
+                                               double2* __restrict__ out)
+{
+    const int col = blockIdx.x * NT + threadIdx.x;
+    if (col >= cols) return;
+    double2 s = make_double2(0, 0);
+    for (long r = 0; r < rows; ++r) {
+        const double2 v = in[r * cols + col];
+        s.x += v.x;
+        s.y += v.y;
+    }
+    out[col] = s;
+}
+
 // Absorbed power of NearField (vie_fft/NearFar.h:186-250) for faces with
 // i,j,k < N-1 whose two cells both carry material `mat`.
 __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV, double k0,

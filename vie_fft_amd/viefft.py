@@ -32,7 +32,7 @@ EXPORTS = [
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
     "vie_model_bytes_matvec", "vie_absorption", "vie_absorption_grad",
-    "vie_set_k0", "vie_set_rhs_incident",
+    "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs",
 ]
 
 
@@ -94,6 +94,8 @@ def load():
     L.vie_absorption.argtypes = [vp, dp]
     L.vie_set_k0.argtypes = [vp, ct.c_double]
     L.vie_set_rhs_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
+    L.vie_far_field.argtypes = [vp, ct.c_int, dp, dp, dp]
+    L.vie_rcs.argtypes = [vp, ct.c_double, dp, dp]
     L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
     _lib = L
     return L
@@ -241,6 +243,23 @@ class Solver:
         v = ct.c_double()
         self._check(self.lib.vie_forward_scatter(self.h, theta, phi, psi, ct.byref(v)))
         return v.value
+
+    def far_field(self, theta_deg, phi_deg):
+        """(ndir, 3) complex far-field amplitudes of the resident solution."""
+        th = np.ascontiguousarray(theta_deg, dtype=np.float64).ravel()
+        ph = np.ascontiguousarray(phi_deg, dtype=np.float64).ravel()
+        if th.shape != ph.shape:
+            raise ValueError("theta and phi must have the same length")
+        S = np.zeros((th.size, 3), np.complex128)
+        self._check(self.lib.vie_far_field(self.h, th.size, _dp(th), _dp(ph), _dp(S)))
+        return S
+
+    def rcs(self, phi=0.0):
+        """FarField2 (vie_fft/NearFar.h:302-456): (E-plane, H-plane) RCS in dB, 181 angles each."""
+        rt = np.zeros(181)
+        rp = np.zeros(181)
+        self._check(self.lib.vie_rcs(self.h, float(phi), _dp(rt), _dp(rp)))
+        return rt, rp
 
     def absorbed(self, mat):
         """Ene_Metal of NearField (vie_fft/NearFar.h:180-250) for material id `mat`."""
