@@ -191,8 +191,9 @@ int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double*
 /* ---- instrumentation ----------------------------------------------------- */
 /* Per-launch HIP-event timing of one kernel class while solving:
    0 off, 1 z-forward, 2 y-forward, 3 x spectral pass, 4 y-inverse,
-   5 z-inverse of the FP64 operator; 11-15 the same passes of the FP32
-   operator.  vie_kernel_time returns launches and mean milliseconds
+   5 z-inverse of the FP64 operator, 6 FP64 vector update (linear
+   combination + dots), 7 solver scalar step; 11-16 the same kernels in FP32
+   (17 = 7).  vie_kernel_time returns launches and mean milliseconds
    (launches enqueued after convergence, which exit at once, are left out). */
 int vie_time_kernel(vie_ctx* ctx, int kernel_class);
 int vie_kernel_time(vie_ctx* ctx, int* launches, double* mean_ms);

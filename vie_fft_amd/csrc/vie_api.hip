@@ -389,15 +389,20 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 {
     const unsigned gsz = grid1((L.n + 1) / 2, NT, 2048);
     L.dots.partial = c->partial;
+    const int cls = sizeof(R) == 8 ? 6 : 16;
+    if (c->timer.cls == cls) c->timer.mark(c->stream);
     hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz), dim3(NT), 0, c->stream, L, done);
     HIPCHK(hipGetLastError());
+    if (c->timer.cls == cls) c->timer.mark(c->stream);
     return gsz;
 }
 
 void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)
 {
+    if (c->timer.cls == 7) c->timer.mark(c->stream);
     hipLaunchKernelGGL(k_scalar, dim3(1), dim3(NT), 0, c->stream, op, s, c->partial, nblk, nd, c->hist);
     HIPCHK(hipGetLastError());
+    if (c->timer.cls == 7) c->timer.mark(c->stream);
 }
 
 void state_init(vie_ctx* c, KState* s, double tol, int maxit)
@@ -1791,7 +1796,8 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
 
 int vie_time_kernel(vie_ctx* c, int kernel_class)
 {
-    if (!c || kernel_class < 0 || kernel_class > 15 || (kernel_class > 5 && kernel_class < 11)) return VIE_ERR_ARG;
+    if (!c || kernel_class < 0 || kernel_class > 17 || (kernel_class > 7 && kernel_class < 11)) return VIE_ERR_ARG;
+    if (kernel_class == 17) kernel_class = 7;
     c->timer.cls = kernel_class;
     c->timed_launches = 0;
     c->timed_ms = 0;

@@ -62,6 +62,7 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
     __shared__ double2 red[NT / 64];
     if (done && *done) return;
     T c[4], c2[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (L.coef) {
             c[q] = mkc((R)L.coef[q].x, (R)L.coef[q].y);
@@ -72,33 +73,56 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
         }
     }
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    auto dots_at = [&](long i, T v, T v2) {
-        for (int d = 0; d < L.dots.n; ++d) {
-            const int sa = L.dots.a[d], sb = L.dots.b[d];
-            const T a = sa == 0 ? v : sa == 1 ? L.dots.e1[i] : sa == 2 ? L.dots.e2[i] : v2;
-            const T b = sb == 0 ? v : sb == 1 ? L.dots.e1[i] : sb == 2 ? L.dots.e2[i] : v2;
-            dot_acc(acc[d], a, b, L.dots.nc[d]);
-        }
-    };
     const long stride = (long)gridDim.x * NT;
     // pairs of entries (16-byte accesses for FP32), then a possible odd tail
     const long npair = L.n / 2;
+    bool use1 = false, use2 = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= L.dots.n) break;
+        use1 = use1 || L.dots.a[d] == 1 || L.dots.b[d] == 1;
+        use2 = use2 || L.dots.a[d] == 2 || L.dots.b[d] == 2;
+    }
+    // operand selection by value (a conditional over lvalues would select an
+    // address and push the candidates to scratch memory)
+    auto pick = [](int sel, T v, T e1, T e2, T v2) {
+        T r = v2;
+        if (sel == 0) r = v;
+        if (sel == 1) r = e1;
+        if (sel == 2) r = e2;
+        return r;
+    };
     for (long i2 = (long)blockIdx.x * NT + threadIdx.x; i2 < npair; i2 += stride) {
         const long i = 2 * i2;
+        // every load of the pair first (more bytes in flight per thread)
+        T a0[4], a1[4], e10 = czero<T>(), e11 = czero<T>(), e20 = czero<T>(), e21 = czero<T>();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a0[q] = a1[q] = czero<T>();
+            if (L.x[q]) ld2(L.x[q] + i, a0[q], a1[q]);
+        }
+        if (use1) ld2(L.dots.e1 + i, e10, e11);
+        if (use2) ld2(L.dots.e2 + i, e20, e21);
         T v = czero<T>(), v2 = czero<T>(), w = czero<T>(), w2 = czero<T>();
-        for (int q = 0; q < 4; ++q)
-            if (L.x[q]) {
-                T a0, a1;
-                ld2(L.x[q] + i, a0, a1);
-                v = cadd(v, cmul(c[q], a0));
-                w = cadd(w, cmul(c[q], a1));
-                if (L.out2) {
-                    v2 = cadd(v2, cmul(c2[q], a0));
-                    w2 = cadd(w2, cmul(c2[q], a1));
-                }
-            }
-        dots_at(i, v, v2);
-        dots_at(i + 1, w, w2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v = cadd(v, cmul(c[q], a0[q]));
+            w = cadd(w, cmul(c[q], a1[q]));
+            v2 = cadd(v2, cmul(c2[q], a0[q]));
+            w2 = cadd(w2, cmul(c2[q], a1[q]));
+        }
+        if (L.dots.n > 0) {
+            dot_acc(acc[0], pick(L.dots.a[0], v, e10, e20, v2), pick(L.dots.b[0], v, e10, e20, v2), L.dots.nc[0]);
+            dot_acc(acc[0], pick(L.dots.a[0], w, e11, e21, w2), pick(L.dots.b[0], w, e11, e21, w2), L.dots.nc[0]);
+        }
+        if (L.dots.n > 1) {
+            dot_acc(acc[1], pick(L.dots.a[1], v, e10, e20, v2), pick(L.dots.b[1], v, e10, e20, v2), L.dots.nc[1]);
+            dot_acc(acc[1], pick(L.dots.a[1], w, e11, e21, w2), pick(L.dots.b[1], w, e11, e21, w2), L.dots.nc[1]);
+        }
+        if (L.dots.n > 2) {
+            dot_acc(acc[2], pick(L.dots.a[2], v, e10, e20, v2), pick(L.dots.b[2], v, e10, e20, v2), L.dots.nc[2]);
+            dot_acc(acc[2], pick(L.dots.a[2], w, e11, e21, w2), pick(L.dots.b[2], w, e11, e21, w2), L.dots.nc[2]);
+        }
         if (L.out) st2(L.out + i, v, w);
         if (L.out2) st2(L.out2 + i, v2, w2);
     }
@@ -111,11 +135,16 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
                 v = cadd(v, cmul(c[q], xv));
                 if (L.out2) v2 = cadd(v2, cmul(c2[q], xv));
             }
-        dots_at(i, v, v2);
+        const T e1 = use1 ? L.dots.e1[i] : czero<T>(), e2 = use2 ? L.dots.e2[i] : czero<T>();
+        if (L.dots.n > 0) dot_acc(acc[0], pick(L.dots.a[0], v, e1, e2, v2), pick(L.dots.b[0], v, e1, e2, v2), L.dots.nc[0]);
+        if (L.dots.n > 1) dot_acc(acc[1], pick(L.dots.a[1], v, e1, e2, v2), pick(L.dots.b[1], v, e1, e2, v2), L.dots.nc[1]);
+        if (L.dots.n > 2) dot_acc(acc[2], pick(L.dots.a[2], v, e1, e2, v2), pick(L.dots.b[2], v, e1, e2, v2), L.dots.nc[2]);
         if (L.out) L.out[i] = v;
         if (L.out2) L.out2[i] = v2;
     }
-    for (int d = 0; d < L.dots.n; ++d) {
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= L.dots.n) break;
         double2 s = block_sum(acc[d], red);
         if (threadIdx.x == 0) L.dots.partial[d * (long)gridDim.x + blockIdx.x] = s;
     }

@@ -73,6 +73,32 @@ template <class T> __device__ __forceinline__ void dot_acc(double2& acc, T a, T 
     acc.y += (double)a.x * (double)b.y - ay * (double)b.x;
 }
 
+// Dot-product operands: selector 0 is the value being produced, 1 is e1,
+// 2 is e2.  Operands are picked by value: a conditional over lvalues selects
+// an address and pushes the candidates to scratch memory.
+template <class R> __device__ __forceinline__ bool dots_use(const Dots<R>& d, int sel)
+{
+    bool u = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k < d.n && (d.a[k] == sel || d.b[k] == sel)) u = true;
+    return u;
+}
+template <class T> __device__ __forceinline__ T dsel(int sel, T v, T e1, T e2)
+{
+    T r = v;
+    if (sel == 1) r = e1;
+    if (sel == 2) r = e2;
+    return r;
+}
+template <class R>
+__device__ __forceinline__ void dots_add(double2 (&acc)[3], const Dots<R>& d, cx<R> v, cx<R> e1, cx<R> e2)
+{
+    if (d.n > 0) dot_acc(acc[0], dsel(d.a[0], v, e1, e2), dsel(d.b[0], v, e1, e2), d.nc[0]);
+    if (d.n > 1) dot_acc(acc[1], dsel(d.a[1], v, e1, e2), dsel(d.b[1], v, e1, e2), d.nc[1]);
+    if (d.n > 2) dot_acc(acc[2], dsel(d.a[2], v, e1, e2), dsel(d.b[2], v, e1, e2), d.nc[2]);
+}
+
 template <class R>
 __device__ __forceinline__ void load_coef(const double2* coef, cx<R>* c)
 {
@@ -413,7 +439,6 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
-    auto dval = [&](long idx) -> T { return dg.conj ? cconj(dg.d[idx]) : dg.d[idx]; };
     for (int e = threadIdx.x; e < B * N; e += NT) {
         const int col = e / N, t = 2 * (e % N);
         const long row = row0 + col;
@@ -427,16 +452,12 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     __syncthreads();
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    auto epi = [&](long idx, T out, T sv, T dv, uint8_t fl, T& yv) {
-        yv = czero<T>();
-        if (fl) yv = cadd(out, cmul(dv, sv));
-    };
-    auto dot1 = [&](long idx, T yv) {
-        for (int d = 0; d < dots.n; ++d) {
-            const T a = dots.a[d] == 0 ? yv : dots.a[d] == 1 ? dots.e1[idx] : dots.e2[idx];
-            const T b = dots.b[d] == 0 ? yv : dots.b[d] == 1 ? dots.e1[idx] : dots.e2[idx];
-            dot_acc(acc[d], a, b, dots.nc[d]);
-        }
+    const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
+    auto term = [&](long idx, T out) {  // y = flag (out + d x), then its dot products
+        const T dv = dg.conj ? cconj(dg.d[idx]) : dg.d[idx];
+        const T yv = flags[idx] ? cadd(out, cmul(dv, src[idx])) : czero<T>();
+        y[idx] = yv;
+        dots_add(acc, dots, yv, need1 ? dots.e1[idx] : czero<T>(), need2 ? dots.e2[idx] : czero<T>());
     };
     if constexpr (N % 2 == 0) {
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {
@@ -444,46 +465,23 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
             const long row = row0 + col;
             if (row < rows) {
                 const long idx = f * g.V + row * N + k;
-                const T o0 = cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k]));
-                const T o1 = cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1]));
-                T s0, s1, d0, d1, y0, y1;
-                ld2(src + idx, s0, s1);
-                if (dg.d) {
-                    ld2(dg.d + idx, d0, d1);
-                    if (dg.conj) {
-                        d0 = cconj(d0);
-                        d1 = cconj(d1);
-                    }
-                } else {
-                    const long vox = row * N + k;
-                    d0 = dval(idx);
-                    d1 = dval(idx + 1);
-                }
-                epi(idx, o0, s0, d0, flags[idx], y0);
-                epi(idx + 1, o1, s1, d1, flags[idx + 1], y1);
-                st2(y + idx, y0, y1);
-                dot1(idx, y0);
-                dot1(idx + 1, y1);
+                term(idx, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
+                term(idx + 1, cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1])));
             }
         }
     } else {
         for (int e = threadIdx.x; e < B * N; e += NT) {
             const int col = e / N, k = e % N;
             const long row = row0 + col;
-            if (row < rows) {
-                const long idx = f * g.V + row * N + k;
-                const T out = cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k]));
-                T yv;
-                epi(idx, out, src[idx], dval(idx), flags[idx], yv);
-                y[idx] = yv;
-                dot1(idx, yv);
-            }
+            if (row < rows) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
         }
     }
     if (dots.n > 0) {
         const long nblk = (long)gridDim.x * gridDim.y;
         const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
-        for (int d = 0; d < dots.n; ++d) {
+        #pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dots.n) break;
             double2 s = block_sum(acc[d], red);
             if (threadIdx.x == 0) dots.partial[d * nblk + blk] = s;
         }

@@ -211,19 +211,18 @@ __device__ __forceinline__ int fold_split(int t, int n)
 
 // grid (2Ny * ceil(2Nz / B)); N == Nx; in place on W2.
 // twP[e] = exp(-2 pi j e / P), e < P.  WPE: waves per SIMD the register
-// allocation aims for (0: compiler default).
-template <class R, int N, int T, class PL, int WPE>
+// allocation aims for (0: compiler default).  GM: how the G octant is read --
+// 0 per frequency from memory in the product, 1 a slice staged in LDS, 2 per
+// frequency into registers before the forward transform.
+template <class R, int N, int T, class PL, int WPE, int GM = (sizeof(R) == 8 ? 1 : 0)>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTabs<R> mt, const int* done)
 {
     using T_ = cx<R>;
-    using C = XCfg<R, N, T, PL, false, sizeof(R) == 4>;
+    using C = XCfg<R, N, T, PL, false, sizeof(R) == 4 && T == 32>;
     constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
-    // FP64: the G octant slice of the tile is staged in LDS (occupancy is
-    // register-bound there); FP32 reads G per frequency, which keeps LDS
-    // small enough for the register-bound occupancy.
-    constexpr bool GSL = sizeof(R) == 8;
+    constexpr bool GSL = GM == 1;
     __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
@@ -271,9 +270,20 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const T_ uy = mt.U[1][ty], vy = mt.Vv[1][ty];
     const int tz = colok ? z0 + col : z0;
     const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
-    // G octant column of this thread: G[fx][fold(ky)][fold(kz)], read in the product
+    // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
     const T_* gcol = mt.G + (long)fold_split(ty, g.ny) * (g.nz + 1) + fold_split(tz, g.nz);
     const long gs = (long)(g.ny + 1) * (g.nz + 1);
+    auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
+        const int kf = kx < P - kx ? kx : P - kx;
+        return (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
+    };
+    T_ gpre[GM == 2 ? E : 1];
+    if constexpr (GM == 2) {
+#pragma unroll
+        for (int b = 0; b < E / rL; ++b)
+#pragma unroll
+            for (int q = 0; q < rL; ++q) gpre[b * rL + q] = gcol[fxof(t + (P / E) * b + q * (P / rL)) * gs];
+    }
     __syncthreads();
     x_forward<T_, C, PL>(v, lds, tw, t, col);
 
@@ -285,13 +295,12 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
         for (int q = 0; q < rL; ++q) {
             const int kx = t + (P / E) * b + q * (P / rL);
-            const int kf = kx < P - kx ? kx : P - kx;
-            const int fx = (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
+            const int fx = fxof(kx);
             T_ Gv;
-            if constexpr (GSL) {
+            if constexpr (GM == 1) {
                 Gv = gsl[fx * B + col];
             } else {
-                Gv = gcol[fx * gs];
+                Gv = GM == 2 ? gpre[b * rL + q] : gcol[fx * gs];
                 if (mt.gconj) Gv = cconj(Gv);
             }
             const int e = b * rL + q;
@@ -448,20 +457,14 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
             for (int q = 0; q < rL; ++q)
                 v[b * rL + q] = rowok ? in[split_of<N>(t + T * b + q * (P / rL))] : czero<T_>();
     }
-    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
-    __syncthreads();
-    r_inverse<T_, C, PL>(v, lds, tw, t, col);
-    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    // epilogue operands of the kept rows n < N: loaded before the transform so
+    // their latency overlaps it
+    T_ sv[E], dv[E], e1v[E], e2v[E];
+    uint8_t fl[E];
+    const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
+    const bool e1src = dots.e1 == src;
+    const long base = f * g.V + (rowok ? row : 0) * N;
     if (rowok) {
-        const long base = f * g.V + row * N;
-        // epilogue operands of the kept rows n < N: all loads first
-        T_ sv[E], dv[E], e1v[E], e2v[E];
-        uint8_t fl[E];
-        const bool need1 = dots.n > 0 && (dots.a[0] == 1 || dots.b[0] == 1 || (dots.n > 1 && (dots.a[1] == 1 || dots.b[1] == 1)) ||
-                                          (dots.n > 2 && (dots.a[2] == 1 || dots.b[2] == 1)));
-        const bool need2 = dots.n > 0 && (dots.a[0] == 2 || dots.b[0] == 2 || (dots.n > 1 && (dots.a[1] == 2 || dots.b[1] == 2)) ||
-                                          (dots.n > 2 && (dots.a[2] == 2 || dots.b[2] == 2)));
-        const bool e1src = dots.e1 == src;
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {
             const int i = t + T * b;
@@ -478,6 +481,12 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                 }
             }
         }
+    }
+    for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
+    __syncthreads();
+    r_inverse<T_, C, PL>(v, lds, tw, t, col);
+    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    if (rowok) {
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {
             const int i = t + T * b;
@@ -489,12 +498,8 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                     T_ yv = czero<T_>();
                     if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
                     y[idx] = yv;
-                    const T_ o1 = e1src ? sv[e] : e1v[e];
-                    for (int d = 0; d < dots.n; ++d) {
-                        const T_ a = dots.a[d] == 0 ? yv : dots.a[d] == 1 ? o1 : e2v[e];
-                        const T_ bb = dots.b[d] == 0 ? yv : dots.b[d] == 1 ? o1 : e2v[e];
-                        dot_acc(acc[d], a, bb, dots.nc[d]);
-                    }
+                    dots_add(acc, dots, yv, e1src ? sv[e] : need1 ? e1v[e] : czero<T_>(),
+                             need2 ? e2v[e] : czero<T_>());
                 }
             }
         }
@@ -502,7 +507,9 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     if (dots.n > 0) {
         const long nblk = (long)gridDim.x * gridDim.y;
         const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
-        for (int d = 0; d < dots.n; ++d) {
+        #pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dots.n) break;
             double2 sm = block_sum(acc[d], red);
             if (threadIdx.x == 0) dots.partial[d * nblk + blk] = sm;
         }
