@@ -127,6 +127,19 @@ int vie_solve(vie_ctx* ctx, int method, double tol, int maxit, const double* b, 
 /* resident variant: b is uploaded once by vie_set_rhs and the solution stays
    on the device (vie_get_solution copies it out); nothing crosses PCIe inside. */
 int vie_set_rhs(vie_ctx* ctx, const double* b);
+/* Several right-hand sides on one operator (e.g. the polarisations of one
+   wavelength): B and X hold nrhs fields of 3 V complex back to back.
+   strategy VIE_BATCH_PLAIN solves each with `method` in the context's
+   precision, VIE_BATCH_MIXED with mixed-precision BiCGSTAB (reliable updates;
+   needs an FP64 context).  The solves run one after the other on the
+   context's stream with the operator, G~ and work buffers resident; infos
+   (may be NULL) receives nrhs records.  Returns VIE_OK, or the first
+   non-OK solver status (VIE_NOT_CONVERGED / VIE_BREAKDOWN) after finishing
+   every right-hand side, or a hard error at once. */
+typedef enum { VIE_BATCH_PLAIN = 0, VIE_BATCH_MIXED = 1 } vie_batch_strategy;
+int vie_solve_batched(vie_ctx* ctx, int nrhs, int strategy, int method, double tol, int maxit, const double* B,
+                      double* X, vie_info* infos);
+
 /* Resident right-hand side computed on the device: Incident_Wave
    (vie_fft/GreenFunction.h:144-248) for angles theta, phi, psi (degrees)
    masked by the flags (vie_fft/CG_Iteration.h:559-562); no host transfer. */

@@ -121,3 +121,19 @@ def test_mixed_rejects_bad_strategy():
             s.solve_mixed(tol=1e-6, strategy=7)
         with pytest.raises(VF.VieError):
             s.solve_mixed(tol=1e-6, strategy=VF.MIXED_RELIABLE, inner_method=VF.CORS)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_equals_single_solves(mixed):
+    """vie_solve_batched over the two polarisations = two single solves."""
+    sc, eps, flags, _ = _case(n=16, R=5.0, eps_phys=-3.0 + 0.6j)
+    with scene_solver(sc) as s:
+        B = np.stack([s.incident(0.0, 0.0, psi) for psi in (0.0, 90.0)])
+        X, infos = s.solve_batched(B, tol=1e-8, mixed=mixed)
+        for k in range(2):
+            s.set_rhs(B[k])
+            ref = s.solve_mixed(tol=1e-8) if mixed else s.solve_resident(VF.BICGSTAB, tol=1e-8)
+            assert infos[k]["iterations"] == ref["iterations"] and infos[k]["converged"]
+            assert np.array_equal(X[k], s.solution())
+        with pytest.raises(VF.VieError):  # unknown strategy
+            s._check(s.lib.vie_solve_batched(s.h, 1, 9, VF.BICGSTAB, 1e-6, 0, VF._dp(B), VF._dp(X), None))

@@ -400,7 +400,7 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)
 {
     if (c->timer.cls == 7) c->timer.mark(c->stream);
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(NT), 0, c->stream, op, s, c->partial, nblk, nd, c->hist);
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(NTS), 0, c->stream, op, s, c->partial, nblk, nd, c->hist);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == 7) c->timer.mark(c->stream);
 }
@@ -1059,6 +1059,8 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
     }
     if (!(desc->k0 > 0) || !std::isfinite(desc->k0)) return VIE_ERR_ARG;
     if (desc->precision != VIE_FP64 && desc->precision != VIE_FP32) return VIE_ERR_ARG;
+    // the pass kernels index the 3 x 4V spectral array with 32-bit offsets
+    if (12L * desc->n[0] * desc->n[1] * desc->n[2] >= (1L << 31)) return VIE_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VIE_ERR_DEVICE;
     vie_ctx* c = new vie_ctx();
@@ -1537,6 +1539,32 @@ int vie_get_solution(vie_ctx* c, double* x)
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int vie_solve_batched(vie_ctx* c, int nrhs, int strategy, int method, double tol, int maxit, const double* B,
+                      double* X, vie_info* infos)
+{
+    if (!c || nrhs < 0 || (nrhs > 0 && (!B || !X))) return VIE_ERR_ARG;
+    if (strategy != VIE_BATCH_PLAIN && strategy != VIE_BATCH_MIXED) {
+        c->err = "unknown batch strategy";
+        return VIE_ERR_ARG;
+    }
+    const long n2 = 6 * c->V;  // doubles per right-hand side
+    int worst = VIE_OK;
+    for (int k = 0; k < nrhs; ++k) {
+        vie_info info{};
+        int rc = vie_set_rhs(c, B + k * n2);
+        if (rc == VIE_OK)
+            rc = strategy == VIE_BATCH_MIXED
+                     ? vie_solve_mixed_resident(c, VIE_MIXED_RELIABLE, VIE_BICGSTAB, tol, 0.0, maxit, &info)
+                     : vie_solve_resident(c, method, tol, maxit, &info);
+        if (rc != VIE_OK && rc != VIE_NOT_CONVERGED && rc != VIE_BREAKDOWN) return rc;  // hard error
+        const int rg = vie_get_solution(c, X + k * n2);
+        if (rg != VIE_OK) return rg;
+        if (infos) infos[k] = info;
+        if (worst == VIE_OK) worst = rc;
+    }
+    return worst;
 }
 
 int vie_solve(vie_ctx* c, int method, double tol, int maxit, const double* b, double* x, vie_info* info)

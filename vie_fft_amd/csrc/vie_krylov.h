@@ -174,19 +174,58 @@ __device__ __forceinline__ bool zbad(double2 a)
 __device__ __forceinline__ double2 zc(double re, double im) { return make_double2(re, im); }
 
 // Sum nd partial arrays (each nblk long) in a fixed order into out[0..nd).
+// The scalar step runs as one workgroup of NTS threads.
+constexpr int NTS = 256;
+
+// out[d] = sum over blocks of part[d * nblk + block], d < nd (<= 4), in a fixed
+// order (strided per thread, xor-butterfly per wave, waves in index order).
 __device__ __forceinline__ void reduce_partials(const double2* part, long nblk, int nd, double2* out,
-                                                double2* red)
+                                                double2 (*red)[NTS / 64])
 {
-    for (int d = 0; d < nd; ++d) {
-        double2 a = make_double2(0, 0);
-        for (long i = threadIdx.x; i < nblk; i += NT) {
-            a.x += part[d * nblk + i].x;
-            a.y += part[d * nblk + i].y;
+    double2 a[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    long i = threadIdx.x;
+    for (; i + 3 * NTS < nblk; i += 4 * NTS) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            if (d >= nd) break;
+            double2 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] = part[d * nblk + i + u * NTS];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[d].x += p[u].x;
+                a[d].y += p[u].y;
+            }
         }
-        double2 s = block_sum(a, red);
-        if (threadIdx.x == 0) out[d] = s;
-        __syncthreads();
     }
+    for (; i < nblk; i += NTS) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            if (d >= nd) break;
+            a[d].x += part[d * nblk + i].x;
+            a[d].y += part[d * nblk + i].y;
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a[d].x += __shfl_xor(a[d].x, off);
+            a[d].y += __shfl_xor(a[d].y, off);
+        }
+        if (lane == 0) red[d][wid] = a[d];
+    }
+    __syncthreads();
+    if (threadIdx.x < nd) {
+        double2 t = make_double2(0, 0);
+        for (int w = 0; w < NTS / 64; ++w) {
+            t.x += red[threadIdx.x][w].x;
+            t.y += red[threadIdx.x][w].y;
+        }
+        out[threadIdx.x] = t;
+    }
+    __syncthreads();
 }
 
 // Scalar programs, one per solver step; v[] are the reduced dot products.
@@ -246,14 +285,34 @@ __device__ __forceinline__ void set8(double2* c, double2 a0, double2 a1, double2
     c[4] = b0; c[5] = b1; c[6] = b2; c[7] = b3;
 }
 
-__global__ __launch_bounds__(NT) void k_scalar(int op, KState* s, const double2* part, long nblk,
-                                                int nd, double* hist)
+__device__ void scalar_program(int op, KState* s, const double2* v, int nd, double* hist);
+
+// One solver scalar step.  The state is staged through LDS: every thread
+// loads one word of it alongside the partial sums (one memory latency for
+// both), thread 0 runs the step on the LDS copy, and every thread writes its
+// word back -- instead of a chain of dependent global accesses by one thread.
+__global__ __launch_bounds__(NTS) void k_scalar(int op, KState* gs, const double2* part, long nblk,
+                                                 int nd, double* hist)
 {
+    constexpr int W = sizeof(KState) / sizeof(unsigned long long);
+    static_assert(sizeof(KState) % sizeof(unsigned long long) == 0 && W <= NTS, "state words");
+    __shared__ KState sh;
     __shared__ double2 v[4];
-    __shared__ double2 red[NT / 64];
-    if (s->done) return;
+    __shared__ double2 red[4][NTS / 64];
+    unsigned long long* shw = reinterpret_cast<unsigned long long*>(&sh);
+    unsigned long long* gw = reinterpret_cast<unsigned long long*>(gs);
+    const unsigned long long mine = threadIdx.x < W ? gw[threadIdx.x] : 0ull;
     reduce_partials(part, nblk, nd, v, red);
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x < W) shw[threadIdx.x] = mine;
+    __syncthreads();
+    if (sh.done) return;
+    if (threadIdx.x == 0) scalar_program(op, &sh, v, nd, hist);
+    __syncthreads();
+    if (threadIdx.x < W) gw[threadIdx.x] = shw[threadIdx.x];
+}
+
+__device__ void scalar_program(int op, KState* s, const double2* v, int nd, double* hist)
+{
     for (int d = 0; d < nd; ++d) s->sums[d] = v[d];
     const double2 one = zc(1, 0), O = zc(0, 0);
     double2* cf = s->cf;

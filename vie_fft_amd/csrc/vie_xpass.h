@@ -234,9 +234,11 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
     const int col = threadIdx.x % B, t = threadIdx.x / B;
     const bool colok = z0 + col < P2;
-    const long rowst = (long)P1 * P2;  // stride between x rows
+    // element offsets fit 32 bits (the host checks 3 x 4V < 2^31 before
+    // launching), so the per-access address arithmetic stays 32-bit
+    const unsigned rowst = (unsigned)P1 * P2;  // stride between x rows
     T_* base = W2 + (long)ty * P2 + z0 + col;
-    const long cst = (long)g.nx * rowst;  // stride between components
+    const unsigned cst = (unsigned)g.nx * rowst;  // stride between components
 
     // issue the pencil loads first, then the tables, so their latencies overlap
     T_ v[3][E];
@@ -248,7 +250,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1);
-                v[c][b * r1 + q] = (n < N && colok) ? base[c * cst + n * rowst] : czero<T_>();
+                v[c][b * r1 + q] = (n < N && colok) ? base[c * cst + (unsigned)n * rowst] : czero<T_>();
             }
         }
     if constexpr (GSL) {
@@ -272,7 +274,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
     // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
     const T_* gcol = mt.G + (long)fold_split(ty, g.ny) * (g.nz + 1) + fold_split(tz, g.nz);
-    const long gs = (long)(g.ny + 1) * (g.nz + 1);
+    const unsigned gs = (unsigned)(g.ny + 1) * (g.nz + 1);
     auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
         const int kf = kx < P - kx ? kx : P - kx;
         return (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
@@ -282,7 +284,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
         for (int b = 0; b < E / rL; ++b)
 #pragma unroll
-            for (int q = 0; q < rL; ++q) gpre[b * rL + q] = gcol[fxof(t + (P / E) * b + q * (P / rL)) * gs];
+            for (int q = 0; q < rL; ++q) gpre[b * rL + q] = gcol[(unsigned)fxof(t + (P / E) * b + q * (P / rL)) * gs];
     }
     __syncthreads();
     x_forward<T_, C, PL>(v, lds, tw, t, col);
@@ -300,7 +302,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             if constexpr (GM == 1) {
                 Gv = gsl[fx * B + col];
             } else {
-                Gv = GM == 2 ? gpre[b * rL + q] : gcol[fx * gs];
+                Gv = GM == 2 ? gpre[b * rL + q] : gcol[(unsigned)fx * gs];
                 if (mt.gconj) Gv = cconj(Gv);
             }
             const int e = b * rL + q;
@@ -322,7 +324,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
                 for (int q = 0; q < r1; ++q) {
                     const int n = i + q * (P / r1);
-                    if (n < N) base[c * cst + n * rowst] = v[c][b * r1 + q];
+                    if (n < N) base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
                 }
             }
     }

@@ -32,7 +32,7 @@ EXPORTS = [
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
     "vie_model_bytes_matvec", "vie_absorption", "vie_absorption_grad",
-    "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs",
+    "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
 ]
 
 
@@ -96,6 +96,8 @@ def load():
     L.vie_set_rhs_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
     L.vie_far_field.argtypes = [vp, ct.c_int, dp, dp, dp]
     L.vie_rcs.argtypes = [vp, ct.c_double, dp, dp]
+    L.vie_solve_batched.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_int, dp, dp,
+                                    ct.POINTER(Info)]
     L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
     _lib = L
     return L
@@ -205,6 +207,17 @@ class Solver:
         self._check(self.lib.vie_solve(self.h, int(method), float(tol), int(maxit), _dp(b), _dp(x),
                                        ct.byref(info)), allow=(OK, NOT_CONVERGED))
         return x, info.as_dict()
+
+    def solve_batched(self, B, method=BICGSTAB, tol=1e-3, maxit=0, mixed=False):
+        """Solve for several right-hand sides B (nrhs, 3, Nx, Ny, Nz): returns (X, infos)."""
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        B = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).reshape((-1, 3) + self.n))
+        X = np.zeros_like(B)
+        infos = (Info * max(B.shape[0], 1))()
+        self._check(self.lib.vie_solve_batched(self.h, B.shape[0], int(bool(mixed)), int(method), float(tol),
+                                               int(maxit), _dp(B), _dp(X), infos),
+                    allow=(OK, NOT_CONVERGED, BREAKDOWN))
+        return X, [infos[k].as_dict() for k in range(B.shape[0])]
 
     def set_rhs(self, b):
         b = self._field(b)
