@@ -33,7 +33,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # HBM bytes per launch of the FP32 x pass at 128^3, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of known size):
 # profiles/k_xreg_f32_128cube_hbm_traffic.md
-XPASS_F32_128_TRAFFIC = 443.1e6
+XPASS_F32_128_TRAFFIC = 437.5e6
 
 
 def model_bytes(V, oct_, s):
@@ -153,7 +153,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": XPASS_F32_128_TRAFFIC if n == (128, 128, 128) else None,
-                     "kernel": "k_xreg<float,128,32,XPlan<8,8,4>> (x-axis forward + spectral product + inverse)",
+                     "kernel": "k_xreg<float,128,16,XPlan<16,16>,0,2> (x-axis forward + spectral product + inverse)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }
     if not args.no_cpu:
