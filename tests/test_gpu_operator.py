@@ -78,6 +78,8 @@ def test_incident_matches_oracle():
 
 
 def test_unsupported_size_is_an_error():
-    assert not VF.supported_n(7)
+    assert not VF.supported_n(7)  # no compiled kernel: accepted, embedded in 8
     with pytest.raises(VF.VieError):
-        VF.Solver((7, 8, 8), (0.02,) * 3, 1.0)
+        VF.Solver((300, 8, 8), (0.02,) * 3, 1.0)
+    with pytest.raises(VF.VieError):
+        VF.Solver((0, 8, 8), (0.02,) * 3, 1.0)

@@ -34,6 +34,7 @@ namespace {
 const double kPi = 3.1415926535897932384626433832795;
 const double kZ0 = 120.0 * kPi;  // vie_fft/common.h:18
 const double kEps = 1.0e-7;      // vie_fft/common.h:16
+const double kEpsBackground = 1.1;  // dummy background permittivity, vie_fft/Model.h:82
 
 struct Err : std::runtime_error {
     int code;
@@ -312,6 +313,9 @@ template <class R> Dots<R> nodots()
 
 // ---------------------------------------------------------------------- ctx
 struct vie_ctx {
+    int nu[3] = {0, 0, 0};  // grid as the caller sees it (desc.n: the compiled grid it is embedded in)
+    long Vu = 0;
+    bool emb = false;         // nu != desc.n
     vie_desc desc{};
     hipStream_t stream = nullptr;
     std::string err;
@@ -1044,6 +1048,55 @@ template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjo
 
 }  // namespace
 
+// Caller-grid <-> compiled-grid copies on the host: the caller's grid sits at
+// the origin of the compiled one, every other cell takes `fill`.
+template <class T> void embed_h(const vie_ctx* c, const T* in, T* out, int ncomp, T fill)
+{
+    const int* n = c->desc.n;
+    const int* u = c->nu;
+    for (int cp = 0; cp < ncomp; ++cp)
+        for (int i = 0; i < n[0]; ++i)
+            for (int j = 0; j < n[1]; ++j) {
+                T* o = out + cp * c->V + ((long)i * n[1] + j) * n[2];
+                if (i < u[0] && j < u[1]) {
+                    const T* src = in + cp * c->Vu + ((long)i * u[1] + j) * u[2];
+                    std::copy(src, src + u[2], o);
+                    std::fill(o + u[2], o + n[2], fill);
+                } else {
+                    std::fill(o, o + n[2], fill);
+                }
+            }
+}
+template <class T> void crop_h(const vie_ctx* c, const T* in, T* out, int ncomp)
+{
+    const int* n = c->desc.n;
+    const int* u = c->nu;
+    for (int cp = 0; cp < ncomp; ++cp)
+        for (int i = 0; i < u[0]; ++i)
+            for (int j = 0; j < u[1]; ++j) {
+                const T* src = in + cp * c->V + ((long)i * n[1] + j) * n[2];
+                std::copy(src, src + u[2], out + cp * c->Vu + ((long)i * u[1] + j) * u[2]);
+            }
+}
+using cd = std::complex<double>;
+
+// On the caller's grid the own-axis last plane of each component is outside
+// the operator (vie_fft/CG_Iteration.h:59-63); embedded, that plane becomes
+// interior, so it must carry no active face (the 3-point flag rule never
+// activates it).
+void check_last_planes(const vie_ctx* c, const unsigned char* flags)
+{
+    const int* u = c->nu;
+    for (int f = 0; f < 3; ++f)
+        for (int i = 0; i < u[0]; ++i)
+            for (int j = 0; j < u[1]; ++j)
+                for (int k = 0; k < u[2]; ++k) {
+                    const int idx[3] = {i, j, k};
+                    if (idx[f] == u[f] - 1 && flags[f * c->Vu + ((long)i * u[1] + j) * u[2] + k])
+                        throw Err(VIE_ERR_ARG, "active face on an own-axis last plane of an embedded grid");
+                }
+}
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -1053,32 +1106,46 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
 {
     if (!out || !desc) return VIE_ERR_ARG;
     *out = nullptr;
+    // a length without compiled pass kernels runs embedded in the next
+    // compiled length: the padded period 2n' >= 2n gives the same operator on
+    // the caller's cells, and the extra cells are inactive background
+    int nint[3];
     for (int a = 0; a < 3; ++a) {
-        if (!size_supported(desc->n[a])) return VIE_ERR_UNSUPPORTED;
+        if (desc->n[a] < 1) return VIE_ERR_ARG;
+        nint[a] = desc->n[a];
+        while (nint[a] <= 256 && !size_supported(nint[a])) ++nint[a];
+        if (nint[a] > 256) return VIE_ERR_UNSUPPORTED;
         if (!(desc->d[a] > 0) || !std::isfinite(desc->d[a])) return VIE_ERR_ARG;
     }
     if (!(desc->k0 > 0) || !std::isfinite(desc->k0)) return VIE_ERR_ARG;
     if (desc->precision != VIE_FP64 && desc->precision != VIE_FP32) return VIE_ERR_ARG;
     // the pass kernels index the 3 x 4V spectral array with 32-bit offsets
-    if (12L * desc->n[0] * desc->n[1] * desc->n[2] >= (1L << 31)) return VIE_ERR_UNSUPPORTED;
+    if (12L * nint[0] * nint[1] * nint[2] >= (1L << 31)) return VIE_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VIE_ERR_DEVICE;
     vie_ctx* c = new vie_ctx();
     c->desc = *desc;
-    c->V = (long)desc->n[0] * desc->n[1] * desc->n[2];
+    for (int a = 0; a < 3; ++a) {
+        c->nu[a] = desc->n[a];
+        c->desc.n[a] = nint[a];
+        c->emb = c->emb || nint[a] != desc->n[a];
+    }
+    c->V = (long)nint[0] * nint[1] * nint[2];
+    c->Vu = (long)desc->n[0] * desc->n[1] * desc->n[2];
     try {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->state = dalloc<KState>(1);
         c->aux = dalloc<KState>(1);
         c->hist = dalloc<double>(MAX_HIST);
-        ensure_partial(c, 3 * std::max<long>(std::max(zinv_blocks<double>(Geo{desc->n[0], desc->n[1], desc->n[2], c->V}),
-                                                      zinv_blocks<float>(Geo{desc->n[0], desc->n[1], desc->n[2], c->V})),
+        const int* ni = c->desc.n;
+        ensure_partial(c, 3 * std::max<long>(std::max(zinv_blocks<double>(Geo{ni[0], ni[1], ni[2], c->V}),
+                                                      zinv_blocks<float>(Geo{ni[0], ni[1], ni[2], c->V})),
                                              4096));
         c->b64 = dalloc<double2>(3 * c->V);
         c->x64 = dalloc<double2>(3 * c->V);
         c->t64 = dalloc<double2>(3 * c->V);
         c->r64 = dalloc<double2>(3 * c->V);
-        if (desc->precision == VIE_FP64) make_op<double>(c); else make_op<float>(c);
+        if (c->desc.precision == VIE_FP64) make_op<double>(c); else make_op<float>(c);
     } catch (const Err& e) {
         int code = e.code;
         vie_destroy(c);
@@ -1147,6 +1214,20 @@ int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_
                    const unsigned char* flags)
 {
     if (!c || !material || !eps_table || !flags || n_mat < 1) return VIE_ERR_ARG;
+    if (c->emb) {  // background (id 0) and inactive faces on the embedding cells
+        try {
+            check_last_planes(c, flags);
+        } catch (const Err& e) {
+            return fail(c, e);
+        }
+        std::vector<unsigned char> m(c->V), f(3 * c->V);
+        embed_h<unsigned char>(c, material, m.data(), 1, 0);
+        embed_h<unsigned char>(c, flags, f.data(), 3, 0);
+        c->emb = false;
+        const int rc = vie_set_medium(c, m.data(), eps_table, n_mat, f.data());
+        c->emb = true;
+        return rc;
+    }
     try {
         c->eps_h.resize(2 * c->V);
         for (long v = 0; v < c->V; ++v) {
@@ -1168,6 +1249,21 @@ int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_
 int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags)
 {
     if (!c || !eps || !flags) return VIE_ERR_ARG;
+    if (c->emb) {  // dummy background permittivity (vie_fft/Model.h:82) and inactive faces outside
+        try {
+            check_last_planes(c, flags);
+        } catch (const Err& e) {
+            return fail(c, e);
+        }
+        std::vector<cd> e(c->V);
+        std::vector<unsigned char> f(3 * c->V);
+        embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
+        embed_h<unsigned char>(c, flags, f.data(), 3, 0);
+        c->emb = false;
+        const int rc = vie_set_medium_eps(c, (const double*)e.data(), f.data());
+        c->emb = true;
+        return rc;
+    }
     try {
         c->eps_h.assign(eps, eps + 2 * c->V);
         dfree(c->mat_dev);
@@ -1215,7 +1311,7 @@ int vie_green_compute(vie_ctx* c)
 int vie_incident(vie_ctx* c, double theta, double phi, double psi, double* E)
 {
     if (!c || !E) return VIE_ERR_ARG;
-    const int* n = c->desc.n;
+    const int* n = c->nu;  // the caller's grid
     const double* d = c->desc.d;
     const double k0 = c->desc.k0;
     const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
@@ -1239,7 +1335,7 @@ int vie_incident(vie_ctx* c, double theta, double phi, double psi, double* E)
                     double pos[3] = {(i + 0.5) * d[0], (j + 0.5) * d[1], (k + 0.5) * d[2]};
                     pos[cpt] += 0.5 * d[cpt];
                     const double a = kv[0] * pos[0] + kv[1] * pos[1] + kv[2] * pos[2];
-                    out[cpt * c->V + ((long)i * n[1] + j) * n[2] + k] = std::polar(1.0, -a) * seg;
+                    out[cpt * c->Vu + ((long)i * n[1] + j) * n[2] + k] = std::polar(1.0, -a) * seg;
                 }
     }
     return VIE_OK;
@@ -1255,10 +1351,33 @@ int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* strea
     try {
         check_ready(c);
         hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-        if (c->desc.precision == VIE_FP64)
-            apply_dev<double>(c, (const double2*)x, (double2*)y, adjoint, st);
-        else
-            apply_dev<float>(c, (const float2*)x, (float2*)y, adjoint, st);
+        const int* n = c->desc.n;
+        const int* u = c->nu;
+        const unsigned ge = grid1(3 * c->V), gc = grid1(3 * c->Vu);
+        if (c->desc.precision == VIE_FP64) {
+            if (!c->emb) {
+                apply_dev<double>(c, (const double2*)x, (double2*)y, adjoint, st);
+            } else {
+                hipLaunchKernelGGL(k_embed<double2>, dim3(ge), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
+                                   (const double2*)x, c->t64);
+                apply_dev<double>(c, c->t64, c->r64, adjoint, st);
+                hipLaunchKernelGGL(k_crop<double2>, dim3(gc), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
+                                   c->r64, (double2*)y);
+            }
+        } else {
+            if (!c->emb) {
+                apply_dev<float>(c, (const float2*)x, (float2*)y, adjoint, st);
+            } else {
+                float2* xs = vec<float>(c, VT);
+                float2* ys = vec<float>(c, VT + 1);
+                hipLaunchKernelGGL(k_embed<float2>, dim3(ge), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
+                                   (const float2*)x, xs);
+                apply_dev<float>(c, xs, ys, adjoint, st);
+                hipLaunchKernelGGL(k_crop<float2>, dim3(gc), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3, ys,
+                                   (float2*)y);
+            }
+        }
+        HIPCHK(hipGetLastError());
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -1271,6 +1390,13 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
     try {
         check_ready(c);
         const long n = 3 * c->V;
+        std::vector<cd> xe, ye;
+        if (c->emb) {
+            xe.resize(n);
+            ye.resize(n);
+            embed_h<cd>(c, (const cd*)x, xe.data(), 3, cd(0, 0));
+            x = (const double*)xe.data();
+        }
         copy_sync(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         if (c->desc.precision == VIE_FP64) {
             apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
@@ -1281,7 +1407,12 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             apply_dev<float>(c, xs, ys, adjoint, c->stream);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
         }
-        copy_sync(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        if (c->emb) {
+            copy_sync(ye.data(), c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+            crop_h<cd>(c, ye.data(), (cd*)y, 3);
+        } else {
+            copy_sync(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        }
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -1325,8 +1456,14 @@ int vie_set_rhs(vie_ctx* c, const double* b)
     try {
         require(c, c->medium_ready, "vie_set_medium has not been called");
         const long n = 3 * c->V;
-        for (long i = 0; i < 2 * n; ++i)
+        for (long i = 0; i < 6 * c->Vu; ++i)
             if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
+        std::vector<cd> be;
+        if (c->emb) {
+            be.resize(n);
+            embed_h<cd>(c, (const cd*)b, be.data(), 3, cd(0, 0));
+            b = (const double*)be.data();
+        }
         copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         Op<double>* op = make_op<double>(c);
         // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
@@ -1534,7 +1671,13 @@ int vie_get_solution(vie_ctx* c, double* x)
     if (!c || !x) return VIE_ERR_ARG;
     try {
         require(c, c->sol_ready, "no solution on the device");
-        copy_sync(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        if (c->emb) {
+            std::vector<cd> xe(3 * c->V);
+            copy_sync(xe.data(), c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+            crop_h<cd>(c, xe.data(), (cd*)x, 3);
+        } else {
+            copy_sync(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        }
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -1549,7 +1692,7 @@ int vie_solve_batched(vie_ctx* c, int nrhs, int strategy, int method, double tol
         c->err = "unknown batch strategy";
         return VIE_ERR_ARG;
     }
-    const long n2 = 6 * c->V;  // doubles per right-hand side
+    const long n2 = 6 * c->Vu;  // doubles per right-hand side (caller's grid)
     int worst = VIE_OK;
     for (int k = 0; k < nrhs; ++k) {
         vie_info info{};
@@ -1717,8 +1860,9 @@ int vie_absorbed(vie_ctx* c, int mat, double* value)
         const int* n = c->desc.n;
         const double* d = c->desc.d;
         const unsigned gsz = grid1(c->V, NT, 1024);
-        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0] * d[1] * d[2],
-                           c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat, c->partial);
+        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[0], c->nu[1],
+                           c->nu[2], d[0] * d[1] * d[2], c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat,
+                           c->partial);
         HIPCHK(hipGetLastError());
         std::vector<double2> h(gsz);
         HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
@@ -1799,8 +1943,10 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
                            gd, gd + c->V);
         HIPCHK(hipGetLastError());
         try {
-            copy_sync(grad_re, gd, c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-            if (grad_im) copy_sync(grad_im, gd + c->V, c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+            std::vector<double> h(2 * c->V);
+            copy_sync(h.data(), gd, 2 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+            crop_h<double>(c, h.data(), grad_re, 1);
+            if (grad_im) crop_h<double>(c, h.data() + c->V, grad_im, 1);
         } catch (...) {
             dfree(gd);
             throw;

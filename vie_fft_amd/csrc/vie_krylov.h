@@ -736,7 +736,9 @@ __global__ __launch_bounds__(NT) void k_colsum(long rows, int cols, const double
 
 // Absorbed power of NearField (vie_fft/NearFar.h:186-250) for faces with
 // i,j,k < N-1 whose two cells both carry material `mat`.
-__global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV, double k0,
+// (b0, b1, b2): the caller's grid, whose N - 1 bounds the loops (the compiled
+// grid n may be larger).
+__global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, int b0, int b1, int b2, double dV, double k0,
                                                double z0, const double2* J, const double2* eps,
                                                const uint8_t* mat, int mid, double2* partial)
 {
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV
     double acc = 0;
     for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
         const int k = v % n2, j = (v / n2) % n1, i = v / ((long)n1 * n2);
-        if (i >= n0 - 1 || j >= n1 - 1 || k >= n2 - 1) continue;
+        if (i >= b0 - 1 || j >= b1 - 1 || k >= b2 - 1) continue;
         for (int f = 0; f < 3; ++f) {
             const long w = v + st[f];
             if (!(mat[w] == mid && mat[v] == mid)) continue;
@@ -765,6 +767,36 @@ __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, double dV
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+
+// ------------------------------------------------- user grid <-> compiled grid
+// A grid whose axis lengths have no compiled pass kernel is embedded at the
+// origin of the next compiled lengths; the extra cells are inactive
+// background.  k_embed writes ncomp fields of the compiled grid (n) from the
+// user grid (u), zero outside; k_crop copies the user cells back.
+template <class T>
+__global__ __launch_bounds__(NT) void k_embed(int n0, int n1, int n2, int u0, int u1, int u2, int ncomp,
+                                              const T* __restrict__ in, T* __restrict__ out)
+{
+    const long V = (long)n0 * n1 * n2, Vu = (long)u0 * u1 * u2;
+    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * V; e += (long)gridDim.x * NT) {
+        const long c = e / V, v = e % V;
+        const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), k = (int)(v % n2);
+        T r{};
+        if (i < u0 && j < u1 && k < u2) r = in[c * Vu + ((long)i * u1 + j) * u2 + k];
+        out[e] = r;
+    }
+}
+template <class T>
+__global__ __launch_bounds__(NT) void k_crop(int n0, int n1, int n2, int u0, int u1, int u2, int ncomp,
+                                             const T* __restrict__ in, T* __restrict__ out)
+{
+    const long V = (long)n0 * n1 * n2, Vu = (long)u0 * u1 * u2;
+    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * Vu; e += (long)gridDim.x * NT) {
+        const long c = e / Vu, v = e % Vu;
+        const int i = (int)(v / ((long)u1 * u2)), j = (int)((v / u2) % u1), k = (int)(v % u2);
+        out[e] = in[c * V + ((long)i * n1 + j) * n2 + k];
+    }
+}
 
 // ------------------------------------------------------------ set-up kernels
 // Diagonal face term d_f[v] = c_f (1/(eps[v + e_f] - 1) + 1/(eps[v] - 1)) for

@@ -64,6 +64,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run vie_fft_amd/build.py (hipcc, gfx950) first")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7);
+    # only the first one loaded is used by the process, and torch cannot run
+    # on another release's runtime.  Load torch's first when it is installed
+    # so that torch and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ct.CDLL(LIB_PATH)
     dp = ct.POINTER(ct.c_double)
     vp = ct.c_void_p
