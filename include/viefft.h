@@ -76,11 +76,18 @@ typedef struct {
 } vie_info;
 
 /* ---- context ---------------------------------------------------------- */
+/* Axis lengths 1..256 are accepted.  A length without compiled pass kernels
+   runs embedded at the origin of the next compiled length with background
+   cells (eps 1.1, no active face) around it; every host array keeps the
+   caller's n, and results are those of the caller's grid (the padded period
+   only grows).  Such grids need no active face on a component's own-axis last
+   plane (the 3-point flag rule never makes one).  Longer axes:
+   VIE_ERR_UNSUPPORTED. */
 int vie_create(vie_ctx** ctx, const vie_desc* desc);
 int vie_destroy(vie_ctx* ctx);
 const char* vie_last_error(const vie_ctx* ctx);
-/* Supported axis lengths are products of 2, 3 and 5 from 2 to 256 (the FFT
-   sub-length is the axis length; the padded length is twice it). */
+/* 1 when n has compiled pass kernels (products of 2, 3 and 5 from a fixed list
+   up to 256; the padded transform length is 2n), 0 when n runs embedded. */
 int vie_supported_n(int n);
 
 /* ---- medium ------------------------------------------------------------- */
