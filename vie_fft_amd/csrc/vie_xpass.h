@@ -101,13 +101,13 @@ __device__ __forceinline__ void xi_stage(T_* v, const T_* tw, int t)
     }
 }
 
-// exchange after forward stage s: write its outputs, read stage s+1 inputs
+// exchange after forward stage s: write its outputs (xf_put), then -- after
+// a barrier -- read stage s+1 inputs (xf_get)
 template <class T_, class C, class PL, int s>
-__device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
+__device__ __forceinline__ void xf_put(const T_* v, T_* lds, int t, int col)
 {
-    constexpr int r = PL::r[s], S = PL::stride(s), r2 = PL::r[s + 1], P = C::P, E = C::E, T = P / E;
+    constexpr int r = PL::r[s], S = PL::stride(s), P = C::P, E = C::E, T = P / E;
     constexpr bool wsep = (S * r) % C::M == 0 && (S % C::M == 0 || C::M % S == 0);
-    constexpr bool rsep = (P / r2) % C::M == 0;
 #pragma unroll
     for (int b = 0; b < E / r; ++b) {
         const int i = t + T * b, k = i % S, j = (i - k) * r + k;
@@ -120,7 +120,12 @@ __device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
             for (int q = 0; q < r; ++q) lds[C::at(j + q * S, col)] = v[b * r + q];
         }
     }
-    __syncthreads();
+}
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xf_get(T_* v, const T_* lds, int t, int col)
+{
+    constexpr int r2 = PL::r[s + 1], P = C::P, E = C::E, T = P / E;
+    constexpr bool rsep = (P / r2) % C::M == 0;
 #pragma unroll
     for (int b = 0; b < E / r2; ++b) {
         const int i = t + T * b;
@@ -133,18 +138,23 @@ __device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
             for (int q = 0; q < r2; ++q) v[b * r2 + q] = lds[C::at(i + q * (P / r2), col)];
         }
     }
+}
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
+{
+    xf_put<T_, C, PL, s>(v, lds, t, col);
+    __syncthreads();
+    xf_get<T_, C, PL, s>(v, lds, t, col);
     __syncthreads();
 }
 
-// exchange after inverse stage s (s > 0): write at stage-s input positions,
-// read at stage s-1 output positions
+// exchange after inverse stage s (s > 0): write at stage-s input positions
+// (xi_put), read at stage s-1 output positions (xi_get)
 template <class T_, class C, class PL, int s>
-__device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
+__device__ __forceinline__ void xi_put(const T_* v, T_* lds, int t, int col)
 {
-    constexpr int r = PL::r[s], rp = PL::r[s - 1], Sp = PL::stride(s - 1), P = C::P, E = C::E,
-                  T = P / E;
+    constexpr int r = PL::r[s], P = C::P, E = C::E, T = P / E;
     constexpr bool wsep = (P / r) % C::M == 0;
-    constexpr bool rsep = (Sp * rp) % C::M == 0 && (Sp % C::M == 0 || C::M % Sp == 0);
 #pragma unroll
     for (int b = 0; b < E / r; ++b) {
         const int i = t + T * b;
@@ -157,7 +167,12 @@ __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
             for (int q = 0; q < r; ++q) lds[C::at(i + q * (P / r), col)] = v[b * r + q];
         }
     }
-    __syncthreads();
+}
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xi_get(T_* v, const T_* lds, int t, int col)
+{
+    constexpr int rp = PL::r[s - 1], Sp = PL::stride(s - 1), P = C::P, E = C::E, T = P / E;
+    constexpr bool rsep = (Sp * rp) % C::M == 0 && (Sp % C::M == 0 || C::M % Sp == 0);
 #pragma unroll
     for (int b = 0; b < E / rp; ++b) {
         const int i = t + T * b, k = i % Sp, j = (i - k) * rp + k;
@@ -170,6 +185,13 @@ __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
             for (int q = 0; q < rp; ++q) v[b * rp + q] = lds[C::at(j + q * Sp, col)];
         }
     }
+}
+template <class T_, class C, class PL, int s>
+__device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
+{
+    xi_put<T_, C, PL, s>(v, lds, t, col);
+    __syncthreads();
+    xi_get<T_, C, PL, s>(v, lds, t, col);
     __syncthreads();
 }
 
