@@ -1043,7 +1043,13 @@ namespace {
 template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjoint, hipStream_t st)
 {
     Op<R>* op = make_op<R>(c);
-    op->apply(lin1<R>(x), x, y, nodots<R>(), adjoint != 0, nullptr, st, nullptr);
+    const bool timed = c->timer.cls != 0;  // vie_time_kernel also covers bare products
+    if (timed) timing_begin(c);
+    op->apply(lin1<R>(x), x, y, nodots<R>(), adjoint != 0, nullptr, st, timed ? &c->timer : nullptr);
+    if (timed) {
+        HIPCHK(hipStreamSynchronize(st));
+        timing_end(c);
+    }
 }
 
 }  // namespace

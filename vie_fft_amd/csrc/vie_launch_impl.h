@@ -183,16 +183,16 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // for T32 <8,8,4> reading G~ in the product and 187 us for T64 <4,4,4,4>; FP64
 // T32 <8,8,4> with the prefetch 229 us against 276 us with an LDS slice.
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
-template <> struct XReg<float, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 0; using PL = XPlan<8, 8>; };
-template <> struct XReg<float, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 0; using PL = XPlan<8, 4, 4>; };
-template <> struct XReg<float, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 0; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct XReg<float, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8>; };
+template <> struct XReg<float, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 4, 4>; };
+template <> struct XReg<float, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<4, 4, 4, 3>; };
 template <> struct XReg<float, 128> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<16, 16>; };
-template <> struct XReg<float, 256> { static constexpr bool ok = true; static constexpr int T = 32; static constexpr int W = 0; static constexpr int GM = 0; using PL = XPlan<16, 16, 2>; };
-template <> struct XReg<double, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 1; using PL = XPlan<8, 8>; };
-template <> struct XReg<double, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 1; using PL = XPlan<8, 4, 4>; };
-template <> struct XReg<double, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 1; using PL = XPlan<4, 4, 4, 3>; };
+template <> struct XReg<float, 256> { static constexpr bool ok = true; static constexpr int T = 32; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<16, 16, 2>; };
+template <> struct XReg<double, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8>; };
+template <> struct XReg<double, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 4, 4>; };
+template <> struct XReg<double, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<4, 4, 4, 3>; };
 template <> struct XReg<double, 128> { static constexpr bool ok = true; static constexpr int T = 32; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8, 4>; };
-template <> struct XReg<double, 256> { static constexpr bool ok = true; static constexpr int T = 64; static constexpr int W = 0; static constexpr int GM = 1; using PL = XPlan<8, 8, 8>; };
+template <> struct XReg<double, 256> { static constexpr bool ok = true; static constexpr int T = 64; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8, 8>; };
 
 // VIE_XPASS_LDS=1 selects the LDS-staged x pass for every size (A/B timing).
 static bool xpass_lds_forced()

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
 void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTabs<R> mt, const int* done)
 {
     using T_ = cx<R>;
-    using C = XCfg<R, N, T, PL, false, sizeof(R) == 4 && T == 32>;
+    using C = XCfg<R, N, T, PL>;
     constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     constexpr bool GSL = GM == 1;
