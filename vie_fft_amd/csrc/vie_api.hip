@@ -537,13 +537,8 @@ void require(vie_ctx* c, bool ok, const char* what)
 
 // ---------------------------------------------------------------- solvers
 // Vectors: index constants into the per-precision work-vector pool.
-enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VG, VNU, NVEC };
-
-template <class R> struct SolveOut {
-    KState s;
-};
-
-template <class R> using T_ = cx<R>;
+// VT/VT2: staging of bare products; VG/VNU: adjoint right-hand side and solution.
+enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, NVEC };
 
 // Drives iterations until the device flags done; `iter` enqueues one.
 template <class F> KState drive(vie_ctx* c, KState* st, F iter)
@@ -1375,7 +1370,7 @@ int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* strea
                 apply_dev<float>(c, (const float2*)x, (float2*)y, adjoint, st);
             } else {
                 float2* xs = vec<float>(c, VT);
-                float2* ys = vec<float>(c, VT + 1);
+                float2* ys = vec<float>(c, VT2);
                 hipLaunchKernelGGL(k_embed<float2>, dim3(ge), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
                                    (const float2*)x, xs);
                 apply_dev<float>(c, xs, ys, adjoint, st);
@@ -1408,7 +1403,7 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
         } else {
             float2* xs = vec<float>(c, VT);
-            float2* ys = vec<float>(c, VT + 1);
+            float2* ys = vec<float>(c, VT2);
             hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->t64, xs);
             apply_dev<float>(c, xs, ys, adjoint, c->stream);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
