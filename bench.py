@@ -182,27 +182,70 @@ def copy_probe(torch, n):
                       "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
 
 
+def _component_seconds(sc, flags):
+    """Seconds for one X_FFT-equivalent component product of the CPU oracle in
+    the reference schedule (one core)."""
+    from oracle import oracle as O
+
+    op = O.Operator(sc.n, sc.d, sc.k0, sc.eps(), flags, O.SCHED_REFERENCE)
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal((3,) + sc.n) + 1j * rng.standard_normal((3,) + sc.n)) * flags
+    t = op.time_component(x, 0)
+    del op
+    return t
+
+
+def _cpu_child():
+    """Child process of the all-cores leg: one independent component product."""
+    from vie_fft_amd import geometry as G
+
+    sc = G.metric_case()
+    print(json.dumps({"t_comp": _component_seconds(sc, sc.flags())}), flush=True)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, flags, iterations):
     """Time the CPU oracle in the reference's schedule on a bounded sample:
     one output component of one product (3 forward + 3 inverse padded FFTs and
     3 per-element symbol loops, the work of one X_FFT call,
-    vie_fft/CG_Iteration.h:131-143), on one core; a BiCGSTAB iteration is 6 such
-    calls (two products), a solve is `iterations` iterations."""
-    from oracle import oracle as O
+    vie_fft/CG_Iteration.h:131-143); a BiCGSTAB iteration is 6 such calls (two
+    products), a solve is `iterations` iterations.  Two legs: one core (the
+    reference's MKL-sequential solve, README.md:37), and K cores each running
+    an independent sample at once (its frequency-farm mode, vie_fft/main.cpp:27-36),
+    K = min(16, cpu_count) child processes."""
+    import subprocess
 
-    eps = sc.eps()
-    t0 = time.perf_counter()
-    op = O.Operator(sc.n, sc.d, sc.k0, eps, flags, O.SCHED_REFERENCE)
-    t_setup = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    x = (rng.standard_normal((3,) + sc.n) + 1j * rng.standard_normal((3,) + sc.n)) * flags
-    t_comp = op.time_component(x, 0)
+    t_comp = _component_seconds(sc, flags)
     t_solve = 6.0 * t_comp * iterations
-    del op
+    k = max(1, min(16, os.cpu_count() or 1))
+    here = os.path.dirname(os.path.abspath(__file__))
+    procs = [subprocess.Popen([sys.executable, "-c", "from bench import _cpu_child; _cpu_child()"], cwd=here,
+                              stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for _ in range(k)]
+    ts = []
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode == 0 and out.strip():
+            ts.append(json.loads(out.strip().splitlines()[-1])["t_comp"])
+    par = None
+    if len(ts) == k:
+        t_par = 6.0 * float(np.mean(ts)) * iterations
+        par = {"value": k / t_par, "unit": "solves/s", "cores": k,
+               "sample": f"{k} concurrent processes, one component product each: mean {np.mean(ts):.2f} s "
+                         f"(max {max(ts):.2f} s); throughput = {k} solves per extrapolated solve time"}
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"one X_FFT-equivalent component product at {sc.n[0]}^3 (reference 9+9 FFT schedule, "
                       f"per-element trig) took {t_comp:.2f} s on 1 core; solve extrapolated as 6 x that x "
-                      f"{iterations} BiCGSTAB iterations ({t_solve:.0f} s); set-up {t_setup:.1f} s not counted"}
+                      f"{iterations} BiCGSTAB iterations ({t_solve:.0f} s); set-up not counted",
+            "cpu_model": _cpu_model(), "all_cores": par}
 
 
 if __name__ == "__main__":
