@@ -153,7 +153,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": XPASS_F32_128_TRAFFIC if n == (128, 128, 128) else None,
-                     "kernel": "k_xreg<float,128,16,XPlan<16,16>,0,2> (x-axis forward + spectral product + inverse)",
+                     "kernel": "k_xreg<float,128,16,XPlan<16,16>,0,2,true> (x-axis forward + spectral product + inverse)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }
     if not args.no_cpu:

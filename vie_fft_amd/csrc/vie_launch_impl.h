@@ -177,22 +177,34 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 }
 
 // Register-resident x-pass configurations: T threads per column, radix plan,
-// W waves-per-SIMD register target (0: compiler default), GM how G~ is read
-// (k_xreg).  At 128 (the 128^3 workload, measured per launch on MI355X):
-// FP32 T16 <16,16> with G~ prefetched into registers 133 us against 149 us
-// for T32 <8,8,4> reading G~ in the product and 187 us for T64 <4,4,4,4>; FP64
-// T32 <8,8,4> with the prefetch 229 us against 276 us with an LDS slice.
+// W waves-per-SIMD register target (0: compiler default), GM how G~ is read,
+// OP per-component lane coordinates (k_xreg).  At 128 (the 128^3 workload,
+// measured per launch on MI355X): FP32 T16 <16,16> with G~ prefetched into
+// registers 133 us against 149 us for T32 <8,8,4> reading G~ in the product
+// and 187 us for T64 <4,4,4,4>; with OP its registers fall from 208 to 164
+// (three waves per SIMD instead of two) and it takes 118 us.  FP64 T32
+// <8,8,4> with the prefetch 229 us against 276 us with an LDS slice; OP
+// leaves it at two waves (169 registers) and costs 262 us.  OP is on where it
+// adds a wave per SIMD (FP32 64: 130 -> 124 registers, 96: 211 -> 159, 128).
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
-template <> struct XReg<float, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8>; };
-template <> struct XReg<float, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 4, 4>; };
-template <> struct XReg<float, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<4, 4, 4, 3>; };
-template <> struct XReg<float, 128> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<16, 16>; };
-template <> struct XReg<float, 256> { static constexpr bool ok = true; static constexpr int T = 32; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<16, 16, 2>; };
-template <> struct XReg<double, 32> { static constexpr bool ok = true; static constexpr int T = 8; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8>; };
-template <> struct XReg<double, 64> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 4, 4>; };
-template <> struct XReg<double, 96> { static constexpr bool ok = true; static constexpr int T = 16; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<4, 4, 4, 3>; };
-template <> struct XReg<double, 128> { static constexpr bool ok = true; static constexpr int T = 32; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8, 4>; };
-template <> struct XReg<double, 256> { static constexpr bool ok = true; static constexpr int T = 64; static constexpr int W = 0; static constexpr int GM = 2; using PL = XPlan<8, 8, 8>; };
+#define VIE_XREG(R, N, T_, GM_, OP_, ...)                                                                    \
+    template <> struct XReg<R, N> {                                                                       \
+        static constexpr bool ok = true;                                                                  \
+        static constexpr int T = T_, W = 0, GM = GM_;                                                     \
+        static constexpr bool OP = OP_;                                                                   \
+        using PL = XPlan<__VA_ARGS__>;                                                                    \
+    };
+VIE_XREG(float, 32, 8, 2, false, 8, 8)
+VIE_XREG(float, 64, 16, 2, true, 8, 4, 4)
+VIE_XREG(float, 96, 16, 2, true, 4, 4, 4, 3)
+VIE_XREG(float, 128, 16, 2, true, 16, 16)
+VIE_XREG(float, 256, 32, 2, false, 16, 16, 2)
+VIE_XREG(double, 32, 8, 2, false, 8, 8)
+VIE_XREG(double, 64, 16, 2, false, 8, 4, 4)
+VIE_XREG(double, 96, 16, 2, false, 4, 4, 4, 3)
+VIE_XREG(double, 128, 32, 2, false, 8, 8, 4)
+VIE_XREG(double, 256, 64, 2, false, 8, 8, 8)
+#undef VIE_XREG
 
 // VIE_XPASS_LDS=1 selects the LDS-staged x pass for every size (A/B timing).
 static bool xpass_lds_forced()
@@ -214,7 +226,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nz + B - 1) / B;
             dim3 grid((unsigned)(2L * g.ny * nzt));
-            hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM>), grid, dim3(NT), 0, st, g, W2, twP, mt,
+            hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP>), grid, dim3(NT), 0, st, g, W2, twP, mt,
                                done);
             return hipGetLastError();
         }

@@ -195,29 +195,41 @@ __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
     __syncthreads();
 }
 
-template <class T_, class C, class PL, int s = 0>
+// A copy of v the compiler cannot see through.  With OPQ each component gets
+// its own copy of the lane coordinates, so its twiddle loads and exchange
+// addresses are recomputed instead of shared (and held live) across the three
+// components: fewer registers for a few more LDS reads and integer ops.
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <class T_, class C, class PL, bool OPQ, int s = 0>
 __device__ __forceinline__ void x_forward(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
 {
     if constexpr (s < PL::n) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            xf_stage<T_, C, PL, s>(v[c], tw, t);
-            if constexpr (s + 1 < PL::n) xf_xchg<T_, C, PL, s>(v[c], lds, t, col);
+            const int tc = OPQ ? opaque(t) : t, cc = OPQ ? opaque(col) : col;
+            xf_stage<T_, C, PL, s>(v[c], tw, tc);
+            if constexpr (s + 1 < PL::n) xf_xchg<T_, C, PL, s>(v[c], lds, tc, cc);
         }
-        x_forward<T_, C, PL, s + 1>(v, lds, tw, t, col);
+        x_forward<T_, C, PL, OPQ, s + 1>(v, lds, tw, t, col);
     }
 }
 
-template <class T_, class C, class PL, int s = PL::n - 1>
+template <class T_, class C, class PL, bool OPQ, int s = PL::n - 1>
 __device__ __forceinline__ void x_inverse(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
 {
     if constexpr (s >= 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            xi_stage<T_, C, PL, s>(v[c], tw, t);
-            if constexpr (s > 0) xi_xchg<T_, C, PL, s>(v[c], lds, t, col);
+            const int tc = OPQ ? opaque(t) : t, cc = OPQ ? opaque(col) : col;
+            xi_stage<T_, C, PL, s>(v[c], tw, tc);
+            if constexpr (s > 0) xi_xchg<T_, C, PL, s>(v[c], lds, tc, cc);
         }
-        x_inverse<T_, C, PL, s - 1>(v, lds, tw, t, col);
+        x_inverse<T_, C, PL, OPQ, s - 1>(v, lds, tw, t, col);
     }
 }
 
@@ -235,8 +247,9 @@ __device__ __forceinline__ int fold_split(int t, int n)
 // twP[e] = exp(-2 pi j e / P), e < P.  WPE: waves per SIMD the register
 // allocation aims for (0: compiler default).  GM: how the G octant is read --
 // 0 per frequency from memory in the product, 1 a slice staged in LDS, 2 per
-// frequency into registers before the forward transform.
-template <class R, int N, int T, class PL, int WPE, int GM = (sizeof(R) == 8 ? 1 : 0)>
+// frequency into registers before the forward transform.  OPQ: per-component
+// lane coordinates (x_forward).
+template <class R, int N, int T, class PL, int WPE, int GM = (sizeof(R) == 8 ? 1 : 0), bool OPQ = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTabs<R> mt, const int* done)
 {
@@ -309,7 +322,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             for (int q = 0; q < rL; ++q) gpre[b * rL + q] = gcol[(unsigned)fxof(t + (P / E) * b + q * (P / rL)) * gs];
     }
     __syncthreads();
-    x_forward<T_, C, PL>(v, lds, tw, t, col);
+    x_forward<T_, C, PL, OPQ>(v, lds, tw, t, col);
 
     // D_f = j k0 Z0 delta_f / NP is imaginary: D_f X = (-Im D_f X.y, Im D_f X.x)
     const R D0 = mt.D[0][1], D1 = mt.D[1][1], D2 = mt.D[2][1];
@@ -336,7 +349,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             v[2][e] = cmul(Gv, cadd(mkc(-D2 * X2.y, D2 * X2.x), cmul(uz, sig)));
         }
     }
-    x_inverse<T_, C, PL>(v, lds, tw, t, col);
+    x_inverse<T_, C, PL, OPQ>(v, lds, tw, t, col);
     if (colok) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
