@@ -184,8 +184,11 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // and 187 us for T64 <4,4,4,4>; with OP its registers fall from 208 to 164
 // (three waves per SIMD instead of two) and it takes 118 us.  FP64 T32
 // <8,8,4> with the prefetch 229 us against 276 us with an LDS slice; OP
-// leaves it at two waves (169 registers) and costs 262 us.  OP is on where it
-// adds a wave per SIMD (FP32 64: 130 -> 124 registers, 96: 211 -> 159, 128).
+// leaves it at two waves (169 registers) and costs 262 us, with G~ read in
+// the product (148 registers) 264 us.  OP is on where it adds a wave per SIMD
+// (FP32 64: 130 -> 124 registers, 96: 211 -> 159, 128); FP32 256 also reads
+// G~ in the product (240 -> 159 registers: 1578 against 1653 us at 256^3,
+// where FP64 that way takes 3884 against 2983 us).
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
 #define VIE_XREG(R, N, T_, GM_, OP_, ...)                                                                    \
     template <> struct XReg<R, N> {                                                                       \
@@ -198,7 +201,7 @@ VIE_XREG(float, 32, 8, 2, false, 8, 8)
 VIE_XREG(float, 64, 16, 2, true, 8, 4, 4)
 VIE_XREG(float, 96, 16, 2, true, 4, 4, 4, 3)
 VIE_XREG(float, 128, 16, 2, true, 16, 16)
-VIE_XREG(float, 256, 32, 2, false, 16, 16, 2)
+VIE_XREG(float, 256, 32, 0, true, 16, 16, 2)
 VIE_XREG(double, 32, 8, 2, false, 8, 8)
 VIE_XREG(double, 64, 16, 2, false, 8, 4, 4)
 VIE_XREG(double, 96, 16, 2, false, 4, 4, 4, 3)
