@@ -38,28 +38,21 @@ template <int... Rs> struct XPlan {
 // use is free of LDS bank conflicts in the column-fast map and within 1.75x
 // of conflict-free in the column-slow map (64 lanes, 32 or 64 banks, 8- and
 // 16-byte elements, checked by enumeration).
-// ROW1 (column-fast only): rows of B + 1 slots instead, which costs the FP32
-// x pass fewer registers (one more wave per SIMD) than it loses to the
-// 2-way conflicts left on two of its exchanges.
-template <class R, int N, int T, class PL, bool CS = false, bool ROW1 = false> struct XCfg {
+template <class R, int N, int T, class PL, bool CS = false> struct XCfg {
     static constexpr int P = 2 * N;
     static constexpr int E = P / T;
     static constexpr int B = NT / T;     // columns per workgroup
     static constexpr int M = (E & (E - 1)) == 0 ? E : 4;
     static constexpr int LM = M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : M == 32 ? 5 : 6;
     static constexpr int PP = P + P / M;  // padded positions per column
-    static constexpr int LDS = ROW1 ? P * (B + 1) : PP * B;  // exchange buffer elements
+    static constexpr int LDS = PP * B;    // exchange buffer elements
     // padded slot of position p, and the LDS index of slot l in column col
-    __device__ static constexpr unsigned pad(unsigned p) { return ROW1 ? p : p + (p >> LM); }
-    __device__ static constexpr unsigned idx(unsigned l, unsigned col)
-    {
-        return CS ? col * PP + l : l * (ROW1 ? B + 1 : B) + col;
-    }
+    __device__ static constexpr unsigned pad(unsigned p) { return p + (p >> LM); }
+    __device__ static constexpr unsigned idx(unsigned l, unsigned col) { return CS ? col * PP + l : l * B + col; }
     __device__ static constexpr unsigned at(unsigned p, unsigned col) { return idx(pad(p), col); }
     // pad(p0 + d) == pad(p0) + pad(d) whenever p0 is a multiple of M: then the
     // slots of one butterfly are a common base plus compile-time offsets
-    static constexpr int step(int d) { return ROW1 ? d * (B + 1) : CS ? d + d / M : (d + d / M) * B; }
-    static_assert(!(CS && ROW1), "row padding applies to the column-fast map");
+    static constexpr int step(int d) { return CS ? d + d / M : (d + d / M) * B; }
     static_assert((1 << LM) == M, "pad period must be a power of two");
     static_assert(P % T == 0, "T must divide P");
     static_assert(B * T == NT, "T must divide the workgroup");
