@@ -130,6 +130,7 @@ template <class R> struct Op {
     T* G = nullptr;
     T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
+    uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
     T* W1 = nullptr;
     T* W2 = nullptr;
     bool own_w = true;
@@ -194,6 +195,7 @@ template <class R> struct Op {
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
         flags = dalloc<uint8_t>(3 * g.V);
+        zr = dalloc<uint32_t>(3L * g.nx * g.ny);
         if (w1) {
             W1 = w1;
             W2 = w2;
@@ -260,6 +262,7 @@ template <class R> struct Op {
         dfree(G);
         dfree(dg);
         dfree(flags);
+        dfree(zr);
         if (own_w) {
             dfree(W1);
             dfree(W2);
@@ -300,6 +303,16 @@ template <class R> Lin<R> lin1(const cx<R>* x)
 {
     Lin<R> l{};
     l.x[0] = x;
+    return l;
+}
+
+// Lin over Krylov vectors (zero on inactive faces: the masked right-hand side
+// and the masked products keep them so), which the prologue then reads only
+// within each z-row's active range.
+template <class R> Lin<R> lin_k(const uint32_t* zr, const cx<R>* x)
+{
+    Lin<R> l = lin1<R>(x);
+    l.zr = zr;
     return l;
 }
 
@@ -389,8 +402,11 @@ template <class R> void fill0(vie_ctx* c, cx<R>* x, long n)
     HIPCHK(hipGetLastError());
 }
 
+// Every linear combination the solvers form is over vectors that vanish on
+// inactive faces, so the kernel reads inputs and dot operands only there.
 template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 {
+    if (Op<R>* o = op_of<R>(c)) L.mask = o->flags;
     const unsigned gsz = grid1((L.n + 1) / 2, NT, 2048);
     L.dots.partial = c->partial;
     const int cls = sizeof(R) == 8 ? 6 : 16;
@@ -438,6 +454,7 @@ template <class R> Dots<R> dots_with(vie_ctx* c, int n, std::initializer_list<in
     d.e1 = e1;
     d.e2 = e2;
     d.partial = c->partial;
+    d.masked = 1;  // solver dot products: every operand vanishes on inactive faces
     return d;
 }
 
@@ -490,6 +507,9 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
                        cim[2], c->eps_dev, op->dg);
     HIPCHK(hipGetLastError());
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
+    const long rows = 3L * n[0] * n[1];
+    hipLaunchKernelGGL(k_zrange, dim3(grid1(rows)), dim3(NT), 0, c->stream, rows, n[2], op->flags, op->zr);
+    HIPCHK(hipGetLastError());
 }
 
 template <> Op<double>* make_op<double>(vie_ctx* c)
@@ -573,12 +593,14 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
     }
     auto iter = [&]() {
         Lin<R> lp{};
+        lp.zr = op->zr;
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
         long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
+        lq.zr = op->zr;
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
@@ -614,12 +636,12 @@ template <class R> KState solve_cg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>*
         L.out2 = r;
         lincomb<R>(c, L, done);
         // p = A^H b ; ||p||^2, ||b||^2
-        long nb = op->apply(lin1<R>(b), b, p, dots_with<R>(c, 2, {0, 0, 1, 1}, b, nullptr), true, done, c->stream,
+        long nb = op->apply(lin_k<R>(op->zr, b), b, p, dots_with<R>(c, 2, {0, 0, 1, 1}, b, nullptr), true, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CG_INIT, nb, 2);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin1<R>(p), p, Ap, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_k<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CG_ALPHA, nb, 1);
         LinComb<R> L{};
@@ -631,7 +653,7 @@ template <class R> KState solve_cg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>*
         L.dots = dots_with<R>(c, 1, {3, 3}, nullptr, nullptr);
         nb = lincomb<R>(c, L, done);
         scalar(c, st, SC_CG_V4, nb, 1);
-        nb = op->apply(lin1<R>(r), r, q, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), true, done, c->stream,
+        nb = op->apply(lin_k<R>(op->zr, r), r, q, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), true, done, c->stream,
                        &c->timer);
         scalar(c, st, SC_CG_BETA, nb, 1);
         LinComb<R> P{};
@@ -671,7 +693,7 @@ template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
     }
     auto iter = [&]() {
         // Ap ; Ap^T p (unconjugated)
-        long nb = op->apply(lin1<R>(p), p, Ap, dots_with<R>(c, 1, {0, 1}, p, nullptr, {1}), false, done, c->stream,
+        long nb = op->apply(lin_k<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 1}, p, nullptr, {1}), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_BICG_ALPHA, nb, 1);
         LinComb<R> L{};
@@ -717,7 +739,7 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         scalar(c, st, SC_BNRM, nb, 1);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin1<R>(r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_k<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CORS_RHO, nb, 1);
         LinComb<R> L1{};  // p = r + b pc ; qc = Ap + b rc
@@ -728,6 +750,7 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         L1.out2 = qc;
         lincomb<R>(c, L1, done);
         Lin<R> lq{};  // q = qc + b rc + b^2 q, then Aq
+        lq.zr = op->zr;
         lq.x[0] = qc; lq.x[1] = rc; lq.x[2] = q;
         lq.coef = &st->cf[CF_CO_Q];
         lq.out = q;
@@ -782,7 +805,7 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
         scalar(c, st, SC_BNRM, nb, 1);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin1<R>(r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_k<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_BICOR_RHO, nb, 1);
         LinComb<R> L1{};  // p = r + b p - b w q
@@ -792,6 +815,7 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
         L1.out = p;
         lincomb<R>(c, L1, done);
         Lin<R> lq{};  // q = Ap + b q - b w Aq, then Aq
+        lq.zr = op->zr;
         lq.x[0] = Ap; lq.x[1] = q; lq.x[2] = Aq;
         lq.coef = &st->cf[CF_BC_Q];
         lq.out = q;
@@ -875,12 +899,14 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     }
     auto iter = [&]() {
         Lin<R> lp{};
+        lp.zr = op->zr;
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
         long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
+        lq.zr = op->zr;
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
@@ -908,7 +934,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
         HIPCHK(hipGetLastError());
         fill0<R>(c, x, n);
-        op64->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, &c->timer);
+        op64->apply(lin_k<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, &c->timer);
         LinComb<double> M{};
         M.n = n;
         M.x[0] = c->b64;
@@ -974,7 +1000,7 @@ double true_relres(vie_ctx* c)
     L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
     long nb = lincomb<double>(c, L, nullptr);
     scalar(c, s, SC_BNRM, nb, 1);
-    op->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, nullptr);
+    op->apply(lin_k<double>(op->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, nullptr);
     LinComb<double> M{};
     M.n = n;
     M.x[0] = c->b64;
@@ -1598,7 +1624,7 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
                 rnorm = std::sqrt(h.bnrm2);
                 err = 1.0;
             } else {
-                op64->apply(lin1<double>(c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream,
+                op64->apply(lin_k<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream,
                             &c->timer);
                 LinComb<double> M{};
                 M.n = n;

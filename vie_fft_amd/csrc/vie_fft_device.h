@@ -261,15 +261,22 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblk)
 // ---------------------------------------------------------------- args
 // Linear combination prologue: z = c0 x0 + c1 x1 + c2 x2 (coefficients in
 // device memory, so a solver can update them without a host round trip).
+// zr: nullptr, or per z-row active ranges (lo | hi << 16, k_zrange) when
+// every input is zero on inactive faces (Krylov vectors: the masked
+// right-hand side and masked products keep them so); entries outside a row's
+// range are then not read.
 template <class R> struct Lin {
     const cx<R>* x[3];
     const double2* coef;  // 3 complex; nullptr means (1, 0, 0)
     cx<R>* out;           // optional: store z (full vector, before padding)
+    const uint32_t* zr;
 };
 
 // Dot-product epilogue: partial[k * nblk + blk] = sum conj(a_k) b_k (or
 // a_k b_k when nc[k] is set: the unconjugated product of zdotu) with operands
 // chosen from {0: the output y, 1: e1, 2: e2, 3: a second output}.
+// masked: e1 and e2 are zero on inactive faces, so they are read only where
+// the output's face is active.
 template <class R> struct Dots {
     int n;          // 0..3
     int a[3], b[3];
@@ -277,6 +284,7 @@ template <class R> struct Dots {
     const cx<R>* e1;
     const cx<R>* e2;
     double2* partial;
+    int masked;
 };
 
 struct Flag {

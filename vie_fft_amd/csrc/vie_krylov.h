@@ -19,6 +19,9 @@ namespace vie {
 // out = sum_i c_i x_i and optionally out2 = sum_i c2_i x_i over n entries
 // (c from device coef[0..3], c2 from coef[4..7], or from hc/hc2), with up to
 // three fused dot products (operands 0 = out, 1 = e1, 2 = e2, 3 = out2).
+// mask: nullptr, or face flags when every input and dot operand is zero on
+// inactive faces: those entries are then not read (the outputs there are
+// written as zero).
 template <class R> struct LinComb {
     long n;
     const cx<R>* x[4];
@@ -28,7 +31,26 @@ template <class R> struct LinComb {
     cx<R>* out;
     cx<R>* out2;
     Dots<R> dots;
+    const uint8_t* mask;
 };
+
+// Active range of each z-row of the face flags (rows = 3 Nx Ny, component
+// major): zr = lo | hi << 16 with every flag outside [lo, hi) zero; 0 for a
+// row without active faces.
+__global__ __launch_bounds__(NT) void k_zrange(long rows, int nz, const uint8_t* __restrict__ flags,
+                                               uint32_t* __restrict__ zr)
+{
+    for (long r = (long)blockIdx.x * NT + threadIdx.x; r < rows; r += (long)gridDim.x * NT) {
+        const uint8_t* f = flags + r * nz;
+        int lo = nz, hi = 0;
+        for (int k = 0; k < nz; ++k)
+            if (f[k]) {
+                lo = lo < k ? lo : k;
+                hi = k + 1;
+            }
+        zr[r] = lo < hi ? (uint32_t)lo | ((uint32_t)hi << 16) : 0u;
+    }
+}
 
 template <class R>
 __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
@@ -71,13 +93,14 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
         const long i = 2 * i2;
         // every load of the pair first (more bytes in flight per thread)
         T a0[4], a1[4], e10 = czero<T>(), e11 = czero<T>(), e20 = czero<T>(), e21 = czero<T>();
+        const bool act = !L.mask || *reinterpret_cast<const uint16_t*>(L.mask + i) != 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             a0[q] = a1[q] = czero<T>();
-            if (L.x[q]) ld2(L.x[q] + i, a0[q], a1[q]);
+            if (L.x[q] && act) ld2(L.x[q] + i, a0[q], a1[q]);
         }
-        if (use1) ld2(L.dots.e1 + i, e10, e11);
-        if (use2) ld2(L.dots.e2 + i, e20, e21);
+        if (use1 && act) ld2(L.dots.e1 + i, e10, e11);
+        if (use2 && act) ld2(L.dots.e2 + i, e20, e21);
         T v = czero<T>(), v2 = czero<T>(), w = czero<T>(), w2 = czero<T>();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {

@@ -153,6 +153,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
+    __shared__ unsigned zrs[B];  // active range of each row (the whole row without lin.zr)
+    for (int t = threadIdx.x; t < B; t += NT)
+        zrs[t] = lin.zr && row0 + t < rows ? lin.zr[s * rows + row0 + t] : ((unsigned)N << 16);
     T c[3];
     load_coef<R>(lin.coef, c);
     __syncthreads();
@@ -165,7 +168,10 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int e = threadIdx.x + it * NT;
             const int col = e / (N / 2), n = 2 * (e % (N / 2));
             const long row = row0 + col;
-            if (e < B * N / 2 && row < rows) {
+            x0[it][0] = x0[it][1] = x1[it][0] = x1[it][1] = x2[it][0] = x2[it][1] = czero<T>();
+            // with row ranges, pairs that miss the row's active range are zero
+            const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
+            if (e < B * N / 2 && row < rows && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
                 const long idx = s * g.V + row * N + n;
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
@@ -208,9 +214,12 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             T v = czero<T>();
             if (row < rows) {
                 const long idx = s * g.V + row * N + n;
-                v = cmul(c[0], lin.x[0][idx]);
-                if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
-                if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
+                const unsigned rg = zrs[col];
+                if (n >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
+                    v = cmul(c[0], lin.x[0][idx]);
+                    if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
+                    if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
+                }
                 if (lin.out) lin.out[idx] = v;
                 const bool drop = (s == 2 && n == N - 1) || (s == 0 && row >= rows - g.ny) ||
                                   (s == 1 && (row % g.ny) == g.ny - 1);
@@ -454,10 +463,15 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
     const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
     auto term = [&](long idx, T out) {  // y = flag (out + d x), then its dot products
-        const T dv = dg.conj ? cconj(dg.d[idx]) : dg.d[idx];
-        const T yv = flags[idx] ? cadd(out, cmul(dv, src[idx])) : czero<T>();
+        const bool on = flags[idx] != 0;
+        T yv = czero<T>();
+        if (on) {  // d and x are read only where the face is active
+            const T dv = dg.conj ? cconj(dg.d[idx]) : dg.d[idx];
+            yv = cadd(out, cmul(dv, src[idx]));
+        }
         y[idx] = yv;
-        dots_add(acc, dots, yv, need1 ? dots.e1[idx] : czero<T>(), need2 ? dots.e2[idx] : czero<T>());
+        const bool rd = on || !dots.masked;
+        dots_add(acc, dots, yv, need1 && rd ? dots.e1[idx] : czero<T>(), need2 && rd ? dots.e2[idx] : czero<T>());
     };
     if constexpr (N % 2 == 0) {
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {

@@ -414,6 +414,13 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     load_coef<R>(lin.coef, c);
     const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
     const long base = s * g.V + row * N;
+    // row range outside which the inputs vanish (the whole row without lin.zr)
+    int lo = 0, hi = N;
+    if (lin.zr && rowok) {
+        const unsigned rg = lin.zr[s * rows + row];
+        lo = (int)(rg & 0xffffu);
+        hi = (int)(rg >> 16);
+    }
     // all loads before any store (lin.out may alias an input)
     T_ v[E], w1[E], w2[E];
 #pragma unroll
@@ -423,7 +430,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
         for (int q = 0; q < r1; ++q) {
             const int n = i + q * (P / r1), e = b * r1 + q;
             v[e] = w1[e] = w2[e] = czero<T_>();
-            if (n < N && rowok) {
+            if (n < N && rowok && n >= lo && n < hi) {
                 v[e] = lin.x[0][base + n];
                 if (lin.x[1]) w1[e] = lin.x[1][base + n];
                 if (lin.x[2]) w2[e] = lin.x[2][base + n];
@@ -495,6 +502,15 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     const bool e1src = dots.e1 == src;
     const long base = f * g.V + (rowok ? row : 0) * N;
     if (rowok) {
+        // flags first: x and d are read only on active faces (the output is
+        // zero elsewhere), masked dot operands likewise
+#pragma unroll
+        for (int b = 0; b < E / r1; ++b)
+#pragma unroll
+            for (int q = 0; q < r1; ++q) {
+                const int n = t + T * b + q * (P / r1);
+                if (n < N) fl[b * r1 + q] = flags[base + n];
+            }
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {
             const int i = t + T * b;
@@ -503,11 +519,13 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                 const int n = i + q * (P / r1), e = b * r1 + q;
                 if (n < N) {
                     const long idx = base + n;
-                    sv[e] = src[idx];
-                    dv[e] = dg.d[idx];
-                    fl[e] = flags[idx];
-                    if (need1 && !e1src) e1v[e] = dots.e1[idx];
-                    if (need2) e2v[e] = dots.e2[idx];
+                    sv[e] = dv[e] = e1v[e] = e2v[e] = czero<T_>();
+                    if (fl[e] || (!dots.masked && need1 && e1src)) sv[e] = src[idx];
+                    if (fl[e]) dv[e] = dg.d[idx];
+                    if (fl[e] || !dots.masked) {
+                        if (need1 && !e1src) e1v[e] = dots.e1[idx];
+                        if (need2) e2v[e] = dots.e2[idx];
+                    }
                 }
             }
         }
