@@ -292,6 +292,7 @@ template <class R> struct Op {
         Diag<R> dd{};
         dd.d = dg;
         dd.conj = adj ? 1 : 0;
+        dd.skip = lin.wskip;
         t0(5);
         HIPCHK(run_zinv<R>(g, W1, src, y, dd, flags, dots, tw[2], twist[2], twP[2], done, st, &nblk));
         t0(5);
@@ -578,8 +579,12 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
     cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
     const int* done = &st->done;
     state_init(c, st, tol, maxit);
+    // p, q and their products hold zero on inactive faces from here on: the
+    // iterations leave those entries unwritten (wskip)
     fill0<R>(c, p, n);
     fill0<R>(c, Ap, n);
+    fill0<R>(c, q, n);
+    fill0<R>(c, Aq, n);
     {   // x = 0 ; r = b ; b^H b
         LinComb<R> L{};
         L.n = n;
@@ -594,6 +599,7 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
     auto iter = [&]() {
         Lin<R> lp{};
         lp.zr = op->zr;
+        lp.wskip = 1;
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
@@ -601,6 +607,7 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
         lq.zr = op->zr;
+        lq.wskip = 1;
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
@@ -612,6 +619,7 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
         L.coef = &st->cf[CF_BS_U];
         L.out = x;
         L.out2 = r;
+        L.wskip = 1;
         L.dots = dots_with<R>(c, 2, {3, 3, 1, 3}, b, nullptr);
         nb = lincomb<R>(c, L, done);
         scalar(c, st, SC_BICGSTAB_END, nb, 2);
@@ -884,8 +892,12 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         scalar(c, ax, SC_BNRM, nb, 1);
     }
     state_init(c, st, tol, maxit);
+    // p, q and their products hold zero on inactive faces from here on: the
+    // iterations leave those entries unwritten (wskip)
     fill0<R>(c, p, n);
     fill0<R>(c, Ap, n);
+    fill0<R>(c, q, n);
+    fill0<R>(c, Aq, n);
     {
         LinComb<R> L{};
         L.n = n;
@@ -900,6 +912,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     auto iter = [&]() {
         Lin<R> lp{};
         lp.zr = op->zr;
+        lp.wskip = 1;
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
@@ -907,6 +920,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
         lq.zr = op->zr;
+        lq.wskip = 1;
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
@@ -918,6 +932,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         L.coef = &st->cf[CF_BS_U];
         L.out = x;
         L.out2 = r;
+        L.wskip = 1;
         L.dots = dots_with<R>(c, 2, {3, 3, 1, 3}, b, nullptr);
         nb = lincomb<R>(c, L, done);
         scalar(c, st, SC_BICGSTAB_END, nb, 2);

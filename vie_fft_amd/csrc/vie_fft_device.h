@@ -265,11 +265,14 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblk)
 // every input is zero on inactive faces (Krylov vectors: the masked
 // right-hand side and masked products keep them so); entries outside a row's
 // range are then not read.
+// wskip (with zr): out, and the product this prologue feeds, already hold
+// zero outside the active faces, so those entries are not written.
 template <class R> struct Lin {
     const cx<R>* x[3];
     const double2* coef;  // 3 complex; nullptr means (1, 0, 0)
     cx<R>* out;           // optional: store z (full vector, before padding)
     const uint32_t* zr;
+    int wskip;
 };
 
 // Dot-product epilogue: partial[k * nblk + blk] = sum conj(a_k) b_k (or

@@ -32,6 +32,7 @@ template <class R> struct LinComb {
     cx<R>* out2;
     Dots<R> dots;
     const uint8_t* mask;
+    int wskip;  // with mask: out and out2 already hold zero on inactive faces
 };
 
 // Active range of each z-row of the face flags (rows = 3 Nx Ny, component
@@ -121,8 +122,10 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
             dot_acc(acc[2], pick(L.dots.a[2], v, e10, e20, v2), pick(L.dots.b[2], v, e10, e20, v2), L.dots.nc[2]);
             dot_acc(acc[2], pick(L.dots.a[2], w, e11, e21, w2), pick(L.dots.b[2], w, e11, e21, w2), L.dots.nc[2]);
         }
-        if (L.out) st2(L.out + i, v, w);
-        if (L.out2) st2(L.out2 + i, v2, w2);
+        if (act || !L.wskip) {
+            if (L.out) st2(L.out + i, v, w);
+            if (L.out2) st2(L.out2 + i, v2, w2);
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (L.n & 1)) {
         const long i = L.n - 1;

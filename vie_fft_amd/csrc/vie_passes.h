@@ -42,6 +42,7 @@ template <class T> __device__ __forceinline__ T czero() { return mkc(decltype(T(
 template <class R> struct Diag {
     const cx<R>* d;       // per-face coefficients (3V)
     int conj;             // adjoint: use conj(d_f)
+    int skip;             // y already holds zero on inactive faces: leave them unwritten
 };
 
 // Block-wide sum of a double2; sh holds NT/64 entries.  Result valid in thread 0.
@@ -197,7 +198,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                     v0 = cadd(v0, cmul(c[2], x2[it][0]));
                     v1 = cadd(v1, cmul(c[2], x2[it][1]));
                 }
-                if (lin.out) st2(lin.out + idx, v0, v1);
+                if (lin.out && (!lin.wskip || (n + 1 >= (int)(zrs[col] & 0xffffu) && n < (int)(zrs[col] >> 16))))
+                    st2(lin.out + idx, v0, v1);
                 const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
                 if (dropall) v0 = v1 = czero<T>();
                 if (s == 2 && n + 1 == N - 1) v1 = czero<T>();
@@ -220,7 +222,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                     if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
                     if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
                 }
-                if (lin.out) lin.out[idx] = v;
+                if (lin.out && (!lin.wskip || (n >= (int)(zrs[col] & 0xffffu) && n < (int)(zrs[col] >> 16))))
+                    lin.out[idx] = v;
                 const bool drop = (s == 2 && n == N - 1) || (s == 0 && row >= rows - g.ny) ||
                                   (s == 1 && (row % g.ny) == g.ny - 1);
                 if (drop) v = czero<T>();
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
             const T dv = dg.conj ? cconj(dg.d[idx]) : dg.d[idx];
             yv = cadd(out, cmul(dv, src[idx]));
         }
-        y[idx] = yv;
+        if (on || !dg.skip) y[idx] = yv;
         const bool rd = on || !dots.masked;
         dots_add(acc, dots, yv, need1 && rd ? dots.e1[idx] : czero<T>(), need2 && rd ? dots.e2[idx] : czero<T>());
     };

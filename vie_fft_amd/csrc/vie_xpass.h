@@ -445,7 +445,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
             const int n = i + q * (P / r1), e = b * r1 + q;
             T_ z = cadd(cadd(cmul(c[0], v[e]), cmul(c[1], w1[e])), cmul(c[2], w2[e]));
             if (n < N && rowok) {
-                if (lin.out) lin.out[base + n] = z;
+                if (lin.out && (!lin.wskip || (n >= lo && n < hi))) lin.out[base + n] = z;
                 if (dropall || (s == 2 && n == N - 1)) z = czero<T_>();
             } else {
                 z = czero<T_>();
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                     const long idx = base + n;
                     T_ yv = czero<T_>();
                     if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
-                    y[idx] = yv;
+                    if (fl[e] || !dg.skip) y[idx] = yv;
                     dots_add(acc, dots, yv, e1src ? sv[e] : need1 ? e1v[e] : czero<T_>(),
                              need2 ? e2v[e] : czero<T_>());
                 }
