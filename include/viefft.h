@@ -213,8 +213,10 @@ int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double*
    0 off, 1 z-forward, 2 y-forward, 3 x spectral pass, 4 y-inverse,
    5 z-inverse of the FP64 operator, 6 FP64 vector update (linear
    combination + dots), 7 solver scalar step; 11-16 the same kernels in FP32
-   (17 = 7).  vie_kernel_time returns launches and mean milliseconds
-   (launches enqueued after convergence, which exit at once, are left out). */
+   (17 = 7).  Every 8th launch of the class is bracketed by events (timing
+   each one would slow the solve being measured).  vie_kernel_time returns the
+   timed launches and their mean milliseconds (launches enqueued after
+   convergence, which exit at once, are left out). */
 int vie_time_kernel(vie_ctx* ctx, int kernel_class);
 int vie_kernel_time(vie_ctx* ctx, int* launches, double* mean_ms);
 /* bytes the algorithmic model charges one operator application */

@@ -89,17 +89,30 @@ template <class R> cx<R> hc(std::complex<double> z)
 }
 
 // Kernel-class timing: HIP events around every launch of one pass.
+// Per-launch HIP-event timing of one kernel class.  mark() is called before
+// and after every launch of the class; one launch in kEvery is bracketed by
+// events (an event pair costs a few microseconds of stream time around the
+// launch, so timing every launch would slow the solve it measures).
 struct Timer {
+    static constexpr long kEvery = 8;
     int cls = 0;
     std::vector<hipEvent_t> ev;
     size_t used = 0;
+    long seen = 0;     // launches of the class since reset
+    bool open = false; // between the two marks of a launch
+    bool take = false; // this launch is sampled
     void reset()
     {
         used = 0;
+        seen = 0;
+        open = false;
     }
     void mark(hipStream_t st)
     {
         if (!cls) return;
+        open = !open;
+        if (open) take = seen++ % kEvery == 0;
+        if (!take) return;
         if (used >= ev.size()) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
@@ -362,6 +375,8 @@ struct vie_ctx {
     int timed_launches = 0;
     double timed_ms = 0.0;
     int ahead = 4;                 // iterations enqueued per host check
+    KState* snap = nullptr;        // two pinned state snapshots (one batch in flight while the other is read)
+    hipEvent_t snap_ev[2] = {};
 };
 
 namespace {
@@ -426,9 +441,11 @@ void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)
     if (c->timer.cls == 7) c->timer.mark(c->stream);
 }
 
-void state_init(vie_ctx* c, KState* s, double tol, int maxit)
+// pause > 0: the iterations also stop (done, not converged) once the
+// recursive residual falls below it (reliable updates)
+void state_init(vie_ctx* c, KState* s, double tol, int maxit, double pause = 0.0)
 {
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit);
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit, pause);
     HIPCHK(hipGetLastError());
 }
 
@@ -438,6 +455,23 @@ KState read_state(vie_ctx* c, KState* s)
     HIPCHK(hipMemcpyAsync(&h, s, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return h;
+}
+
+// Enqueue a copy of the state into pinned snapshot `slot`, read later with
+// wait_snap: the host checks one batch of iterations while the next runs.
+void snap_state(vie_ctx* c, const KState* s, int slot)
+{
+    if (!c->snap) {
+        HIPCHK(hipHostMalloc((void**)&c->snap, 2 * sizeof(KState), hipHostMallocDefault));
+        for (hipEvent_t& e : c->snap_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    HIPCHK(hipMemcpyAsync(&c->snap[slot], s, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->snap_ev[slot], c->stream));
+}
+KState wait_snap(vie_ctx* c, int slot)
+{
+    HIPCHK(hipEventSynchronize(c->snap_ev[slot]));
+    return c->snap[slot];
 }
 
 template <class R> Dots<R> dots_with(vie_ctx* c, int n, std::initializer_list<int> ab, const cx<R>* e1,
@@ -561,13 +595,21 @@ void require(vie_ctx* c, bool ok, const char* what)
 // VT/VT2: staging of bare products; VG/VNU: adjoint right-hand side and solution.
 enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, NVEC };
 
-// Drives iterations until the device flags done; `iter` enqueues one.
+// Drives iterations until the device flags done; `iter` enqueues one.  Two
+// batches of c->ahead iterations are in flight: the snapshot taken after one
+// is read while the next runs, so the device never waits for the host (the
+// iterations enqueued after convergence exit at once).
 template <class F> KState drive(vie_ctx* c, KState* st, F iter)
 {
+    int slot = 0;
+    for (int k = 0; k < c->ahead; ++k) iter();
+    snap_state(c, st, slot);
     for (;;) {
         for (int k = 0; k < c->ahead; ++k) iter();
-        KState h = read_state(c, st);
-        if (h.done) return h;
+        snap_state(c, st, slot ^ 1);
+        const KState h = wait_snap(c, slot);
+        slot ^= 1;
+        if (h.done) return read_state(c, st);
     }
 }
 
@@ -891,7 +933,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         long nb = lincomb<double>(c, L, nullptr);
         scalar(c, ax, SC_BNRM, nb, 1);
     }
-    state_init(c, st, tol, maxit);
+    state_init(c, st, tol, maxit, delta);  // err_ref = 1 until the first replacement
     // p, q and their products hold zero on inactive faces from here on: the
     // iterations leave those entries unwritten (wskip)
     fill0<R>(c, p, n);
@@ -940,11 +982,24 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     double err_ref = 1.0, terr = 1.0;
     int nrep = 0;
     KState h{};
+    // As in drive(): one batch of iterations runs while the snapshot of the
+    // previous one is checked; a replacement waits for the batch in flight.
+    int slot = 0;
+    bool primed = false;
     for (;;) {
+        if (!primed) {
+            for (int k = 0; k < c->ahead; ++k) iter();
+            snap_state(c, st, slot);
+            primed = true;
+        }
         for (int k = 0; k < c->ahead; ++k) iter();
+        snap_state(c, st, slot ^ 1);
+        h = wait_snap(c, slot);
+        slot ^= 1;
+        if (!(h.done || h.err < delta * err_ref || h.status != KS_OK || h.iter >= h.maxit)) continue;
         h = read_state(c, st);
+        primed = false;
         const bool stop = h.status != KS_OK || h.iter >= h.maxit;
-        if (!(h.done || h.err < delta * err_ref || stop)) continue;
         // reliable update: x64 += x32 ; x32 = 0 ; r = b - A64 x64 (in FP64, then FP32)
         hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
         HIPCHK(hipGetLastError());
@@ -965,7 +1020,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         if (terr < tol || stop || !std::isfinite(terr)) break;
         hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->r64, r);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_state_resume, dim3(1), dim3(1), 0, c->stream, st);
+        hipLaunchKernelGGL(k_state_resume, dim3(1), dim3(1), 0, c->stream, st, delta * terr);
         HIPCHK(hipGetLastError());
         LinComb<R> Rr{};  // b^H r, r^H r of the replaced residual
         Rr.n = n;
@@ -1216,6 +1271,9 @@ int vie_destroy(vie_ctx* c)
     dfree(c->x64);
     dfree(c->t64);
     dfree(c->r64);
+    if (c->snap) (void)hipHostFree(c->snap);
+    for (hipEvent_t e : c->snap_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : c->vec64) if (p) (void)hipFree(p);
     for (void* p : c->vec32) if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

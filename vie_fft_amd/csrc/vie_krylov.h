@@ -205,21 +205,23 @@ __global__ void k_scale_from64(long n, double s, const double2* in, cx<R>* out)
 
 // Let a solve that stopped on its recursive residual continue (after a
 // residual replacement); a breakdown or the iteration limit stays final.
-__global__ void k_state_resume(KState* s)
+__global__ void k_state_resume(KState* s, double pause)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0 && s->status == KS_OK && s->iter < s->maxit) {
         s->done = 0;
         s->converged = 0;
+        s->pause = pause;
     }
 }
 
-// KState initialisation from the host-side tolerance and limit.
-__global__ void k_state_init(KState* s, double tol, int maxit)
+// KState initialisation from the host-side tolerance, limit and pause level.
+__global__ void k_state_init(KState* s, double tol, int maxit, double pause)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         KState z = {};
         z.tol = tol;
         z.maxit = maxit;
+        z.pause = pause;
         *s = z;
     }
 }

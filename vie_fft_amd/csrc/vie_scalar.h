@@ -20,6 +20,7 @@ struct KState {
     double2 cf[48];
     double2 sums[4];
     double bnrm2, err, tol;
+    double pause;         // recursive residual below which the iterations stop for a residual replacement (0: never)
     int iter, maxit, done, converged, status;
     int fail_op;          // scalar program that stopped the solve, if any
 };
@@ -156,7 +157,7 @@ __device__ __forceinline__ void ks_finish(KState* s, double err, double* hist)
     s->iter += 1;
     if (!isfinite(err)) { ks_fail(s, KS_NAN); return; }
     if (err < s->tol) { s->converged = 1; s->done = 1; return; }
-    if (s->iter >= s->maxit) s->done = 1;
+    if (err < s->pause || s->iter >= s->maxit) s->done = 1;
 }
 
 __device__ __forceinline__ void set3(double2* c, double2 a, double2 b, double2 d)
