@@ -513,7 +513,10 @@ template <class R> constexpr int pick_b(int n, int cols_per_b, int bmax, int bud
     while (b > 1 && (long)n * cols_per_b * b * (long)sizeof(cx<R>) > budget) b /= 2;
     return b;
 }
-template <class R> constexpr int b_z(int n) { return pick_b<R>(n, 2, sizeof(R) == 8 ? 8 : 16, 40 * 1024); }
+// z passes: at most 8 rows per workgroup in either precision (FP32 128^3 z
+// forward: 44 us per launch against 48 us with 16 rows, from more workgroups
+// per CU)
+template <class R> constexpr int b_z(int n) { return pick_b<R>(n, 2, 8, 40 * 1024); }
 template <class R> constexpr int b_y(int n) { return pick_b<R>(n, 2, sizeof(R) == 8 ? 8 : 16, 40 * 1024); }
 template <class R> constexpr int b_x(int n) { return pick_b<R>(n, 6, sizeof(R) == 8 ? 8 : 16, 56 * 1024); }
 
