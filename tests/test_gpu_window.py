@@ -1,0 +1,183 @@
+"""Slab media: the active faces fill only some planes of one axis.  The
+library then reorders the axes internally (the slab's axis first) and runs
+the passes of every Krylov product only over the x planes that hold active
+faces; the caller's layout is unchanged.  Every result must equal the
+oracle's, which works on the caller's grid with every plane:
+
+* operator products (forward and adjoint, FP64 and FP32, host and device
+  buffers) on slabs along each axis;
+* 8 fixed iterations of each of the five methods (iterates to 1e-9);
+* converged solves, the incident right-hand side built on the device, the
+  far-field, absorbed-power and gradient outputs, and the mixed solve;
+* an embedded grid (lengths without compiled kernels) with a slab;
+* the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import random_field, rel
+from vie_fft_amd import viefft as VF
+
+pytestmark = pytest.mark.gpu
+
+N = (16, 12, 20)
+D = (0.02, 0.025, 0.03)
+K0 = 2 * np.pi / 4.0
+TABLE = np.array([1.1 + 0j, 2.5 - 0.3j, 4.0 - 0.8j])
+SLABS = [(0, 5, 11), (1, 3, 8), (2, 6, 15)]  # axis, first plane, end plane
+
+
+def slab_scene(axis, lo, hi, seed, n=N):
+    """Material ids 1-2 on the planes [lo, hi) of `axis`, background 0 elsewhere;
+    a face is active where both cells it joins are in the slab."""
+    rng = np.random.default_rng(seed)
+    mat = np.zeros(n, np.uint8)
+    sl = [slice(None)] * 3
+    sl[axis] = slice(lo, hi)
+    mat[tuple(sl)] = rng.integers(1, 3, size=mat[tuple(sl)].shape)
+    inside = mat > 0
+    flags = np.zeros((3,) + tuple(n), np.uint8)
+    for f in range(3):
+        nb = np.zeros_like(inside)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        src[f] = slice(1, None)
+        dst[f] = slice(0, -1)
+        nb[tuple(dst)] = inside[tuple(src)]
+        flags[f] = inside & nb
+    return mat, TABLE[mat], flags
+
+
+def solver(n, mat, flags, precision=VF.FP64):
+    s = VF.Solver(n, D, K0, precision)
+    s.set_medium(mat, TABLE, flags)
+    s.green_compute()
+    return s
+
+
+def oracle(n, eps, flags):
+    return O.Operator(n, D, K0, eps, flags, O.SCHED_FUSED)
+
+
+@pytest.mark.parametrize("axis,lo,hi", SLABS)
+def test_slab_operator_matches_oracle(axis, lo, hi):
+    mat, eps, flags = slab_scene(axis, lo, hi, 1)
+    op = oracle(N, eps, flags)
+    x = random_field(N, 2) * flags
+    with solver(N, mat, flags) as s:
+        for adj in (False, True):
+            assert rel(s.apply(x, adj), op.apply(x, adj)) <= 1e-12
+    with solver(N, mat, flags, VF.FP32) as s:
+        assert rel(s.apply(x), op.apply(x)) <= 3e-6
+
+
+@pytest.mark.parametrize("precision", [VF.FP64, VF.FP32])
+def test_slab_apply_async_equals_apply(precision):
+    import torch
+
+    mat, eps, flags = slab_scene(2, 6, 15, 3)
+    x = random_field(N, 4) * flags
+    dt = torch.complex128 if precision == VF.FP64 else torch.complex64
+    with solver(N, mat, flags, precision) as s:
+        ref = s.apply(x)
+        xt = torch.tensor(x, dtype=dt, device="cuda")
+        yt = torch.empty_like(xt)
+        s.apply_async(xt.data_ptr(), yt.data_ptr())
+        torch.cuda.synchronize()
+        got = yt.cpu().numpy()
+    assert rel(got, ref) <= (1e-14 if precision == VF.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("axis,lo,hi", SLABS)
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
+def test_slab_fixed_iterations_match_oracle(axis, lo, hi, method):
+    mat, eps, flags = slab_scene(axis, lo, hi, 5)
+    b = O.incident(N, D, K0, 20.0, 30.0, 40.0) * flags
+    ref = oracle(N, eps, flags).solve(b, method, tol=1e-300, maxit=8)
+    with solver(N, mat, flags) as s:
+        x, info = s.solve(b, method, tol=1e-300, maxit=8)
+    assert info["iterations"] == ref["iterations"] == 8
+    assert rel(x, ref["x"]) <= 1e-9, rel(x, ref["x"])
+
+
+@pytest.mark.parametrize("axis,lo,hi", SLABS)
+def test_slab_solve_and_outputs_match_oracle(axis, lo, hi):
+    mat, eps, flags = slab_scene(axis, lo, hi, 6)
+    op = oracle(N, eps, flags)
+    ang = (20.0, 30.0, 40.0)
+    b = O.incident(N, D, K0, *ang) * flags
+    ref = op.solve(b, 3, tol=1e-10, maxit=2000)
+    with solver(N, mat, flags) as s:
+        s.set_rhs_incident(*ang)  # built on the device in the internal axis order
+        info = s.solve_resident(VF.BICGSTAB, tol=1e-10)
+        xs = s.solution()
+        assert info["converged"] and info["true_relres"] < 1e-9
+        assert abs(info["iterations"] - ref["iterations"]) <= 1
+        assert rel(xs, ref["x"]) <= 1e-8
+        for th, ph, ps in ((0.0, 0.0, 0.0), (25.0, 40.0, 10.0), (70.0, 200.0, 80.0)):
+            fs = s.forward_scatter(th, ph, ps)
+            assert abs(fs - O.forward_scatter(N, D, K0, xs, th, ph, ps)) <= 1e-11 * abs(fs)
+            S = s.far_field([th], [ph])[0]
+            t, p, q = np.radians([th, ph, ps])
+            Z0 = 120 * np.pi
+            tm = Z0 * (S[0].real * np.cos(t) * np.cos(p) + S[1].real * np.cos(t) * np.sin(p) - S[2].real * np.sin(t))
+            te = Z0 * (S[0].real * np.sin(p) - S[1].real * np.cos(p))
+            assert abs(np.cos(q) * tm - np.sin(q) * te - fs) <= 1e-11 * abs(fs)
+        rt, rp = s.rcs(30.0)
+        ot, opp = O.rcs(N, D, K0, xs, 0.0, 30.0)
+        for got, want in ((rt, ot), (rp, opp)):
+            ok = np.isfinite(want) & (want > np.nanmax(want) - 100)
+            assert np.max(np.abs(got[ok] - want[ok])) < 1e-9
+        for m in (1, 2):
+            a = s.absorbed(m)
+            assert abs(a - O.absorbed(N, D, K0, xs, eps, mat, m)) <= 1e-11 * abs(a)
+        P = s.absorption()
+        assert abs(P - O.absorption(N, D, K0, xs, eps, flags)) <= 1e-12 * abs(P)
+        P2, gre, gim, ai = s.absorption_grad(tol=1e-13)
+        assert ai["converged"] and P2 == P
+        mixed = s.solve_mixed(tol=1e-10)
+        assert mixed["converged"] and mixed["true_relres"] < 1e-10
+        assert rel(s.solution(), xs) <= 1e-8
+    Po, gro, gio, _ = O.absorption_grad(op, xs, tol=1e-13)
+    assert gre.shape == N
+    assert rel(gre, gro) <= 1e-9 and rel(gim, gio) <= 1e-9
+
+
+def test_embedded_slab_matches_oracle():
+    n = (18, 14, 11)  # compiled as 20 x 16 x 12
+    assert not VF.supported_n(18) and not VF.supported_n(14) and not VF.supported_n(11)
+    mat, eps, flags = slab_scene(2, 3, 8, 7, n)
+    op = oracle(n, eps, flags)
+    x = random_field(n, 8) * flags
+    b = O.incident(n, D, K0) * flags
+    ref = op.solve(b, 3, tol=1e-10, maxit=2000)
+    with solver(n, mat, flags) as s:
+        assert rel(s.apply(x), op.apply(x)) <= 1e-12
+        assert rel(s.apply(x, True), op.apply(x, True)) <= 1e-12
+        xs, info = s.solve(b, VF.BICGSTAB, tol=1e-10)
+        assert abs(info["iterations"] - ref["iterations"]) <= 1
+        assert rel(xs, ref["x"]) <= 1e-8
+        assert abs(s.absorbed(1) - O.absorbed(n, D, K0, xs, eps, mat, 1)) <= 1e-11 * abs(s.absorbed(1))
+
+
+@pytest.mark.parametrize("axis,lo,hi", SLABS)
+def test_window_off_gives_the_same_solves(axis, lo, hi):
+    mat, eps, flags = slab_scene(axis, lo, hi, 9)
+    b = O.incident(N, D, K0, 10.0, 50.0, 0.0) * flags
+    out = {}
+    for w in ("1", "0"):
+        os.environ["VIE_WINDOW"] = w
+        try:
+            with solver(N, mat, flags) as s:
+                x8, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+                s.set_rhs(b)
+                mixed = s.solve_mixed(tol=1e-9)
+                out[w] = (x8, s.solution(), mixed)
+        finally:
+            os.environ.pop("VIE_WINDOW", None)
+    assert rel(out["1"][0], out["0"][0]) <= 1e-11
+    assert out["1"][2]["converged"] and out["0"][2]["converged"]
+    assert rel(out["1"][1], out["0"][1]) <= 1e-8

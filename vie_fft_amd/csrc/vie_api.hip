@@ -144,6 +144,7 @@ template <class R> struct Op {
     T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
     uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
+    int wx0 = 0, wx1 = 0;      // x window: the x planes with an active face
     T* W1 = nullptr;
     T* W2 = nullptr;
     bool own_w = true;
@@ -160,6 +161,8 @@ template <class R> struct Op {
         g.ny = n[1];
         g.nz = n[2];
         g.V = (long)n[0] * n[1] * n[2];
+        g.x0 = wx0 = 0;
+        g.x1 = wx1 = n[0];
         const double NP = 8.0 * (double)g.V;
         for (int a = 0; a < 3; ++a) {
             const int N = n[a];
@@ -284,30 +287,47 @@ template <class R> struct Op {
 
     // y = A z (or A^H z) where z is the Lin prologue; src is the vector the
     // diagonal term multiplies (z itself).  Returns the partial-sum count.
+    // With row ranges (lin.zr: the inputs vanish on inactive faces) the passes
+    // cover only the x window; outside it the output and lin.out are zero,
+    // written here unless the caller keeps them zero (wskip).
     long apply(const Lin<R>& lin, const T* src, T* y, Dots<R> dots, bool adj, const int* done,
                hipStream_t st, Timer* tm)
     {
         long nblk = 0;
+        Geo gw = g;
+        if (lin.zr) {
+            gw.x0 = wx0;
+            gw.x1 = wx1;
+        }
+        if ((gw.x0 > 0 || gw.x1 < g.nx) && !lin.wskip) {
+            const long pl = (long)g.ny * g.nz;
+            const unsigned gz = grid1(3 * (g.V - (gw.x1 - gw.x0) * pl));
+            if (lin.out)
+                hipLaunchKernelGGL(k_zero_outside<R>, dim3(gz), dim3(NT), 0, st, g.V, gw.x0 * pl, gw.x1 * pl,
+                                   lin.out, done);
+            hipLaunchKernelGGL(k_zero_outside<R>, dim3(gz), dim3(NT), 0, st, g.V, gw.x0 * pl, gw.x1 * pl, y, done);
+            HIPCHK(hipGetLastError());
+        }
         const int off = sizeof(R) == 8 ? 0 : 10;  // timing classes 1-5 FP64, 11-15 FP32
         auto t0 = [&](int c) { if (tm && tm->cls == c + off) tm->mark(st); };
         t0(1);
-        HIPCHK(run_zfwd<R>(g, lin, W1, tw[2], twist[2], twP[2], done, st));
+        HIPCHK(run_zfwd<R>(gw, lin, W1, tw[2], twist[2], twP[2], done, st));
         t0(1);
         t0(2);
-        HIPCHK(run_yfwd<R>(g, W1, W2, tw[1], twist[1], twP[1], done, st));
+        HIPCHK(run_yfwd<R>(gw, W1, W2, tw[1], twist[1], twP[1], done, st));
         t0(2);
         t0(3);
-        HIPCHK(run_xmix<R>(g, W2, tw[0], twist[0], twP[0], adj ? ma : mf, done, st));
+        HIPCHK(run_xmix<R>(gw, W2, tw[0], twist[0], twP[0], adj ? ma : mf, done, st));
         t0(3);
         t0(4);
-        HIPCHK(run_yinv<R>(g, W2, W1, tw[1], twist[1], twP[1], done, st));
+        HIPCHK(run_yinv<R>(gw, W2, W1, tw[1], twist[1], twP[1], done, st));
         t0(4);
         Diag<R> dd{};
         dd.d = dg;
         dd.conj = adj ? 1 : 0;
         dd.skip = lin.wskip;
         t0(5);
-        HIPCHK(run_zinv<R>(g, W1, src, y, dd, flags, dots, tw[2], twist[2], twP[2], done, st, &nblk));
+        HIPCHK(run_zinv<R>(gw, W1, src, y, dd, flags, dots, tw[2], twist[2], twP[2], done, st, &nblk));
         t0(5);
         return nblk;
     }
@@ -342,7 +362,16 @@ template <class R> Dots<R> nodots()
 struct vie_ctx {
     int nu[3] = {0, 0, 0};  // grid as the caller sees it (desc.n: the compiled grid it is embedded in)
     long Vu = 0;
-    bool emb = false;         // nu != desc.n
+    bool emb = false;         // the caller's grid is embedded in a larger compiled one
+    // Internal axis order: internal axis a is the caller's axis perm[a]
+    // (desc.n and desc.d are in internal order); set_medium puts the axis
+    // whose active faces span the fewest planes first, so the passes' x
+    // window is the narrowest.
+    int perm[3] = {0, 1, 2};
+    int nr[3] = {0, 0, 0};     // compiled grid in the caller's axis order
+    double dr[3] = {0, 0, 0};  // cell sizes in the caller's axis order
+    bool remap = false;        // caller layout != internal layout (embedding or axis order)
+    bool windows = true;       // x windows and the axis order that favours them (VIE_WINDOW=0: off)
     vie_desc desc{};
     hipStream_t stream = nullptr;
     std::string err;
@@ -542,6 +571,24 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
                        cim[2], c->eps_dev, op->dg);
     HIPCHK(hipGetLastError());
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
+    // x window: the x planes holding an active face of any component (all of
+    // them when windows are off or no face is active)
+    const long pl = (long)n[1] * n[2];
+    int lo = n[0], hi = 0;
+    for (int f = 0; f < 3; ++f)
+        for (int i = 0; i < n[0]; ++i) {
+            const uint8_t* p = c->flags_h.data() + f * c->V + i * pl;
+            if (std::any_of(p, p + pl, [](uint8_t v) { return v != 0; })) {
+                lo = std::min(lo, i);
+                hi = std::max(hi, i + 1);
+            }
+        }
+    if (hi <= lo || !c->windows) {
+        lo = 0;
+        hi = n[0];
+    }
+    op->wx0 = lo;
+    op->wx1 = hi;
     const long rows = 3L * n[0] * n[1];
     hipLaunchKernelGGL(k_zrange, dim3(grid1(rows)), dim3(NT), 0, c->stream, rows, n[2], op->flags, op->zr);
     HIPCHK(hipGetLastError());
@@ -1145,36 +1192,108 @@ template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjo
 
 }  // namespace
 
-// Caller-grid <-> compiled-grid copies on the host: the caller's grid sits at
-// the origin of the compiled one, every other cell takes `fill`.
+Remap remap_of(const vie_ctx* c)
+{
+    Remap m{};
+    for (int a = 0; a < 3; ++a) {
+        m.n[a] = c->desc.n[a];
+        m.u[a] = c->nu[a];
+        m.perm[a] = c->perm[a];
+        m.iperm[c->perm[a]] = a;
+    }
+    m.V = c->V;
+    m.Vu = c->Vu;
+    return m;
+}
+
+// Caller layout <-> internal layout on the host (k_embed / k_crop): the
+// caller's grid sits at the origin of the internal one with its axes in the
+// order perm, every other cell takes `fill`.
 template <class T> void embed_h(const vie_ctx* c, const T* in, T* out, int ncomp, T fill)
 {
-    const int* n = c->desc.n;
-    const int* u = c->nu;
-    for (int cp = 0; cp < ncomp; ++cp)
-        for (int i = 0; i < n[0]; ++i)
-            for (int j = 0; j < n[1]; ++j) {
-                T* o = out + cp * c->V + ((long)i * n[1] + j) * n[2];
-                if (i < u[0] && j < u[1]) {
-                    const T* src = in + cp * c->Vu + ((long)i * u[1] + j) * u[2];
-                    std::copy(src, src + u[2], o);
-                    std::fill(o + u[2], o + n[2], fill);
+    const Remap m = remap_of(c);
+    const long cs[3] = {(long)m.u[1] * m.u[2], m.u[2], 1};
+    const long s0 = cs[m.perm[0]], s1 = cs[m.perm[1]], s2 = cs[m.perm[2]];
+    const int e0 = m.u[m.perm[0]], e1 = m.u[m.perm[1]], e2 = m.u[m.perm[2]];
+    for (int cp = 0; cp < ncomp; ++cp) {
+        const T* src = in + (ncomp == 3 ? m.perm[cp] : cp) * m.Vu;
+        T* o = out + cp * m.V;
+        for (int i = 0; i < m.n[0]; ++i)
+            for (int j = 0; j < m.n[1]; ++j) {
+                T* row = o + ((long)i * m.n[1] + j) * m.n[2];
+                if (i < e0 && j < e1) {
+                    const T* sr = src + i * s0 + j * s1;
+                    for (int k = 0; k < e2; ++k) row[k] = sr[k * s2];
+                    std::fill(row + e2, row + m.n[2], fill);
                 } else {
-                    std::fill(o, o + n[2], fill);
+                    std::fill(row, row + m.n[2], fill);
                 }
             }
+    }
 }
 template <class T> void crop_h(const vie_ctx* c, const T* in, T* out, int ncomp)
 {
-    const int* n = c->desc.n;
+    const Remap m = remap_of(c);
+    const long is[3] = {(long)m.n[1] * m.n[2], m.n[2], 1};
+    const long s0 = is[m.iperm[0]], s1 = is[m.iperm[1]], s2 = is[m.iperm[2]];
+    for (int cp = 0; cp < ncomp; ++cp) {
+        const T* src = in + (ncomp == 3 ? m.iperm[cp] : cp) * m.V;
+        T* o = out + cp * m.Vu;
+        for (int i = 0; i < m.u[0]; ++i)
+            for (int j = 0; j < m.u[1]; ++j) {
+                const T* sr = src + i * s0 + j * s1;
+                T* row = o + ((long)i * m.u[1] + j) * m.u[2];
+                for (int k = 0; k < m.u[2]; ++k) row[k] = sr[k * s2];
+            }
+    }
+}
+
+// Internal axis order for a medium with these (caller-layout) flags: the
+// axis whose active faces span the smallest fraction of its planes first,
+// the other two in the caller's order; the caller's order on ties, when no
+// face is active or with windows off.
+void choose_frame(const vie_ctx* c, const unsigned char* flags, int* perm)
+{
+    perm[0] = 0;
+    perm[1] = 1;
+    perm[2] = 2;
+    if (!c->windows) return;
     const int* u = c->nu;
-    for (int cp = 0; cp < ncomp; ++cp)
+    int lo[3] = {u[0], u[1], u[2]}, hi[3] = {-1, -1, -1};
+    for (int f = 0; f < 3; ++f)
         for (int i = 0; i < u[0]; ++i)
             for (int j = 0; j < u[1]; ++j) {
-                const T* src = in + cp * c->V + ((long)i * n[1] + j) * n[2];
-                std::copy(src, src + u[2], out + cp * c->Vu + ((long)i * u[1] + j) * u[2]);
+                const unsigned char* row = flags + f * c->Vu + ((long)i * u[1] + j) * u[2];
+                int k0 = -1, k1 = -1;
+                for (int k = 0; k < u[2]; ++k)
+                    if (row[k]) {
+                        if (k0 < 0) k0 = k;
+                        k1 = k;
+                    }
+                if (k0 < 0) continue;
+                lo[0] = std::min(lo[0], i);
+                hi[0] = std::max(hi[0], i);
+                lo[1] = std::min(lo[1], j);
+                hi[1] = std::max(hi[1], j);
+                lo[2] = std::min(lo[2], k0);
+                hi[2] = std::max(hi[2], k1);
             }
+    if (hi[0] < 0) return;
+    int best = 0;
+    double fb = 2.0;
+    for (int b = 0; b < 3; ++b) {
+        const double fr = (double)(hi[b] - lo[b] + 1) / c->nr[b];
+        if (fr < fb) {
+            fb = fr;
+            best = b;
+        }
+    }
+    int k = 1;
+    perm[0] = best;
+    for (int b = 0; b < 3; ++b)
+        if (b != best) perm[k++] = b;
 }
+
 using cd = std::complex<double>;
 
 // On the caller's grid the own-axis last plane of each component is outside
@@ -1224,9 +1343,11 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
     c->desc = *desc;
     for (int a = 0; a < 3; ++a) {
         c->nu[a] = desc->n[a];
-        c->desc.n[a] = nint[a];
+        c->nr[a] = c->desc.n[a] = nint[a];
+        c->dr[a] = desc->d[a];
         c->emb = c->emb || nint[a] != desc->n[a];
     }
+    c->remap = c->emb;
     c->V = (long)nint[0] * nint[1] * nint[2];
     c->Vu = (long)desc->n[0] * desc->n[1] * desc->n[2];
     try {
@@ -1310,37 +1431,78 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
     return VIE_OK;
 }
 
+// Switches the internal axis order to perm.  The operators are rebuilt for the
+// permuted grid (and G~ recomputed when it was computed); the medium,
+// right-hand side and solution of the old order are dropped.
+static void set_frame(vie_ctx* c, const int* perm)
+{
+    if (perm[0] == c->perm[0] && perm[1] == c->perm[1] && perm[2] == c->perm[2]) return;
+    const bool had64 = c->op64 != nullptr, had32 = c->op32 != nullptr, green = c->green_ready;
+    delete c->op32;  // shares buffers owned by op64: delete first
+    delete c->op64;
+    c->op32 = nullptr;
+    c->op64 = nullptr;
+    bool ident = true;
+    for (int a = 0; a < 3; ++a) {
+        c->perm[a] = perm[a];
+        c->desc.n[a] = c->nr[perm[a]];
+        c->desc.d[a] = c->dr[perm[a]];
+        ident = ident && perm[a] == a;
+    }
+    c->remap = c->emb || !ident;
+    c->green_ready = c->medium_ready = c->rhs_ready = c->sol_ready = false;
+    const int* ni = c->desc.n;
+    ensure_partial(c, 3 * std::max<long>(std::max(zinv_blocks<double>(Geo{ni[0], ni[1], ni[2], c->V}),
+                                                  zinv_blocks<float>(Geo{ni[0], ni[1], ni[2], c->V})),
+                                         4096));
+    if (had64) make_op<double>(c);
+    if (had32) make_op<float>(c);
+    if (green) green_compute(c);
+}
+
+// Checks the caller-layout flags of an embedded grid and picks the internal
+// axis order for them.  A medium rejected later by set_medium_common leaves
+// the context without a medium, as after any failed set_medium.
+static void prepare_frame(vie_ctx* c, const unsigned char* flags)
+{
+    if (c->emb) check_last_planes(c, flags);
+    const char* e = std::getenv("VIE_WINDOW");
+    c->windows = !(e && e[0] == '0');
+    int perm[3];
+    choose_frame(c, flags, perm);
+    set_frame(c, perm);
+}
+
+static int set_medium_int(vie_ctx* c, const unsigned char* material, const double* eps_table, int n_mat,
+                          const unsigned char* flags)
+{
+    c->eps_h.resize(2 * c->V);
+    for (long v = 0; v < c->V; ++v) {
+        const int m = material[v];
+        if (m >= n_mat) throw Err(VIE_ERR_ARG, "material id outside the eps table");
+        c->eps_h[2 * v] = eps_table[2 * m];
+        c->eps_h[2 * v + 1] = eps_table[2 * m + 1];
+    }
+    require(c, n_mat <= 256, "at most 256 materials");
+    dfree(c->mat_dev);
+    c->mat_dev = dalloc<uint8_t>(c->V);
+    copy_sync(c->mat_dev, material, c->V, hipMemcpyHostToDevice, c->stream);
+    return set_medium_common(c, flags);
+}
+
 int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_table, int n_mat,
                    const unsigned char* flags)
 {
     if (!c || !material || !eps_table || !flags || n_mat < 1) return VIE_ERR_ARG;
-    if (c->emb) {  // background (id 0) and inactive faces on the embedding cells
-        try {
-            check_last_planes(c, flags);
-        } catch (const Err& e) {
-            return fail(c, e);
-        }
-        std::vector<unsigned char> m(c->V), f(3 * c->V);
-        embed_h<unsigned char>(c, material, m.data(), 1, 0);
-        embed_h<unsigned char>(c, flags, f.data(), 3, 0);
-        c->emb = false;
-        const int rc = vie_set_medium(c, m.data(), eps_table, n_mat, f.data());
-        c->emb = true;
-        return rc;
-    }
     try {
-        c->eps_h.resize(2 * c->V);
-        for (long v = 0; v < c->V; ++v) {
-            const int m = material[v];
-            if (m >= n_mat) throw Err(VIE_ERR_ARG, "material id outside the eps table");
-            c->eps_h[2 * v] = eps_table[2 * m];
-            c->eps_h[2 * v + 1] = eps_table[2 * m + 1];
+        prepare_frame(c, flags);
+        if (c->remap) {  // background (id 0) and inactive faces on the embedding cells
+            std::vector<unsigned char> m(c->V), f(3 * c->V);
+            embed_h<unsigned char>(c, material, m.data(), 1, 0);
+            embed_h<unsigned char>(c, flags, f.data(), 3, 0);
+            return set_medium_int(c, m.data(), eps_table, n_mat, f.data());
         }
-        require(c, n_mat <= 256, "at most 256 materials");
-        dfree(c->mat_dev);
-        c->mat_dev = dalloc<uint8_t>(c->V);
-        copy_sync(c->mat_dev, material, c->V, hipMemcpyHostToDevice, c->stream);
-        return set_medium_common(c, flags);
+        return set_medium_int(c, material, eps_table, n_mat, flags);
     } catch (const Err& e) {
         return fail(c, e);
     }
@@ -1349,22 +1511,17 @@ int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_
 int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags)
 {
     if (!c || !eps || !flags) return VIE_ERR_ARG;
-    if (c->emb) {  // dummy background permittivity (vie_fft/Model.h:82) and inactive faces outside
-        try {
-            check_last_planes(c, flags);
-        } catch (const Err& e) {
-            return fail(c, e);
-        }
-        std::vector<cd> e(c->V);
-        std::vector<unsigned char> f(3 * c->V);
-        embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
-        embed_h<unsigned char>(c, flags, f.data(), 3, 0);
-        c->emb = false;
-        const int rc = vie_set_medium_eps(c, (const double*)e.data(), f.data());
-        c->emb = true;
-        return rc;
-    }
     try {
+        prepare_frame(c, flags);
+        if (c->remap) {  // dummy background permittivity (vie_fft/Model.h:82) and inactive faces outside
+            std::vector<cd> e(c->V);
+            std::vector<unsigned char> f(3 * c->V);
+            embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
+            embed_h<unsigned char>(c, flags, f.data(), 3, 0);
+            c->eps_h.assign((const double*)e.data(), (const double*)e.data() + 2 * c->V);
+            dfree(c->mat_dev);
+            return set_medium_common(c, f.data());
+        }
         c->eps_h.assign(eps, eps + 2 * c->V);
         dfree(c->mat_dev);
         return set_medium_common(c, flags);
@@ -1411,8 +1568,8 @@ int vie_green_compute(vie_ctx* c)
 int vie_incident(vie_ctx* c, double theta, double phi, double psi, double* E)
 {
     if (!c || !E) return VIE_ERR_ARG;
-    const int* n = c->nu;  // the caller's grid
-    const double* d = c->desc.d;
+    const int* n = c->nu;     // the caller's grid
+    const double* d = c->dr;  // and cell sizes
     const double k0 = c->desc.k0;
     const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
     const double kv[3] = {k0 * std::sin(th) * std::cos(ph), k0 * std::sin(th) * std::sin(ph), k0 * std::cos(th)};
@@ -1451,30 +1608,25 @@ int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* strea
     try {
         check_ready(c);
         hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-        const int* n = c->desc.n;
-        const int* u = c->nu;
+        const Remap m = remap_of(c);
         const unsigned ge = grid1(3 * c->V), gc = grid1(3 * c->Vu);
         if (c->desc.precision == VIE_FP64) {
-            if (!c->emb) {
+            if (!c->remap) {
                 apply_dev<double>(c, (const double2*)x, (double2*)y, adjoint, st);
             } else {
-                hipLaunchKernelGGL(k_embed<double2>, dim3(ge), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
-                                   (const double2*)x, c->t64);
+                hipLaunchKernelGGL(k_embed<double2>, dim3(ge), dim3(NT), 0, st, m, 3, (const double2*)x, c->t64);
                 apply_dev<double>(c, c->t64, c->r64, adjoint, st);
-                hipLaunchKernelGGL(k_crop<double2>, dim3(gc), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
-                                   c->r64, (double2*)y);
+                hipLaunchKernelGGL(k_crop<double2>, dim3(gc), dim3(NT), 0, st, m, 3, c->r64, (double2*)y);
             }
         } else {
-            if (!c->emb) {
+            if (!c->remap) {
                 apply_dev<float>(c, (const float2*)x, (float2*)y, adjoint, st);
             } else {
                 float2* xs = vec<float>(c, VT);
                 float2* ys = vec<float>(c, VT2);
-                hipLaunchKernelGGL(k_embed<float2>, dim3(ge), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3,
-                                   (const float2*)x, xs);
+                hipLaunchKernelGGL(k_embed<float2>, dim3(ge), dim3(NT), 0, st, m, 3, (const float2*)x, xs);
                 apply_dev<float>(c, xs, ys, adjoint, st);
-                hipLaunchKernelGGL(k_crop<float2>, dim3(gc), dim3(NT), 0, st, n[0], n[1], n[2], u[0], u[1], u[2], 3, ys,
-                                   (float2*)y);
+                hipLaunchKernelGGL(k_crop<float2>, dim3(gc), dim3(NT), 0, st, m, 3, ys, (float2*)y);
             }
         }
         HIPCHK(hipGetLastError());
@@ -1489,15 +1641,15 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
     if (!c || !x || !y) return VIE_ERR_ARG;
     try {
         check_ready(c);
-        const long n = 3 * c->V;
-        std::vector<cd> xe, ye;
-        if (c->emb) {
-            xe.resize(n);
-            ye.resize(n);
-            embed_h<cd>(c, (const cd*)x, xe.data(), 3, cd(0, 0));
-            x = (const double*)xe.data();
+        const long n = 3 * c->V, nu3 = 3 * c->Vu;
+        const Remap m = remap_of(c);
+        if (c->remap) {  // caller layout staged in r64, internal layout in t64
+            copy_sync(c->r64, x, nu3 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+            hipLaunchKernelGGL(k_embed<double2>, dim3(grid1(n)), dim3(NT), 0, c->stream, m, 3, c->r64, c->t64);
+            HIPCHK(hipGetLastError());
+        } else {
+            copy_sync(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         }
-        copy_sync(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         if (c->desc.precision == VIE_FP64) {
             apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
         } else {
@@ -1507,9 +1659,10 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             apply_dev<float>(c, xs, ys, adjoint, c->stream);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
         }
-        if (c->emb) {
-            copy_sync(ye.data(), c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
-            crop_h<cd>(c, ye.data(), (cd*)y, 3);
+        if (c->remap) {
+            hipLaunchKernelGGL(k_crop<double2>, dim3(grid1(nu3)), dim3(NT), 0, c->stream, m, 3, c->r64, c->t64);
+            HIPCHK(hipGetLastError());
+            copy_sync(y, c->t64, nu3 * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         } else {
             copy_sync(y, c->r64, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         }
@@ -1532,14 +1685,18 @@ int vie_set_rhs_incident(vie_ctx* c, double theta, double phi, double psi)
         const double pol[3] = {std::cos(th) * std::cos(ph) * std::cos(ps) - std::sin(ph) * std::sin(ps),
                                std::cos(th) * std::sin(ph) * std::cos(ps) + std::cos(ph) * std::sin(ps),
                                -std::sin(th) * std::cos(ps)};
+        // wave vector and per-component factors along the internal axes
+        double ki[3];
         double2 seg[3];
-        for (int cpt = 0; cpt < 3; ++cpt) {
-            const double s = std::fabs(kv[cpt]) < kEps ? d[cpt] : 2.0 * std::sin(kv[cpt] * d[cpt] / 2.0) / kv[cpt];
-            seg[cpt] = make_double2(s * pol[cpt], 0.0);
+        for (int a = 0; a < 3; ++a) {
+            const int b = c->perm[a];
+            ki[a] = kv[b];
+            const double s = std::fabs(kv[b]) < kEps ? d[a] : 2.0 * std::sin(kv[b] * d[a] / 2.0) / kv[b];
+            seg[a] = make_double2(s * pol[b], 0.0);
         }
         Op<double>* op = make_op<double>(c);
         hipLaunchKernelGGL(k_incident, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2],
-                           kv[0], kv[1], kv[2], seg[0], seg[1], seg[2], op->flags, c->b64);
+                           ki[0], ki[1], ki[2], seg[0], seg[1], seg[2], op->flags, c->b64);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         c->rhs_ready = true;
@@ -1558,13 +1715,14 @@ int vie_set_rhs(vie_ctx* c, const double* b)
         const long n = 3 * c->V;
         for (long i = 0; i < 6 * c->Vu; ++i)
             if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
-        std::vector<cd> be;
-        if (c->emb) {
-            be.resize(n);
-            embed_h<cd>(c, (const cd*)b, be.data(), 3, cd(0, 0));
-            b = (const double*)be.data();
+        if (c->remap) {  // caller layout staged in r64
+            copy_sync(c->r64, b, 3 * c->Vu * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+            hipLaunchKernelGGL(k_embed<double2>, dim3(grid1(n)), dim3(NT), 0, c->stream, remap_of(c), 3, c->r64,
+                               c->t64);
+            HIPCHK(hipGetLastError());
+        } else {
+            copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         }
-        copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         Op<double>* op = make_op<double>(c);
         // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
         hipLaunchKernelGGL(k_mask<double>, dim3(grid1(n, NT, 1L << 30)), dim3(NT), 0, c->stream, n, c->t64, op->flags,
@@ -1771,10 +1929,12 @@ int vie_get_solution(vie_ctx* c, double* x)
     if (!c || !x) return VIE_ERR_ARG;
     try {
         require(c, c->sol_ready, "no solution on the device");
-        if (c->emb) {
-            std::vector<cd> xe(3 * c->V);
-            copy_sync(xe.data(), c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
-            crop_h<cd>(c, xe.data(), (cd*)x, 3);
+        if (c->remap) {  // caller layout staged in t64
+            const long nu3 = 3 * c->Vu;
+            hipLaunchKernelGGL(k_crop<double2>, dim3(grid1(nu3)), dim3(NT), 0, c->stream, remap_of(c), 3, c->x64,
+                               c->t64);
+            HIPCHK(hipGetLastError());
+            copy_sync(x, c->t64, nu3 * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         } else {
             copy_sync(x, c->x64, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
         }
@@ -1829,7 +1989,8 @@ int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double
         const int* n = c->desc.n;
         const double* d = c->desc.d;
         const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
-        const double vd[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+        const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+        const double vd[3] = {vc[c->perm[0]], vc[c->perm[1]], vc[c->perm[2]]};  // internal axes
         const unsigned gsz = grid1(c->V, NT, 1024);
         hipLaunchKernelGGL(k_farsum, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], c->desc.k0,
                            vd[0], vd[1], vd[2], c->x64, c->partial);
@@ -1842,7 +2003,7 @@ int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double
         for (int cpt = 0; cpt < 3; ++cpt) {
             std::complex<double> acc = 0;
             for (unsigned b = 0; b < gsz; ++b) acc += std::complex<double>(h[cpt * gsz + b].x, h[cpt * gsz + b].y);
-            S[cpt] = std::polar(1.0, c->desc.k0 * 0.5 * d[cpt] * vd[cpt]) * acc * dV;
+            S[c->perm[cpt]] = std::polar(1.0, c->desc.k0 * 0.5 * d[cpt] * vd[cpt]) * acc * dV;
         }
         const double tm = kZ0 * (S[0].real() * std::cos(th) * std::cos(ph) + S[1].real() * std::cos(th) * std::sin(ph) -
                                  S[2].real() * std::sin(th));
@@ -1866,9 +2027,8 @@ static void far_field_dev(vie_ctx* c, int ndir, const double* theta_deg, const d
     std::vector<double> vh(3 * ndir);
     for (int q = 0; q < ndir; ++q) {
         const double th = kPi * theta_deg[q] / 180.0, ph = kPi * phi_deg[q] / 180.0;
-        vh[3 * q] = std::sin(th) * std::cos(ph);
-        vh[3 * q + 1] = std::sin(th) * std::sin(ph);
-        vh[3 * q + 2] = std::cos(th);
+        const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+        for (int a = 0; a < 3; ++a) vh[3 * q + a] = vc[c->perm[a]];  // internal axes
     }
     const long npen = (long)n[0] * n[1];
     const unsigned nblk = (unsigned)std::min<long>(npen, 1024);
@@ -1890,7 +2050,7 @@ static void far_field_dev(vie_ctx* c, int ndir, const double* theta_deg, const d
         const double dV = d[0] * d[1] * d[2];
         for (int q = 0; q < ndir; ++q)
             for (int cpt = 0; cpt < 3; ++cpt)
-                S[3 * q + cpt] = std::polar(1.0, k0 * 0.5 * d[cpt] * vh[3 * q + cpt]) *
+                S[3 * q + c->perm[cpt]] = std::polar(1.0, k0 * 0.5 * d[cpt] * vh[3 * q + cpt]) *
                                  std::complex<double>(h[cpt * ndir + q].x, h[cpt * ndir + q].y) * dV;
     } catch (...) {
         dfree(vdev);
@@ -1960,8 +2120,9 @@ int vie_absorbed(vie_ctx* c, int mat, double* value)
         const int* n = c->desc.n;
         const double* d = c->desc.d;
         const unsigned gsz = grid1(c->V, NT, 1024);
-        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[0], c->nu[1],
-                           c->nu[2], d[0] * d[1] * d[2], c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat,
+        const int* p = c->perm;  // the caller's extents along the internal axes
+        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[p[0]], c->nu[p[1]],
+                           c->nu[p[2]], d[0] * d[1] * d[2], c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat,
                            c->partial);
         HIPCHK(hipGetLastError());
         std::vector<double2> h(gsz);

@@ -184,6 +184,19 @@ __global__ void k_fill_zero(long n, cx<R>* x)
         x[t] = czero<cx<R>>();
 }
 
+// Zero a 3-component field (V entries per component) outside the entries
+// [lo, hi) of each component: the x planes outside a product's x window.
+template <class R>
+__global__ void k_zero_outside(long V, long lo, long hi, cx<R>* x, const int* done)
+{
+    if (done && *done) return;
+    const long out = V - (hi - lo);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < 3 * out; t += (long)gridDim.x * blockDim.x) {
+        const long c = t / out, r = t % out;
+        x[c * V + (r < lo ? r : r + (hi - lo))] = czero<cx<R>>();
+    }
+}
+
 // x64 += s * e (e in the inner precision): the correction update of the
 // FP64 iterative refinement.
 template <class R>
@@ -457,33 +470,51 @@ __global__ __launch_bounds__(NT) void k_absorb(int n0, int n1, int n2, int b0, i
 }
 
 
-// ------------------------------------------------- user grid <-> compiled grid
-// A grid whose axis lengths have no compiled pass kernel is embedded at the
-// origin of the next compiled lengths; the extra cells are inactive
-// background.  k_embed writes ncomp fields of the compiled grid (n) from the
-// user grid (u), zero outside; k_crop copies the user cells back.
+// ------------------------------------------------- caller grid <-> internal grid
+// The internal grid holds the caller's grid at its origin with the axes in the
+// order perm (internal axis a is the caller's axis perm[a], and component a,
+// the faces normal to that axis, is the caller's component perm[a]); a length
+// without compiled pass kernels is embedded in the next compiled one, whose
+// extra cells are inactive background.
+struct Remap {
+    int n[3];      // internal grid
+    int u[3];      // caller grid, caller axis order
+    int perm[3];   // internal axis -> caller axis
+    int iperm[3];  // caller axis -> internal axis
+    long V, Vu;
+};
+
+// k_embed writes ncomp internal fields from the caller's (zero on the
+// embedding cells); k_crop writes the caller's cells back.  A field of three
+// components follows perm, a single one is component 0 on both sides.
 template <class T>
-__global__ __launch_bounds__(NT) void k_embed(int n0, int n1, int n2, int u0, int u1, int u2, int ncomp,
-                                              const T* __restrict__ in, T* __restrict__ out)
+__global__ __launch_bounds__(NT) void k_embed(Remap m, int ncomp, const T* __restrict__ in, T* __restrict__ out)
 {
-    const long V = (long)n0 * n1 * n2, Vu = (long)u0 * u1 * u2;
-    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * V; e += (long)gridDim.x * NT) {
-        const long c = e / V, v = e % V;
-        const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), k = (int)(v % n2);
+    const long cs[3] = {(long)m.u[1] * m.u[2], m.u[2], 1};
+    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * m.V; e += (long)gridDim.x * NT) {
+        const long c = e / m.V, v = e % m.V;
+        const int i[3] = {(int)(v / ((long)m.n[1] * m.n[2])), (int)((v / m.n[2]) % m.n[1]), (int)(v % m.n[2])};
+        bool in_grid = true;
+        long off = (ncomp == 3 ? m.perm[c] : c) * m.Vu;
+        for (int a = 0; a < 3; ++a) {
+            in_grid = in_grid && i[a] < m.u[m.perm[a]];
+            off += i[a] * cs[m.perm[a]];
+        }
         T r{};
-        if (i < u0 && j < u1 && k < u2) r = in[c * Vu + ((long)i * u1 + j) * u2 + k];
+        if (in_grid) r = in[off];
         out[e] = r;
     }
 }
 template <class T>
-__global__ __launch_bounds__(NT) void k_crop(int n0, int n1, int n2, int u0, int u1, int u2, int ncomp,
-                                             const T* __restrict__ in, T* __restrict__ out)
+__global__ __launch_bounds__(NT) void k_crop(Remap m, int ncomp, const T* __restrict__ in, T* __restrict__ out)
 {
-    const long V = (long)n0 * n1 * n2, Vu = (long)u0 * u1 * u2;
-    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * Vu; e += (long)gridDim.x * NT) {
-        const long c = e / Vu, v = e % Vu;
-        const int i = (int)(v / ((long)u1 * u2)), j = (int)((v / u2) % u1), k = (int)(v % u2);
-        out[e] = in[c * V + ((long)i * n1 + j) * n2 + k];
+    const long is[3] = {(long)m.n[1] * m.n[2], m.n[2], 1};
+    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < ncomp * m.Vu; e += (long)gridDim.x * NT) {
+        const long c = e / m.Vu, v = e % m.Vu;
+        const int r[3] = {(int)(v / ((long)m.u[1] * m.u[2])), (int)((v / m.u[2]) % m.u[1]), (int)(v % m.u[2])};
+        long off = (ncomp == 3 ? m.iperm[c] : c) * m.V;
+        for (int b = 0; b < 3; ++b) off += r[b] * is[m.iperm[b]];
+        out[e] = in[off];
     }
 }
 

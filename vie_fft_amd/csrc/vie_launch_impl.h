@@ -61,7 +61,7 @@ template <class R, int n>
 static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
-    const long rows = (long)g.nx * g.ny;
+    const long rows = (long)(g.x1 - g.x0) * g.ny;  // the x window's rows
     if constexpr (YZReg<R, n>::ok) {
         if (zfwd_reg() && !yz_lds_forced()) {
             using X = YZReg<R, n>;
@@ -86,14 +86,14 @@ static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nz + B - 1) / B;
-            dim3 grid((unsigned)(g.nx * nzt), 3);
+            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
             hipLaunchKernelGGL((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, W2, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nz + B - 1) / B;
-    dim3 grid((unsigned)(g.nx * nzt), 3);
+    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
     hipLaunchKernelGGL((k_yfwd<R, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done);
     return hipGetLastError();
 }
@@ -107,14 +107,14 @@ static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nz + B - 1) / B;
-            dim3 grid((unsigned)(g.nx * nzt), 3);
+            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
             hipLaunchKernelGGL((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nz + B - 1) / B;
-    dim3 grid((unsigned)(g.nx * nzt), 3);
+    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
     hipLaunchKernelGGL((k_yinv<R, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done);
     return hipGetLastError();
 }
@@ -124,7 +124,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
                               const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st, long* nblk)
 {
-    const long rows = (long)g.nx * g.ny;
+    const long rows = (long)(g.x1 - g.x0) * g.ny;  // the x window's rows
     if constexpr (ZInvReg<R, n>::ok) {
         if (!yz_lds_forced() && dg.d) {
             using X = ZInvReg<R, n>;

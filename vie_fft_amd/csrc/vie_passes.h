@@ -14,9 +14,14 @@
 
 namespace vie {
 
+// [x0, x1) is the x window the passes cover (0 <= x0 < x1 <= nx, set by the
+// owner of the Geo): the x planes outside it carry no active face, so a
+// product whose input vanishes there needs neither its z/y transforms on
+// those planes nor its x-pencil entries there.
 struct Geo {
     int nx, ny, nz;
     long V;
+    int x0 = 0, x1 = 0;
 };
 
 // Tables of the spectral product, per axis a in split order t = e*N_a + m
@@ -149,14 +154,15 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     if (done && *done) return;
     const int s = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
-    const long row0 = (long)xcd_tile(blockIdx.x, gridDim.x) * B;
+    const long rend = (long)g.x1 * g.ny;  // rows of the x window: [x0 Ny, x1 Ny)
+    const long row0 = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
     __shared__ unsigned zrs[B];  // active range of each row (the whole row without lin.zr)
     for (int t = threadIdx.x; t < B; t += NT)
-        zrs[t] = lin.zr && row0 + t < rows ? lin.zr[s * rows + row0 + t] : ((unsigned)N << 16);
+        zrs[t] = lin.zr && row0 + t < rend ? lin.zr[s * rows + row0 + t] : ((unsigned)N << 16);
     T c[3];
     load_coef<R>(lin.coef, c);
     __syncthreads();
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             x0[it][0] = x0[it][1] = x1[it][0] = x1[it][1] = x2[it][0] = x2[it][1] = czero<T>();
             // with row ranges, pairs that miss the row's active range are zero
             const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
-            if (e < B * N / 2 && row < rows && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
+            if (e < B * N / 2 && row < rend && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
                 const long idx = s * g.V + row * N + n;
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int col = e / (N / 2), n = 2 * (e % (N / 2));
             const long row = row0 + col;
             T v0 = czero<T>(), v1 = czero<T>();
-            if (row < rows) {
+            if (row < rend) {
                 const long idx = s * g.V + row * N + n;
                 v0 = cmul(c[0], x0[it][0]);
                 v1 = cmul(c[0], x0[it][1]);
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int col = e / N, n = e % N;
             const long row = row0 + col;
             T v = czero<T>();
-            if (row < rows) {
+            if (row < rend) {
                 const long idx = s * g.V + row * N + n;
                 const unsigned rg = zrs[col];
                 if (n >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     for (int e = threadIdx.x; e < B * N; e += NT) {
         const int col = e / N, t = 2 * (e % N);
         const long row = row0 + col;
-        if (row < rows) {
+        if (row < rend) {
             const int ep = t >= N, m = t - ep * N;
             T* dst = W1 + (s * rows + row) * (2 * N) + t;
             if constexpr (N % 2 == 0)
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
     const int P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
@@ -348,7 +354,8 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
         const int s = e / (N * B), r = e % (N * B);
         const int i = r / B, col = r % B;
-        T v = (z0 + col < P2) ? W2[(((long)s * g.nx + i) * P1 + ty) * P2 + z0 + col] : czero<T>();
+        const bool in = z0 + col < P2 && i >= g.x0 && i < g.x1;
+        T v = in ? W2[(((long)s * g.nx + i) * P1 + ty) * P2 + z0 + col] : czero<T>();
         buf[i * LD + (2 * s) * B + col] = v;
         buf[i * LD + (2 * s + 1) * B + col] = cmul(v, tws[i]);
     }
@@ -376,7 +383,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
         const int s = e / (N * B), r = e % (N * B);
         const int i = r / B, col = r % B;
-        if (z0 + col < P2) {
+        if (z0 + col < P2 && i >= g.x0 && i < g.x1) {
             const T a = buf[i * LD + (2 * s) * B + col];
             const T b = buf[i * LD + (2 * s + 1) * B + col];
             W2[(((long)s * g.nx + i) * P1 + ty) * P2 + z0 + col] = cadd(a, cmulc(b, tws[i]));
@@ -403,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
     const int P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
@@ -446,7 +453,8 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     if (done && *done) return;
     const int f = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
-    const long row0 = (long)xcd_tile(blockIdx.x, gridDim.x) * B;
+    const long rend = (long)g.x1 * g.ny;  // rows of the x window: [x0 Ny, x1 Ny)
+    const long row0 = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         const int col = e / N, t = 2 * (e % N);
         const long row = row0 + col;
         T v0 = czero<T>(), v1 = czero<T>();
-        if (row < rows) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
+        if (row < rend) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
         const int ep = t >= N, m = t - ep * N;
         buf[m * LD + ep * B + col] = v0;
         const int t1 = t + 1, ep1 = t1 >= N, m1 = t1 - ep1 * N;
@@ -480,7 +488,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {
             const int col = e / (N / 2), k = 2 * (e % (N / 2));
             const long row = row0 + col;
-            if (row < rows) {
+            if (row < rend) {
                 const long idx = f * g.V + row * N + k;
                 term(idx, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
                 term(idx + 1, cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1])));
@@ -490,7 +498,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         for (int e = threadIdx.x; e < B * N; e += NT) {
             const int col = e / N, k = e % N;
             const long row = row0 + col;
-            if (row < rows) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
+            if (row < rend) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
         }
     }
     if (dots.n > 0) {

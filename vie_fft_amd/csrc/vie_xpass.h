@@ -278,7 +278,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1);
-                v[c][b * r1 + q] = (n < N && colok) ? base[c * cst + (unsigned)n * rowst] : czero<T_>();
+                v[c][b * r1 + q] = (n < N && colok && n >= g.x0 && n < g.x1) ? base[c * cst + (unsigned)n * rowst]
+                                                                                : czero<T_>();
             }
         }
     if constexpr (GSL) {
@@ -352,7 +353,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
                 for (int q = 0; q < r1; ++q) {
                     const int n = i + q * (P / r1);
-                    if (n < N) base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
+                    if (n < N && n >= g.x0 && n < g.x1) base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
                 }
             }
     }
@@ -408,8 +409,9 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     const int s = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;  // consecutive lanes: consecutive cells
-    const long row = (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
-    const bool rowok = row < rows;
+    // rows of the x window: [x0 Ny, x1 Ny)
+    const long row = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = row < (long)g.x1 * g.ny;
     T_ c[3];
     load_coef<R>(lin.coef, c);
     const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
@@ -483,8 +485,9 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     const int f = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;
-    const long row = (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
-    const bool rowok = row < rows;
+    // rows of the x window: [x0 Ny, x1 Ny)
+    const long row = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = row < (long)g.x1 * g.ny;
     T_ v[E];
     {
         const T_* in = W1 + (f * rows + (rowok ? row : 0)) * (2 * N);
@@ -581,7 +584,7 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
     const int P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
     const int col = threadIdx.x % B, t = threadIdx.x / B;
     const bool colok = z0 + col < P2;
     const T_* in = W1 + ((long)s * g.nx + i) * N * P2 + z0 + col;
@@ -624,7 +627,7 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
     const int P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i = tile / nzt, z0 = (tile % nzt) * B;
+    const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
     const int col = threadIdx.x % B, t = threadIdx.x / B;
     const bool colok = z0 + col < P2;
     const T_* in = W2 + ((long)s * g.nx + i) * (2 * N) * P2 + z0 + col;
