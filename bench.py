@@ -10,8 +10,9 @@ already resident in HBM.  `value` is the mixed-precision solve rate
 replaced by the true FP64 residual every 10x reduction, FP64 stop rule); the
 plain FP64 solve is reported beside it.  The roofline entry covers the
 dominant kernel of the mixed solve (the FP32 x-axis pass: forward transform,
-spectral Green's-function product, inverse transform), timed per launch with
-HIP events on the library's stream during the timed solves.
+spectral Green's-function product, inverse transform, over the x window of
+the internal axis order on its pencil length), timed per launch with HIP
+events on the library's stream during the timed solves.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -30,23 +31,38 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s to rel. residual 1e-6 @128³ grid, mixed vs FP64; % of HBM roofline"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
-# HBM bytes per launch of the FP32 x pass at 128^3, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE (scaled by a plain copy of known size):
+# HBM bytes per launch of the FP32 x pass of the 128^3 workload (x window of
+# 48 planes on 64-length pencils), measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (scaled by a plain copy of known size):
 # profiles/k_xreg_f32_128cube_hbm_traffic.md
-XPASS_F32_128_TRAFFIC = 437.5e6
+XPASS_F32_128_TRAFFIC = 168.6e6
+XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil": 64, "grid": (128, 128, 128)}
+
+# register-resident x-pass kernels by pencil length (FP32)
+XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false>", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,2,true>",
+            96: "k_xreg<float,96,16,XPlan<4,4,4,3>,0,2,true>", 128: "k_xreg<float,128,16,XPlan<16,16>,0,2,true>",
+            256: "k_xreg<float,256,32,XPlan<16,16,2>,0,0,true>"}
 
 
-def model_bytes(V, oct_, s):
-    """Algorithmic bytes: per product 81 V s + octant s + 4 V; BiCGSTAB
-    iteration = 2 products + 16 fused 3-component vector streams (48 V s)."""
-    mv = 81.0 * V * s + oct_ * s + 4.0 * V
-    return mv, 2.0 * mv + 48.0 * V * s
+def product_bytes(plan, s):
+    """Algorithmic bytes of one Krylov product: 81 entries of s bytes per cell
+    of the x window (input, output and diagonal; the padded W1/W2 arrays each
+    pass reads and writes), the G~ octant of the x pass's pencil length, and
+    4 bytes of flags and row ranges per window cell (vie_model_bytes_matvec)."""
+    _, ny, nz = plan["grid"]
+    vw = (plan["x_window"][1] - plan["x_window"][0]) * ny * nz
+    oct_ = (plan["x_pencil"] + 1) * (ny + 1) * (nz + 1)
+    return 81.0 * vw * s + oct_ * s + 4.0 * vw
 
 
-def xmix_bytes(V, oct_, s):
-    """x spectral pass: reads and writes 3 components x 4V (in place) and the
-    Green's-function octant once."""
-    return 24.0 * V * s + oct_ * s
+def xpass_bytes(plan, s):
+    """x pass per launch: reads and writes the 3 components of the x window's
+    planes of the padded (2Ny x 2Nz) spectrum in place, and reads the G~ octant
+    of its pencil length once."""
+    _, ny, nz = plan["grid"]
+    w = plan["x_window"][1] - plan["x_window"][0]
+    oct_ = (plan["x_pencil"] + 1) * (ny + 1) * (nz + 1)
+    return 2.0 * 3 * (2 * ny) * (2 * nz) * w * s + oct_ * s
 
 
 def main():
@@ -77,8 +93,6 @@ def main():
     sc = G.metric_case() if args.n == 128 else G.layered_case(n=(args.n,) * 3, band_start=args.n // 2 - 24,
                                                                sphere_z=args.n // 2 + 4)
     n = sc.n
-    V = int(np.prod(n))
-    oct_ = (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
     flags = sc.flags()
     s = VF.Solver(n, sc.d, sc.k0, VF.FP64)
     s.set_medium(sc.material(), sc.eps_table, flags)
@@ -115,12 +129,12 @@ def main():
     it64 = infos64[-1]["iterations"]
     itm = infosm[-1]["iterations"]
     outer = infosm[-1]["outer"]
-    mv64, iter64 = model_bytes(V, oct_, 16.0)
-    mv32, iter32 = model_bytes(V, oct_, 8.0)
-    solve_bytes_64 = it64 * iter64
-    solve_bytes_m = itm * iter32 + outer * (mv64 + 12 * V * 16.0)
-    xb = xmix_bytes(V, oct_, 8.0)
+    plan = s.plan()
+    xb = xpass_bytes(plan, 8.0)
     achieved = xb / (ms13 * 1e-3) if ms13 > 0 else 0.0
+    nxp = plan["x_pencil"]
+    kernel = XREG_F32.get(nxp, f"k_xmix<float,{nxp},B>")
+    w = plan["x_window"][1] - plan["x_window"][0]
 
     out = {
         "metric": METRIC,
@@ -141,19 +155,21 @@ def main():
             "grid": list(n),
         },
         "mixed": {"solves_per_s": 1.0 / tm, "ms_per_solve": tm * 1e3, "iterations": itm,
-                  "residual_replacements": outer,
-                  "true_relres": infosm[-1]["true_relres"],
-                  "model_hbm_frac": solve_bytes_m / tm / HBM_PEAK},
+                  "us_per_iteration": tm * 1e6 / itm, "residual_replacements": outer,
+                  "true_relres": infosm[-1]["true_relres"]},
         "fp64": {"solves_per_s": 1.0 / t64, "ms_per_solve": t64 * 1e3, "iterations": it64,
-                 "true_relres": infos64[-1]["true_relres"],
-                 "model_hbm_frac": solve_bytes_64 / t64 / HBM_PEAK,
-                 "xmix_ms": ms3_64, "xmix_frac": xmix_bytes(V, oct_, 16.0) / (ms3_64 * 1e-3) / HBM_PEAK
+                 "us_per_iteration": t64 * 1e6 / it64, "true_relres": infos64[-1]["true_relres"],
+                 "xpass_ms": ms3_64, "xpass_frac": xpass_bytes(plan, 16.0) / (ms3_64 * 1e-3) / HBM_PEAK
                  if ms3_64 > 0 else None},
         "speedup_mixed_vs_fp64": t64 / tm,
+        "plan": {"axis_order": list(plan["axis_order"]), "x_window": list(plan["x_window"]),
+                 "x_pencil": nxp, "grid": list(plan["grid"]),
+                 "product_bytes_f32": product_bytes(plan, 8.0), "product_bytes_f64": product_bytes(plan, 16.0)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
-                     "traffic": XPASS_F32_128_TRAFFIC if n == (128, 128, 128) else None,
-                     "kernel": "k_xreg<float,128,16,XPlan<16,16>,0,2,true> (x-axis forward + spectral product + inverse)",
+                     "traffic": XPASS_F32_128_TRAFFIC if plan == XPASS_F32_128_PLAN else None,
+                     "kernel": f"{kernel} (x pass: forward transform, spectral product, inverse, "
+                               f"{w}-plane window on {nxp}-length pencils)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }
     # measured practical HBM rate (device copy) beside the published peak

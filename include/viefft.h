@@ -219,8 +219,15 @@ int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double*
    convergence, which exit at once, are left out). */
 int vie_time_kernel(vie_ctx* ctx, int kernel_class);
 int vie_kernel_time(vie_ctx* ctx, int* launches, double* mean_ms);
-/* bytes the algorithmic model charges one operator application */
+/* bytes the algorithmic model charges one operator application inside a
+   solve (the passes over the x window, the x pass on its pencil length) */
 double vie_model_bytes_matvec(const vie_ctx* ctx);
+/* Internal plan of the resident medium (no counterpart in the reference; for
+   profiling and tests): out[0..2] internal axis order (internal axis a is the
+   caller's axis out[a]); out[3], out[4] the x window [x0, x1) of the products
+   inside solves, on the internal grid; out[5] the pencil length of their x
+   pass (the short period, or the internal Nx); out[6..8] the internal grid. */
+int vie_plan(const vie_ctx* ctx, int* out);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,8 @@ oracle's, which works on the caller's grid with every plane:
 * 8 fixed iterations of each of the five methods (iterates to 1e-9);
 * converged solves, the incident right-hand side built on the device, the
   far-field, absorbed-power and gradient outputs, and the mixed solve;
+* the x pass on a shorter period than the grid's (register-resident and
+  LDS-staged pencils);
 * an embedded grid (lengths without compiled kernels) with a slab;
 * the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
 """
@@ -60,6 +62,23 @@ def solver(n, mat, flags, precision=VF.FP64):
 
 def oracle(n, eps, flags):
     return O.Operator(n, D, K0, eps, flags, O.SCHED_FUSED)
+
+
+@pytest.mark.parametrize("axis,lo,hi,order,pencil", [(0, 5, 11, (0, 1, 2), 6), (1, 3, 8, (1, 0, 2), 5),
+                                                     (2, 6, 15, (2, 0, 1), 10)])
+def test_plan_puts_the_slab_axis_first(axis, lo, hi, order, pencil):
+    mat, eps, flags = slab_scene(axis, lo, hi, 1)
+    with solver(N, mat, flags) as s:
+        p = s.plan()
+    assert p["axis_order"] == order and p["grid"] == tuple(N[a] for a in order)
+    assert p["x_window"] == (lo, hi) and p["x_pencil"] == pencil
+    os.environ["VIE_WINDOW"] = "0"
+    try:
+        with solver(N, mat, flags) as s:
+            p = s.plan()
+    finally:
+        os.environ.pop("VIE_WINDOW", None)
+    assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N}
 
 
 @pytest.mark.parametrize("axis,lo,hi", SLABS)
@@ -144,6 +163,22 @@ def test_slab_solve_and_outputs_match_oracle(axis, lo, hi):
     Po, gro, gio, _ = O.absorption_grad(op, xs, tol=1e-13)
     assert gre.shape == N
     assert rel(gre, gro) <= 1e-9 and rel(gim, gio) <= 1e-9
+
+
+@pytest.mark.parametrize("n,lo,hi", [((64, 8, 10), 20, 40), ((128, 6, 8), 30, 70), ((96, 8, 6), 10, 30)])
+def test_short_period_register_pass_matches_oracle(n, lo, hi):
+    """Windows of 20 and 40 planes in x extents 64, 96 and 128: the x pass runs
+    on the register-resident 32- or 64-length pencils (window shifted to plane 0)."""
+    mat, eps, flags = slab_scene(0, lo, hi, 11, n)
+    b = O.incident(n, D, K0, 20.0, 30.0, 40.0) * flags
+    ref = oracle(n, eps, flags).solve(b, 3, tol=1e-300, maxit=8)
+    with solver(n, mat, flags) as s:
+        assert s.plan()["x_pencil"] == (32 if hi - lo <= 32 else 64)
+        x, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+    assert rel(x, ref["x"]) <= 1e-9, rel(x, ref["x"])
+    with solver(n, mat, flags, VF.FP32) as s:
+        x32, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+    assert rel(x32, ref["x"]) <= 1e-4, rel(x32, ref["x"])
 
 
 def test_embedded_slab_matches_oracle():

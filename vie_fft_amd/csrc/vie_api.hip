@@ -132,23 +132,87 @@ template <class R> struct Op {
     Geo g{};
     int n[3];
     double d[3], k0;
-    T* tw[3] = {};
-    T* twist[3] = {};
-    T* twP[3] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
-    T* U[3] = {};
-    T* Vv[3] = {};
-    T* Ua[3] = {};
-    T* Va[3] = {};
-    int* fold[3] = {};
+    // per-axis tables, slots 0-2 the grid's axes; slot 3 the x axis of the
+    // short-period x pass (length nxs)
+    T* tw[4] = {};
+    T* twist[4] = {};
+    T* twP[4] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
+    T* U[4] = {};
+    T* Vv[4] = {};
+    T* Ua[4] = {};
+    T* Va[4] = {};
+    int* fold[4] = {};
     T* G = nullptr;
     T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
     uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
     int wx0 = 0, wx1 = 0;      // x window: the x planes with an active face
+    // Short-period x pass: a product whose input and output live on the x
+    // window [wx0, wx1) is a linear convolution along x of width
+    // w = wx1 - wx0, which a padded transform of any length 2 nxs with
+    // nxs >= w reproduces (the window shifted to plane 0).  nxs = 0: the
+    // x pass runs at the grid's length.  Gs is the G~ octant of the
+    // (nxs, Ny, Nz) grid scaled by Nx / nxs (the tables keep the grid's
+    // normalisation 1/(8 Nx Ny Nz)).
+    int nxs = 0;
+    T* Gs = nullptr;
     T* W1 = nullptr;
     T* W2 = nullptr;
     bool own_w = true;
-    MixTabs<R> mf{}, ma{};
+    MixTabs<R> mf{}, ma{}, mfs{}, mas{};
+
+    // Length-N tables of slot a: LDS-pass twiddles, the register passes'
+    // twP, U, the adjoint's V and the G~ fold (the k0-dependent Vv and Ua
+    // come from axis_k0).
+    void build_axis(int a, int N, hipStream_t st)
+    {
+        std::vector<T> h_tw(N), h_twist(N), h_u(2 * N), h_va(2 * N), h_p(2 * N);
+        std::vector<int> h_fold(2 * N);
+        for (int t = 0; t < N; ++t) {
+            const long double a1 = -2.0L * (long double)kPi * t / N;
+            const long double a2 = -2.0L * (long double)kPi * t / (2 * N);
+            h_tw[t] = hc<R>(std::complex<double>((double)cosl(a1), (double)sinl(a1)));
+            h_twist[t] = hc<R>(std::complex<double>((double)cosl(a2), (double)sinl(a2)));
+        }
+        for (int t = 0; t < 2 * N; ++t) {
+            const int k = 2 * (t % N) + t / N;
+            const long double th = (long double)kPi * k / N;
+            const double s = (double)sinl(th), h = (double)sinl(th / 2);
+            // e^{j th} - 1 = -2 sin^2(th/2) + j sin th
+            const std::complex<double> u(-2.0 * h * h, s);
+            h_u[t] = hc<R>(u);
+            h_va[t] = hc<R>(std::conj(u));
+            const int kf = std::min(k, 2 * N - k);
+            h_fold[t] = (kf % 2 == 0) ? kf / 2 : (N / 2 + 1) + (kf - 1) / 2;
+            const long double ang = -2.0L * (long double)kPi * t / (2 * N);
+            h_p[t] = hc<R>(std::complex<double>((double)cosl(ang), (double)sinl(ang)));
+        }
+        tw[a] = dalloc<T>(N);
+        twist[a] = dalloc<T>(N);
+        twP[a] = dalloc<T>(2 * N);
+        U[a] = dalloc<T>(2 * N);
+        Vv[a] = dalloc<T>(2 * N);
+        Ua[a] = dalloc<T>(2 * N);
+        Va[a] = dalloc<T>(2 * N);
+        fold[a] = dalloc<int>(2 * N);
+        copy_sync(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(twP[a], h_p.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice, st);
+    }
+    void free_axis(int a)
+    {
+        dfree(tw[a]);
+        dfree(twist[a]);
+        dfree(twP[a]);
+        dfree(U[a]);
+        dfree(Vv[a]);
+        dfree(Ua[a]);
+        dfree(Va[a]);
+        dfree(fold[a]);
+    }
 
     Op(const int* nn, const double* dd, double kk, T* w1, T* w2, hipStream_t st)
     {
@@ -163,51 +227,7 @@ template <class R> struct Op {
         g.V = (long)n[0] * n[1] * n[2];
         g.x0 = wx0 = 0;
         g.x1 = wx1 = n[0];
-        const double NP = 8.0 * (double)g.V;
-        for (int a = 0; a < 3; ++a) {
-            const int N = n[a];
-            std::vector<T> h_tw(N), h_twist(N), h_u(2 * N), h_va(2 * N);
-            std::vector<int> h_fold(2 * N);
-            for (int t = 0; t < N; ++t) {
-                const long double a1 = -2.0L * (long double)kPi * t / N;
-                const long double a2 = -2.0L * (long double)kPi * t / (2 * N);
-                h_tw[t] = hc<R>(std::complex<double>((double)cosl(a1), (double)sinl(a1)));
-                h_twist[t] = hc<R>(std::complex<double>((double)cosl(a2), (double)sinl(a2)));
-            }
-            for (int t = 0; t < 2 * N; ++t) {
-                const int k = 2 * (t % N) + t / N;
-                const long double th = (long double)kPi * k / N;
-                const double s = (double)sinl(th), h = (double)sinl(th / 2);
-                // e^{j th} - 1 = -2 sin^2(th/2) + j sin th
-                const std::complex<double> u(-2.0 * h * h, s);
-                h_u[t] = hc<R>(u);
-                h_va[t] = hc<R>(std::conj(u));
-                const int kf = std::min(k, 2 * N - k);
-                h_fold[t] = (kf % 2 == 0) ? kf / 2 : (N / 2 + 1) + (kf - 1) / 2;
-            }
-            tw[a] = dalloc<T>(N);
-            twist[a] = dalloc<T>(N);
-            U[a] = dalloc<T>(2 * N);
-            Vv[a] = dalloc<T>(2 * N);
-            Ua[a] = dalloc<T>(2 * N);
-            Va[a] = dalloc<T>(2 * N);
-            fold[a] = dalloc<int>(2 * N);
-            copy_sync(tw[a], h_tw.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(twist[a], h_twist.data(), N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(U[a], h_u.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(Va[a], h_va.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(fold[a], h_fold.data(), 2 * N * sizeof(int), hipMemcpyHostToDevice, st);
-        }
-        for (int a = 0; a < 3; ++a) {
-            const int P = 2 * n[a];
-            std::vector<T> h(P);
-            for (int e = 0; e < P; ++e) {
-                const long double ang = -2.0L * (long double)kPi * e / P;
-                h[e] = hc<R>(std::complex<double>((double)cosl(ang), (double)sinl(ang)));
-            }
-            twP[a] = dalloc<T>(P);
-            copy_sync(twP[a], h.data(), P * sizeof(T), hipMemcpyHostToDevice, st);
-        }
+        for (int a = 0; a < 3; ++a) build_axis(a, n[a], st);
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
         flags = dalloc<uint8_t>(3 * g.V);
@@ -234,48 +254,78 @@ template <class R> struct Op {
         set_k0(kk, st);
     }
 
-    // The wavenumber enters the tables through v_s = (j Z0/(k0 delta_s NP))
-    // (1 - e^{-j theta_s}) (its conjugate is the adjoint's u) and D_f =
-    // j k0 Z0 delta_f / NP (vie_fft/fft.h:76-100, 256-281).
+    // Short-period x pass of length ns (0: none): slot-3 tables, Gs (filled
+    // by the caller from green_octant) and the MixTabs that use them.
+    void set_short(int ns, hipStream_t st)
+    {
+        if (ns == nxs) return;
+        free_axis(3);
+        dfree(Gs);
+        nxs = ns;
+        if (!ns) return;
+        build_axis(3, ns, st);
+        Gs = dalloc<T>((long)(ns + 1) * (n[1] + 1) * (n[2] + 1));
+        axis_k0(3, ns, d[0], st);
+        short_tabs();
+    }
+    void short_tabs()
+    {
+        mfs = mf;
+        mas = ma;
+        mfs.U[0] = U[3];
+        mfs.Vv[0] = Vv[3];
+        mfs.fold[0] = fold[3];
+        mas.U[0] = Ua[3];
+        mas.Vv[0] = Va[3];
+        mas.fold[0] = fold[3];
+        mfs.G = mas.G = Gs;
+    }
+
+    // v_s = (j Z0/(k0 delta_s NP)) (1 - e^{-j theta_s}) of slot a (its
+    // conjugate is the adjoint's u), NP = 8 Nx Ny Nz of the grid.
+    void axis_k0(int a, int N, double da, hipStream_t st)
+    {
+        const double NP = 8.0 * (double)g.V;
+        const std::complex<double> J(0, 1);
+        std::vector<T> h_v(2 * N), h_ua(2 * N);
+        for (int t = 0; t < 2 * N; ++t) {
+            const int k = 2 * (t % N) + t / N;
+            const long double th = (long double)kPi * k / N;
+            const double s = (double)sinl(th), h = (double)sinl(th / 2);
+            const std::complex<double> w(2.0 * h * h, s);  // 1 - e^{-j th}
+            const std::complex<double> vv = J * kZ0 / (k0 * da * NP) * w;
+            h_v[t] = hc<R>(vv);
+            h_ua[t] = hc<R>(std::conj(vv));
+        }
+        copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+    }
+
+    // The wavenumber enters the tables through v_s and D_f = j k0 Z0
+    // delta_f / NP (vie_fft/fft.h:76-100, 256-281).
     void set_k0(double kk, hipStream_t st)
     {
         k0 = kk;
         const double NP = 8.0 * (double)g.V;
         const std::complex<double> J(0, 1);
         for (int a = 0; a < 3; ++a) {
-            const int N = n[a];
-            std::vector<T> h_v(2 * N), h_ua(2 * N);
-            for (int t = 0; t < 2 * N; ++t) {
-                const int k = 2 * (t % N) + t / N;
-                const long double th = (long double)kPi * k / N;
-                const double s = (double)sinl(th), h = (double)sinl(th / 2);
-                const std::complex<double> w(2.0 * h * h, s);  // 1 - e^{-j th}
-                const std::complex<double> vv = J * kZ0 / (k0 * d[a] * NP) * w;
-                h_v[t] = hc<R>(vv);
-                h_ua[t] = hc<R>(std::conj(vv));
-            }
-            copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+            axis_k0(a, n[a], d[a], st);
             const std::complex<double> D = J * k0 * kZ0 * d[a] / NP;
             mf.D[a][0] = (R)D.real();
             mf.D[a][1] = (R)D.imag();
             ma.D[a][0] = (R)D.real();
             ma.D[a][1] = (R)-D.imag();
         }
+        if (nxs) {
+            axis_k0(3, nxs, d[0], st);
+            short_tabs();
+        }
     }
     ~Op()
     {
-        for (int a = 0; a < 3; ++a) {
-            dfree(tw[a]);
-            dfree(twist[a]);
-            dfree(U[a]);
-            dfree(Vv[a]);
-            dfree(Ua[a]);
-            dfree(Va[a]);
-            dfree(fold[a]);
-        }
-        for (int a = 0; a < 3; ++a) dfree(twP[a]);
+        for (int a = 0; a < 4; ++a) free_axis(a);
         dfree(G);
+        dfree(Gs);
         dfree(dg);
         dfree(flags);
         dfree(zr);
@@ -316,8 +366,16 @@ template <class R> struct Op {
         t0(2);
         HIPCHK(run_yfwd<R>(gw, W1, W2, tw[1], twist[1], twP[1], done, st));
         t0(2);
+        Geo gx = gw;
+        int ax = 0, nxp = g.nx;
+        if (lin.zr && nxs) {  // the window's pencils on the short period
+            gx.xs = wx0;
+            ax = 3;
+            nxp = nxs;
+        }
+        const MixTabs<R>& mt = ax ? (adj ? mas : mfs) : (adj ? ma : mf);
         t0(3);
-        HIPCHK(run_xmix<R>(gw, W2, tw[0], twist[0], twP[0], adj ? ma : mf, done, st));
+        HIPCHK(run_xmix<R>(gx, nxp, W2, tw[ax], twist[ax], twP[ax], mt, done, st));
         t0(3);
         t0(4);
         HIPCHK(run_yinv<R>(gw, W2, W1, tw[1], twist[1], twP[1], done, st));
@@ -379,6 +437,7 @@ struct vie_ctx {
     Op<double>* op64 = nullptr;
     Op<float>* op32 = nullptr;
     double2* Goff = nullptr;       // FP64 octant of G~ (parity order)
+    double2* Gshort = nullptr;     // FP64 octant of the short-period x pass (sized as Goff)
     double2* gwork = nullptr;      // second octant buffer of green_compute
     double* gdct = nullptr;        // cosine tables of green_compute
     bool green_ready = false;
@@ -524,32 +583,44 @@ template <class R> Dots<R> dots_with(vie_ctx* c, int n, std::initializer_list<in
 
 template <class R> Op<R>* make_op(vie_ctx* c);
 
-void green_compute(vie_ctx* c)
+void short_sync(vie_ctx* c);
+
+// G~ octant of an (n[0], n[1], n[2]) grid with the context's cell sizes and
+// k0, in parity order, into out ((n0+1)(n1+1)(n2+1) FP64 values); gwork is
+// the other buffer of the ping-pong.
+void green_octant(vie_ctx* c, const int* n, double2* out)
 {
-    const int* n = c->desc.n;
     const double* d = c->desc.d;
     const long L0 = n[0] + 1, L1 = n[1] + 1, L2 = n[2] + 1, tot = L0 * L1 * L2;
-    // ping-pong between gwork and Goff; both kept for later calls
-    if (!c->Goff) c->Goff = dalloc<double2>(tot);
-    if (!c->gwork) c->gwork = dalloc<double2>(tot);
-    const int nmax = std::max(n[0], std::max(n[1], n[2]));
+    const int* nf = c->desc.n;
+    const long full = (long)(nf[0] + 1) * (nf[1] + 1) * (nf[2] + 1);
+    if (!c->gwork) c->gwork = dalloc<double2>(full);
+    const int nmax = std::max(nf[0], std::max(nf[1], nf[2]));
     if (!c->gdct) c->gdct = dalloc<double>(3 * 2 * nmax);
     std::vector<double> ctab(3 * 2 * nmax);
     for (int ax = 0; ax < 3; ++ax)
         for (int j = 0; j < 2 * n[ax]; ++j) ctab[ax * 2 * nmax + j] = (double)cosl((long double)kPi * j / n[ax]);
     copy_sync(c->gdct, ctab.data(), ctab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
     double2* src = c->gwork;
-    double2* dst = c->Goff;
+    double2* dst = out;
     hipLaunchKernelGGL(k_green_offsets, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, n[0], n[1], n[2],
                        d[0], d[1], d[2], c->desc.k0, src);
     HIPCHK(hipGetLastError());
+    // x: gwork -> out, y: out -> gwork, z: gwork -> out
     for (int ax = 0; ax < 3; ++ax) {
         hipLaunchKernelGGL(k_green_cos, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, (int)L0, (int)L1,
                            (int)L2, ax, c->gdct + ax * 2 * nmax, src, dst);
         HIPCHK(hipGetLastError());
         std::swap(src, dst);
     }
-    // offsets in gwork; x: gwork -> Goff, y: Goff -> gwork, z: gwork -> Goff
+}
+
+void green_compute(vie_ctx* c)
+{
+    const int* n = c->desc.n;
+    const long tot = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    if (!c->Goff) c->Goff = dalloc<double2>(tot);
+    green_octant(c, n, c->Goff);
     c->green_ready = true;
     if (c->op64) HIPCHK(hipMemcpyAsync(c->op64->G, c->Goff, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
     if (c->op32) {
@@ -558,6 +629,7 @@ void green_compute(vie_ctx* c)
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    short_sync(c);
 }
 
 template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
@@ -594,6 +666,57 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
     HIPCHK(hipGetLastError());
 }
 
+// Length of the short-period x pass for a window of w planes: the
+// shortest length >= w below Nx with a register-resident x pass in both
+// precisions, else the shortest compiled one below Nx, else none (also with
+// windows off).  At 128^3 with w = 48 (MI355X, per launch): 64 register
+// 60 us FP32 / 105 us FP64, 48 LDS-staged 83 / 153 us, 48 register (4-stage
+// plan) 67 / 155 us, the full 128 84 / 196 us.  VIE_XSHORT=0 turns it off,
+// VIE_XSHORT=m asks for length m.
+int short_len(const vie_ctx* c, int w)
+{
+    const int nx = c->desc.n[0];
+    if (!c->windows || w >= nx) return 0;
+    if (const char* e = std::getenv("VIE_XSHORT")) {
+        const int m = std::atoi(e);
+        return (m >= w && m < nx && size_supported(m)) ? m : 0;
+    }
+    for (int m = w; m < nx; ++m)
+        if (size_supported(m) && x_register<double>(m) && x_register<float>(m)) return m;
+    for (int m = w; m < nx; ++m)
+        if (size_supported(m)) return m;
+    return 0;
+}
+
+// Short-period x pass of every operator for the current window, with its
+// G~ octant (scaled by Nx / nxs) once G~ is computed.
+void short_sync(vie_ctx* c)
+{
+    Op<double>* o64 = c->op64;
+    Op<float>* o32 = c->op32;
+    if (!o64 && !o32) return;
+    const int w = o64 ? o64->wx1 - o64->wx0 : o32->wx1 - o32->wx0;
+    const int ns = c->medium_ready ? short_len(c, w) : 0;
+    if (o64) o64->set_short(ns, c->stream);
+    if (o32) o32->set_short(ns, c->stream);
+    if (!ns || !c->green_ready) return;
+    const int* n = c->desc.n;
+    const int n3[3] = {ns, n[1], n[2]};
+    const long tot = (long)(ns + 1) * (n[1] + 1) * (n[2] + 1);
+    if (!c->Gshort) c->Gshort = dalloc<double2>((long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1));
+    green_octant(c, n3, c->Gshort);
+    hipLaunchKernelGGL(k_scale_c64, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                       (double)n[0] / ns, c->Gshort);
+    HIPCHK(hipGetLastError());
+    if (o64) HIPCHK(hipMemcpyAsync(o64->Gs, c->Gshort, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    if (o32) {
+        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                           c->Gshort, o32->Gs);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 template <> Op<double>* make_op<double>(vie_ctx* c)
 {
     if (!c->op64) {
@@ -607,7 +730,10 @@ template <> Op<double>* make_op<double>(vie_ctx* c)
             copy_sync(c->op64->G, c->Goff,
                       (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1) * sizeof(double2),
                       hipMemcpyDeviceToDevice, c->stream);
-        if (c->medium_ready) upload_medium<double>(c, c->op64);
+        if (c->medium_ready) {
+            upload_medium<double>(c, c->op64);
+            short_sync(c);
+        }
     }
     return c->op64;
 }
@@ -626,7 +752,10 @@ template <> Op<float>* make_op<float>(vie_ctx* c)
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(c->stream));
         }
-        if (c->medium_ready) upload_medium<float>(c, c->op32);
+        if (c->medium_ready) {
+            upload_medium<float>(c, c->op32);
+            short_sync(c);
+        }
     }
     return c->op32;
 }
@@ -1380,6 +1509,7 @@ int vie_destroy(vie_ctx* c)
     delete c->op32;   // shares buffers owned by op64: delete first
     delete c->op64;
     dfree(c->Goff);
+    dfree(c->Gshort);
     dfree(c->gwork);
     dfree(c->gdct);
     dfree(c->eps_dev);
@@ -1428,6 +1558,7 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
     c->medium_ready = true;
     if (c->op64) upload_medium<double>(c, c->op64);
     if (c->op32) upload_medium<float>(c, c->op32);
+    short_sync(c);
     return VIE_OK;
 }
 
@@ -2247,13 +2378,43 @@ int vie_kernel_time(vie_ctx* c, int* launches, double* mean_ms)
     return VIE_OK;
 }
 
+// Krylov-product model: 81 entries of the precision per cell of the x window
+// (input and output vectors, diagonal, the padded W1/W2 arrays each pass
+// reads and writes), the G~ octant of the x pass's pencil length and 4 bytes
+// of flags and row ranges per window cell.
 double vie_model_bytes_matvec(const vie_ctx* c)
 {
     if (!c) return 0.0;
+    int p[9];
+    if (vie_plan(c, p) != VIE_OK) return 0.0;
     const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
-    const double V = (double)c->V;
-    const double oct = (double)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1);
-    return 81.0 * V * s + oct * s + 4.0 * V;
+    const double Vw = (double)(p[4] - p[3]) * p[7] * p[8];
+    const double oct = (double)(p[5] + 1) * (p[7] + 1) * (p[8] + 1);
+    return 81.0 * Vw * s + oct * s + 4.0 * Vw;
+}
+
+int vie_plan(const vie_ctx* c, int* out)
+{
+    if (!c || !out) return VIE_ERR_ARG;
+    const int* n = c->desc.n;
+    int x0 = 0, x1 = n[0], ns = 0;
+    if (c->op64) {
+        x0 = c->op64->wx0;
+        x1 = c->op64->wx1;
+        ns = c->op64->nxs;
+    } else if (c->op32) {
+        x0 = c->op32->wx0;
+        x1 = c->op32->wx1;
+        ns = c->op32->nxs;
+    }
+    for (int a = 0; a < 3; ++a) {
+        out[a] = c->perm[a];
+        out[6 + a] = n[a];
+    }
+    out[3] = x0;
+    out[4] = x1;
+    out[5] = ns ? ns : n[0];
+    return VIE_OK;
 }
 
 }  // extern "C"

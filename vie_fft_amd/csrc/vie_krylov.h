@@ -184,6 +184,15 @@ __global__ void k_fill_zero(long n, cx<R>* x)
         x[t] = czero<cx<R>>();
 }
 
+// x *= s over n complex FP64 values.
+__global__ void k_scale_c64(long n, double s, double2* x)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        x[t].x *= s;
+        x[t].y *= s;
+    }
+}
+
 // Zero a 3-component field (V entries per component) outside the entries
 // [lo, hi) of each component: the x planes outside a product's x window.
 template <class R>

@@ -19,10 +19,13 @@ hipError_t run_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw,
 template <class R>
 hipError_t run_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx<R>* tw, const cx<R>* twist,
                     const cx<R>* twP, const int* done, hipStream_t st);
-// twP: exp(-2 pi j e / (2 Nx)), e < 2 Nx (register x pass)
+// Pencils of length nxp (Nx, or the short period with g.xs = g.x0); twP:
+// exp(-2 pi j e / (2 nxp)), e < 2 nxp (register x pass)
 template <class R>
-hipError_t run_xmix(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const cx<R>* twP,
+hipError_t run_xmix(const Geo& g, int nxp, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const cx<R>* twP,
                     const MixTabs<R>& mt, const int* done, hipStream_t st);
+// whether length n has a register-resident x pass (else the LDS-staged one)
+template <class R> bool x_register(int n);
 template <class R>
 hipError_t run_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                     const cx<R>* twP, const int* done, hipStream_t st);

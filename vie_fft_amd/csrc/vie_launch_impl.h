@@ -247,11 +247,24 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
     return hipGetLastError();
 }
 
+template <> bool x_register<R_>(int n)
+{
+    if (xpass_lds_forced()) return false;
+    switch (n) {
+#define CASE(m) \
+    case m: return XReg<R_, m>::ok;
+        VIE_SIZES(CASE)
+#undef CASE
+    default:
+        return false;
+    }
+}
+
 template <>
-hipError_t run_xmix<R_>(const Geo& g, cx<R_>* W2, const cx<R_>* tw, const cx<R_>* twist, const cx<R_>* twP,
+hipError_t run_xmix<R_>(const Geo& g, int nxp, cx<R_>* W2, const cx<R_>* tw, const cx<R_>* twist, const cx<R_>* twP,
                         const MixTabs<R_>& mt, const int* done, hipStream_t st)
 {
-    switch (g.nx) {
+    switch (nxp) {
 #define CASE(n) \
     case n: return launch_x<R_, n>(g, W2, tw, twist, twP, mt, done, st);
         VIE_SIZES(CASE)

@@ -17,11 +17,13 @@ namespace vie {
 // [x0, x1) is the x window the passes cover (0 <= x0 < x1 <= nx, set by the
 // owner of the Geo): the x planes outside it carry no active face, so a
 // product whose input vanishes there needs neither its z/y transforms on
-// those planes nor its x-pencil entries there.
+// those planes nor its x-pencil entries there.  The x pass's pencil entry n
+// is plane n + xs (xs = x0 when the pencils have the short period).
 struct Geo {
     int nx, ny, nz;
     long V;
     int x0 = 0, x1 = 0;
+    int xs = 0;
 };
 
 // Tables of the spectral product, per axis a in split order t = e*N_a + m
@@ -353,9 +355,9 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     // x-pencils of the three components: rows i < Nx, columns z0..z0+B
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
         const int s = e / (N * B), r = e % (N * B);
-        const int i = r / B, col = r % B;
-        const bool in = z0 + col < P2 && i >= g.x0 && i < g.x1;
-        T v = in ? W2[(((long)s * g.nx + i) * P1 + ty) * P2 + z0 + col] : czero<T>();
+        const int i = r / B, col = r % B, pl = i + g.xs;
+        const bool in = z0 + col < P2 && pl >= g.x0 && pl < g.x1;
+        T v = in ? W2[(((long)s * g.nx + pl) * P1 + ty) * P2 + z0 + col] : czero<T>();
         buf[i * LD + (2 * s) * B + col] = v;
         buf[i * LD + (2 * s + 1) * B + col] = cmul(v, tws[i]);
     }
@@ -382,11 +384,11 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
         const int s = e / (N * B), r = e % (N * B);
-        const int i = r / B, col = r % B;
-        if (z0 + col < P2 && i >= g.x0 && i < g.x1) {
+        const int i = r / B, col = r % B, pl = i + g.xs;
+        if (z0 + col < P2 && pl >= g.x0 && pl < g.x1) {
             const T a = buf[i * LD + (2 * s) * B + col];
             const T b = buf[i * LD + (2 * s + 1) * B + col];
-            W2[(((long)s * g.nx + i) * P1 + ty) * P2 + z0 + col] = cadd(a, cmulc(b, tws[i]));
+            W2[(((long)s * g.nx + pl) * P1 + ty) * P2 + z0 + col] = cadd(a, cmulc(b, tws[i]));
         }
     }
 }

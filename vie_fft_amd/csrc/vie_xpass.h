@@ -265,7 +265,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     // element offsets fit 32 bits (the host checks 3 x 4V < 2^31 before
     // launching), so the per-access address arithmetic stays 32-bit
     const unsigned rowst = (unsigned)P1 * P2;  // stride between x rows
-    T_* base = W2 + (long)ty * P2 + z0 + col;
+    T_* base = W2 + ((long)g.xs * P1 + ty) * P2 + z0 + col;  // pencil entry 0: plane xs
     const unsigned cst = (unsigned)g.nx * rowst;  // stride between components
 
     // issue the pencil loads first, then the tables, so their latencies overlap
@@ -278,8 +278,9 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1);
-                v[c][b * r1 + q] = (n < N && colok && n >= g.x0 && n < g.x1) ? base[c * cst + (unsigned)n * rowst]
-                                                                                : czero<T_>();
+                v[c][b * r1 + q] = (n < N && colok && n + g.xs >= g.x0 && n + g.xs < g.x1)
+                                       ? base[c * cst + (unsigned)n * rowst]
+                                       : czero<T_>();
             }
         }
     if constexpr (GSL) {
@@ -353,7 +354,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
                 for (int q = 0; q < r1; ++q) {
                     const int n = i + q * (P / r1);
-                    if (n < N && n >= g.x0 && n < g.x1) base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
+                    if (n < N && n + g.xs >= g.x0 && n + g.xs < g.x1)
+                        base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
                 }
             }
     }

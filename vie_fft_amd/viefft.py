@@ -31,7 +31,7 @@ EXPORTS = [
     "vie_set_medium_eps", "vie_green_compute", "vie_incident", "vie_apply", "vie_apply_async",
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
-    "vie_model_bytes_matvec", "vie_absorption", "vie_absorption_grad",
+    "vie_model_bytes_matvec", "vie_plan", "vie_absorption", "vie_absorption_grad",
     "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
 ]
 
@@ -99,6 +99,7 @@ def load():
     L.vie_kernel_time.argtypes = [vp, ct.POINTER(ct.c_int), dp]
     L.vie_model_bytes_matvec.argtypes = [vp]
     L.vie_model_bytes_matvec.restype = ct.c_double
+    L.vie_plan.argtypes = [vp, ct.POINTER(ct.c_int)]
     L.vie_absorption.argtypes = [vp, dp]
     L.vie_set_k0.argtypes = [vp, ct.c_double]
     L.vie_set_rhs_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
@@ -317,6 +318,13 @@ class Solver:
 
     def model_bytes_matvec(self):
         return self.lib.vie_model_bytes_matvec(self.h)
+
+    def plan(self):
+        """Internal plan of the resident medium: axis order, x window of the
+        products inside solves, pencil length of their x pass, internal grid."""
+        a = (ct.c_int * 9)()
+        self._check(self.lib.vie_plan(self.h, a))
+        return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9])}
 
 
 def supported_n(n):
