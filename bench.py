@@ -39,7 +39,7 @@ XPASS_F32_128_TRAFFIC = 168.6e6
 XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil": 64, "grid": (128, 128, 128)}
 
 # register-resident x-pass kernels by pencil length (FP32)
-XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false>", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,2,true>",
+XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false>", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,0,false>",
             96: "k_xreg<float,96,16,XPlan<4,4,4,3>,0,2,true>", 128: "k_xreg<float,128,16,XPlan<16,16>,0,2,true>",
             256: "k_xreg<float,256,32,XPlan<16,16,2>,0,0,true>"}
 

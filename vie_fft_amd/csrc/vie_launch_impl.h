@@ -192,9 +192,13 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // <8,8,4> with the prefetch 229 us against 276 us with an LDS slice; OP
 // leaves it at two waves (169 registers) and costs 262 us, with G~ read in
 // the product (148 registers) 264 us.  OP is on where it adds a wave per SIMD
-// (FP32 64: 130 -> 124 registers, 96: 211 -> 159, 128); FP32 256 also reads
-// G~ in the product (240 -> 159 registers: 1578 against 1653 us at 256^3,
-// where FP64 that way takes 3884 against 2983 us).
+// (FP32 96: 211 -> 159 registers, 128); FP32 256 also reads G~ in the
+// product (240 -> 159 registers: 1578 against 1653 us at 256^3, where FP64
+// that way takes 3884 against 2983 us).  FP32 64 (the short x period of the
+// 128^3 layered workload, 48-plane window): G~ read in the product without
+// OP is 128 registers, four waves, and 407 VALU instructions per pencil
+// against 503 with OP: 56.9-58.0 us per launch against 60.8 us (G~
+// prefetched with OP), 61.9 us (prefetched, no OP) and 60.2 us (T8 <16,8>).
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
 #define VIE_XREG(R, N, T_, GM_, OP_, ...)                                                                    \
     template <> struct XReg<R, N> {                                                                       \
@@ -204,7 +208,7 @@ template <class R, int N> struct XReg { static constexpr bool ok = false; };
         using PL = XPlan<__VA_ARGS__>;                                                                    \
     };
 VIE_XREG(float, 32, 8, 2, false, 8, 8)
-VIE_XREG(float, 64, 16, 2, true, 8, 4, 4)
+VIE_XREG(float, 64, 16, 0, false, 8, 4, 4)
 VIE_XREG(float, 96, 16, 2, true, 4, 4, 4, 3)
 VIE_XREG(float, 128, 16, 2, true, 16, 16)
 VIE_XREG(float, 256, 32, 0, true, 16, 16, 2)

@@ -29,6 +29,7 @@ template <int... Rs> struct XPlan {
         for (int u = 0; u < s; ++u) v *= r[u];
         return v;
     }
+    static constexpr int product = (Rs * ...);
 };
 
 // CS: column-slow thread map (col = tid / T, the z passes, whose lanes walk
@@ -55,6 +56,7 @@ template <class R, int N, int T, class PL, bool CS = false> struct XCfg {
     static constexpr int step(int d) { return CS ? d + d / M : (d + d / M) * B; }
     static_assert((1 << LM) == M, "pad period must be a power of two");
     static_assert(P % T == 0, "T must divide P");
+    static_assert(PL::product == P, "the radices must multiply to the padded length");
     static_assert(B * T == NT, "T must divide the workgroup");
 };
 
