@@ -198,6 +198,36 @@ def test_embedded_slab_matches_oracle():
         assert abs(s.absorbed(1) - O.absorbed(n, D, K0, xs, eps, mat, 1)) <= 1e-11 * abs(s.absorbed(1))
 
 
+def test_one_context_follows_a_new_medium_and_k0():
+    """One context: a z slab (internal order z, x, y) solved in mixed precision,
+    then an x slab (caller's order) after G~ was computed -- the operators,
+    G~ and the short period are rebuilt for the new order without a second
+    green_compute -- then a new wavelength."""
+    mat_z, eps_z, fl_z = slab_scene(2, 6, 15, 12)
+    mat_x, eps_x, fl_x = slab_scene(0, 5, 11, 13)
+    b_z = O.incident(N, D, K0) * fl_z
+    b_x = O.incident(N, D, K0, 30.0, 10.0, 20.0) * fl_x
+    with solver(N, mat_z, fl_z) as s:
+        s.set_rhs(b_z)
+        m = s.solve_mixed(tol=1e-9)
+        assert m["converged"] and s.plan()["axis_order"] == (2, 0, 1)
+        assert rel(s.solution(), oracle(N, eps_z, fl_z).solve(b_z, 3, tol=1e-10, maxit=2000)["x"]) <= 1e-7
+        s.set_medium(mat_x, TABLE, fl_x)
+        assert s.plan()["axis_order"] == (0, 1, 2) and s.plan()["x_window"] == (5, 11)
+        x, info = s.solve(b_x, VF.BICGSTAB, tol=1e-300, maxit=8)
+        ref = oracle(N, eps_x, fl_x).solve(b_x, 3, tol=1e-300, maxit=8)
+        assert rel(x, ref["x"]) <= 1e-9
+        s.set_rhs(b_x)
+        m = s.solve_mixed(tol=1e-9)
+        assert m["converged"]
+        k1 = 2 * np.pi / 3.3
+        s.set_k0(k1)
+        s.green_compute()
+        x, _ = s.solve(b_x, VF.BICGSTAB, tol=1e-300, maxit=8)
+        ref = O.Operator(N, D, k1, eps_x, fl_x, O.SCHED_FUSED).solve(b_x, 3, tol=1e-300, maxit=8)
+        assert rel(x, ref["x"]) <= 1e-9
+
+
 @pytest.mark.parametrize("axis,lo,hi", SLABS)
 def test_window_off_gives_the_same_solves(axis, lo, hi):
     mat, eps, flags = slab_scene(axis, lo, hi, 9)
