@@ -35,7 +35,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # 48 planes on 64-length pencils), measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (scaled by a plain copy of known size):
 # profiles/k_xreg_f32_128cube_hbm_traffic.md
-XPASS_F32_128_TRAFFIC = 168.6e6
+XPASS_F32_128_TRAFFIC = 168.5e6
 XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil": 64, "grid": (128, 128, 128)}
 
 # register-resident x-pass kernels by pencil length (FP32)
