@@ -37,6 +37,11 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # profiles/k_xreg_f32_128cube_hbm_traffic.md
 XPASS_F32_128_TRAFFIC = 168.5e6
 XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil": 64, "grid": (128, 128, 128)}
+# VALU instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, median launch):
+# profiles/k_xreg_f32_128cube_valu.md.  The pass is VALU-bound; MI355X issues at most
+# 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles = 614.4e9 wave64 VALU instructions per second.
+XPASS_F32_128_VALU = 25878528
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # register-resident x-pass kernels by pencil length (FP32)
 XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false>", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,0,false>",
@@ -172,6 +177,9 @@ def main():
                                f"{w}-plane window on {nxp}-length pencils)",
                      "bytes_per_launch": xb, "mean_ms": ms13, "launches": n13},
     }
+    if plan == XPASS_F32_128_PLAN and ms13 > 0:
+        out["roofline"]["valu"] = {"insts_per_launch": XPASS_F32_128_VALU, "issue_peak_per_s": VALU_ISSUE_PEAK,
+                                   "frac": XPASS_F32_128_VALU / VALU_ISSUE_PEAK / (ms13 * 1e-3)}
     # measured practical HBM rate (device copy) beside the published peak
     _, _, copy_gbps = copy_rate(torch, n[0])
     out["roofline"]["copy_GBps"] = copy_gbps

@@ -2,6 +2,7 @@
 
     python -m vie_fft_amd.prof_summary trace  <dir>/bench_kernel_trace.csv
     python -m vie_fft_amd.prof_summary traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <copy_bytes>
+    python -m vie_fft_amd.prof_summary valu <counter_collection.csv> [<trace.csv>]
 
 ``trace`` lists every kernel with its launch count and mean duration.  Krylov
 launches enqueued after convergence return at once (a device flag), so the
@@ -11,6 +12,12 @@ the same rule ``vie_kernel_time`` applies to its HIP-event timings.
 ``traffic`` turns per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) into bytes per
 launch, each scaled by the ratio of the known size of a plain device copy
 to what the same counter reads for that copy.
+
+``valu`` lists per-launch SQ_INSTS_VALU and SQ_WAVES (the median launch, so
+launches that returned at once do not count) and, with a kernel trace of the
+same command, the fraction of MI355X's VALU issue rate that the mean active
+launch reaches: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
+cycles at 2.4 GHz = 614.4e9 wave instructions per second.
 """
 from __future__ import annotations
 
@@ -74,10 +81,38 @@ def traffic(fetch, write, copy_fetch, copy_write, copy_bytes):
     return "\n".join(lines)
 
 
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second
+
+
+def valu(counters, trace_csv=None):
+    ins, waves = _counter(counters, "SQ_INSTS_VALU"), _counter(counters, "SQ_WAVES")
+    dur = {}
+    if trace_csv:
+        by = {}
+        for r in csv.DictReader(open(trace_csv)):
+            by.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        dur = {k: _robust(v)[0] for k, v in by.items()}
+    lines = ["| kernel | VALU instructions / launch | waves | per wave | µs (trace, active) | VALU issue fraction |",
+             "|---|---|---|---|---|---|"]
+    for k in sorted(ins, key=lambda kk: -statistics.median(ins[kk])):
+        if k not in waves:
+            continue
+        i, wv = statistics.median(ins[k]), statistics.median(waves[k])
+        if i < 1e6:
+            continue
+        us = dur.get(k)
+        frac = f"{i / VALU_ISSUE_PEAK / (us * 1e-6):.3f}" if us else ""
+        lines.append(f"| `{k[:120]}` | {i:.0f} | {wv:.0f} | {i / wv:.1f} | {us:.1f} | {frac} |" if us else
+                     f"| `{k[:120]}` | {i:.0f} | {wv:.0f} | {i / wv:.1f} | | |")
+    return "\n".join(lines)
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
     elif sys.argv[1] == "traffic":
         print(traffic(*sys.argv[2:7]))
+    elif sys.argv[1] == "valu":
+        print(valu(*sys.argv[2:4]))
     else:
         raise SystemExit(__doc__)
