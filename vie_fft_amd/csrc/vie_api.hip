@@ -510,15 +510,24 @@ template <class R> void fill0(vie_ctx* c, cx<R>* x, long n)
 // inactive faces, so the kernel reads inputs and dot operands only there.
 template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 {
-    if (Op<R>* o = op_of<R>(c)) L.mask = o->flags;
-    const unsigned gsz = grid1((L.n + 1) / 2, NT, 2048);
+    Op<R>* o = op_of<R>(c);
+    if (o) L.mask = o->flags;
+    // outputs kept zero on inactive faces (wskip): run over the x window only
+    const long pl = o ? (long)o->g.ny * o->g.nz : 0;
+    if (o && L.wskip && L.n == 3 * c->V && (o->wx0 > 0 || o->wx1 < o->g.nx) && pl % 2 == 0) {
+        L.wV = c->V;
+        L.wlo = o->wx0 * pl;
+        L.whi = o->wx1 * pl;
+    }
+    const unsigned gsz = grid1(L.wV ? (L.whi - L.wlo) / 2 : (L.n + 1) / 2, NT, L.wV ? 1024 : 2048);
+    const unsigned gy = L.wV ? 3 : 1;
     L.dots.partial = c->partial;
     const int cls = sizeof(R) == 8 ? 6 : 16;
     if (c->timer.cls == cls) c->timer.mark(c->stream);
-    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz), dim3(NT), 0, c->stream, L, done);
+    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz, gy), dim3(NT), 0, c->stream, L, done);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == cls) c->timer.mark(c->stream);
-    return gsz;
+    return (long)gsz * gy;
 }
 
 void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)

@@ -33,6 +33,10 @@ template <class R> struct LinComb {
     Dots<R> dots;
     const uint8_t* mask;
     int wskip;  // with mask: out and out2 already hold zero on inactive faces
+    // with mask and wskip, a window: only the entries [wlo, whi) of each of
+    // the three components (length wV, all even) hold active faces; the
+    // kernel then runs over them alone (grid.y = 3).  wV = 0: every entry.
+    long wV, wlo, whi;
 };
 
 // Active range of each z-row of the face flags (rows = 3 Nx Ny, component
@@ -72,8 +76,10 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
     }
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
     const long stride = (long)gridDim.x * NT;
-    // pairs of entries (16-byte accesses for FP32), then a possible odd tail
-    const long npair = L.n / 2;
+    // pairs of entries (16-byte accesses for FP32), then a possible odd tail;
+    // with a window, the pairs of component blockIdx.y within it
+    const long npair = L.wV ? (L.whi - L.wlo) / 2 : L.n / 2;
+    const long base = L.wV ? (long)blockIdx.y * L.wV + L.wlo : 0;
     bool use1 = false, use2 = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
         return r;
     };
     for (long i2 = (long)blockIdx.x * NT + threadIdx.x; i2 < npair; i2 += stride) {
-        const long i = 2 * i2;
+        const long i = base + 2 * i2;
         // every load of the pair first (more bytes in flight per thread)
         T a0[4], a1[4], e10 = czero<T>(), e11 = czero<T>(), e20 = czero<T>(), e21 = czero<T>();
         const bool act = !L.mask || *reinterpret_cast<const uint16_t*>(L.mask + i) != 0;
@@ -127,7 +133,7 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
             if (L.out2) st2(L.out2 + i, v2, w2);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (L.n & 1)) {
+    if (!L.wV && blockIdx.x == 0 && threadIdx.x == 0 && (L.n & 1)) {
         const long i = L.n - 1;
         T v = czero<T>(), v2 = czero<T>();
         for (int q = 0; q < 4; ++q)
@@ -147,7 +153,8 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
     for (int d = 0; d < 3; ++d) {
         if (d >= L.dots.n) break;
         double2 s = block_sum(acc[d], red);
-        if (threadIdx.x == 0) L.dots.partial[d * (long)gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0)
+            L.dots.partial[d * (long)gridDim.x * gridDim.y + (long)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
 }
 
