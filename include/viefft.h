@@ -93,7 +93,11 @@ int vie_supported_n(int n);
 /* ---- medium ------------------------------------------------------------- */
 /* material: V bytes (Material, vie_fft/Model.h:17), eps_table: n_mat complex
    (Real_EPR/Imag_EPR, vie_fft/Model.h:82-83, engineering sign), flags: 3*V
-   bytes Flag_X|Flag_Y|Flag_Z (vie_fft/Model.h:21-23). */
+   bytes Flag_X|Flag_Y|Flag_Z (vie_fft/Model.h:21-23).  The library may order
+   its internal grid differently (the axis whose active faces span the fewest
+   planes first, see vie_plan); every array at this interface keeps the
+   caller's layout.  A medium with another internal order drops the resident
+   right-hand side and solution. */
 int vie_set_medium(vie_ctx* ctx, const unsigned char* material, const double* eps_table,
                    int n_mat, const unsigned char* flags);
 /* per-voxel permittivity, V complex */
