@@ -11,6 +11,7 @@ oracle's, which works on the caller's grid with every plane:
   far-field, absorbed-power and gradient outputs, and the mixed solve;
 * the x pass on a shorter period than the grid's (register-resident and
   LDS-staged pencils);
+* boxes limited in two or three axes (x and y windows);
 * an embedded grid (lengths without compiled kernels) with a slab;
 * the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
 """
@@ -53,6 +54,26 @@ def slab_scene(axis, lo, hi, seed, n=N):
     return mat, TABLE[mat], flags
 
 
+def box_scene(lo, hi, seed, n=N):
+    """Material ids 1-2 in the box [lo, hi) (per axis), background elsewhere;
+    flags as in slab_scene."""
+    rng = np.random.default_rng(seed)
+    mat = np.zeros(n, np.uint8)
+    sl = tuple(slice(a, b) for a, b in zip(lo, hi))
+    mat[sl] = rng.integers(1, 3, size=mat[sl].shape)
+    inside = mat > 0
+    flags = np.zeros((3,) + tuple(n), np.uint8)
+    for f in range(3):
+        nb = np.zeros_like(inside)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        src[f] = slice(1, None)
+        dst[f] = slice(0, -1)
+        nb[tuple(dst)] = inside[tuple(src)]
+        flags[f] = inside & nb
+    return mat, TABLE[mat], flags
+
+
 def solver(n, mat, flags, precision=VF.FP64):
     s = VF.Solver(n, D, K0, precision)
     s.set_medium(mat, TABLE, flags)
@@ -78,7 +99,7 @@ def test_plan_puts_the_slab_axis_first(axis, lo, hi, order, pencil):
             p = s.plan()
     finally:
         os.environ.pop("VIE_WINDOW", None)
-    assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N}
+    assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N, "y_window": (0, N[1])}
 
 
 @pytest.mark.parametrize("axis,lo,hi", SLABS)
@@ -179,6 +200,42 @@ def test_short_period_register_pass_matches_oracle(n, lo, hi):
     with solver(n, mat, flags, VF.FP32) as s:
         x32, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
     assert rel(x32, ref["x"]) <= 1e-4, rel(x32, ref["x"])
+
+
+# boxes limited in two or three axes: x window and y window (the BiCGSTAB
+# products skip the z rows and y entries outside the latter)
+BOXES = [((4, 2, 0), (12, 9, 20), (0, 1, 2), (4, 12), (2, 9)),
+         ((0, 3, 5), (16, 10, 11), (2, 1, 0), (5, 11), (3, 10)),
+         ((3, 0, 4), (13, 12, 18), (0, 2, 1), (3, 13), (4, 18))]
+
+
+@pytest.mark.parametrize("lo,hi,order,xw,yw", BOXES)
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
+def test_box_fixed_iterations_match_oracle(lo, hi, order, xw, yw, method):
+    mat, eps, flags = box_scene(lo, hi, 14)
+    b = O.incident(N, D, K0, 20.0, 30.0, 40.0) * flags
+    ref = oracle(N, eps, flags).solve(b, method, tol=1e-300, maxit=8)
+    with solver(N, mat, flags) as s:
+        p = s.plan()
+        assert p["axis_order"] == order and p["x_window"] == xw and p["y_window"] == yw
+        x, info = s.solve(b, method, tol=1e-300, maxit=8)
+    assert info["iterations"] == 8
+    assert rel(x, ref["x"]) <= 1e-9, rel(x, ref["x"])
+
+
+@pytest.mark.parametrize("lo,hi,order,xw,yw", BOXES)
+def test_box_converged_and_mixed_match_oracle(lo, hi, order, xw, yw):
+    mat, eps, flags = box_scene(lo, hi, 15)
+    b = O.incident(N, D, K0, 10.0, 60.0, 30.0) * flags
+    ref = oracle(N, eps, flags).solve(b, 3, tol=1e-10, maxit=2000)
+    with solver(N, mat, flags) as s:
+        s.set_rhs(b)
+        info = s.solve_resident(VF.BICGSTAB, tol=1e-10)
+        assert info["converged"] and abs(info["iterations"] - ref["iterations"]) <= 1
+        assert rel(s.solution(), ref["x"]) <= 1e-8
+        m = s.solve_mixed(tol=1e-9)
+        assert m["converged"] and m["true_relres"] < 1e-9
+        assert rel(s.solution(), ref["x"]) <= 1e-7
 
 
 def test_embedded_slab_matches_oracle():

@@ -147,6 +147,7 @@ template <class R> struct Op {
     uint8_t* flags = nullptr;
     uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
     int wx0 = 0, wx1 = 0;      // x window: the x planes with an active face
+    int wy0 = 0, wy1 = 0;      // y window: the y planes with an active face
     // Short-period x pass: a product whose input and output live on the x
     // window [wx0, wx1) is a linear convolution along x of width
     // w = wx1 - wx0, which a padded transform of any length 2 nxs with
@@ -227,6 +228,8 @@ template <class R> struct Op {
         g.V = (long)n[0] * n[1] * n[2];
         g.x0 = wx0 = 0;
         g.x1 = wx1 = n[0];
+        g.y0 = wy0 = 0;
+        g.y1 = wy1 = n[1];
         for (int a = 0; a < 3; ++a) build_axis(a, n[a], st);
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
@@ -339,7 +342,8 @@ template <class R> struct Op {
     // diagonal term multiplies (z itself).  Returns the partial-sum count.
     // With row ranges (lin.zr: the inputs vanish on inactive faces) the passes
     // cover only the x window; outside it the output and lin.out are zero,
-    // written here unless the caller keeps them zero (wskip).
+    // written here unless the caller keeps them zero (wskip), in which case
+    // the z rows and y entries outside the y window are skipped too.
     long apply(const Lin<R>& lin, const T* src, T* y, Dots<R> dots, bool adj, const int* done,
                hipStream_t st, Timer* tm)
     {
@@ -348,6 +352,10 @@ template <class R> struct Op {
         if (lin.zr) {
             gw.x0 = wx0;
             gw.x1 = wx1;
+            if (lin.wskip) {
+                gw.y0 = wy0;
+                gw.y1 = wy1;
+            }
         }
         if ((gw.x0 > 0 || gw.x1 < g.nx) && !lin.wskip) {
             const long pl = (long)g.ny * g.nz;
@@ -654,22 +662,27 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
     // x window: the x planes holding an active face of any component (all of
     // them when windows are off or no face is active)
-    const long pl = (long)n[1] * n[2];
-    int lo = n[0], hi = 0;
+    int lo = n[0], hi = 0, ylo = n[1], yhi = 0;
     for (int f = 0; f < 3; ++f)
-        for (int i = 0; i < n[0]; ++i) {
-            const uint8_t* p = c->flags_h.data() + f * c->V + i * pl;
-            if (std::any_of(p, p + pl, [](uint8_t v) { return v != 0; })) {
-                lo = std::min(lo, i);
-                hi = std::max(hi, i + 1);
+        for (int i = 0; i < n[0]; ++i)
+            for (int j = 0; j < n[1]; ++j) {
+                const uint8_t* p = c->flags_h.data() + f * c->V + ((long)i * n[1] + j) * n[2];
+                if (std::any_of(p, p + n[2], [](uint8_t v) { return v != 0; })) {
+                    lo = std::min(lo, i);
+                    hi = std::max(hi, i + 1);
+                    ylo = std::min(ylo, j);
+                    yhi = std::max(yhi, j + 1);
+                }
             }
-        }
     if (hi <= lo || !c->windows) {
-        lo = 0;
+        lo = ylo = 0;
         hi = n[0];
+        yhi = n[1];
     }
     op->wx0 = lo;
     op->wx1 = hi;
+    op->wy0 = ylo;
+    op->wy1 = yhi;
     const long rows = 3L * n[0] * n[1];
     hipLaunchKernelGGL(k_zrange, dim3(grid1(rows)), dim3(NT), 0, c->stream, rows, n[2], op->flags, op->zr);
     HIPCHK(hipGetLastError());
@@ -1387,9 +1400,9 @@ template <class T> void crop_h(const vie_ctx* c, const T* in, T* out, int ncomp)
 }
 
 // Internal axis order for a medium with these (caller-layout) flags: the
-// axis whose active faces span the smallest fraction of its planes first,
-// the other two in the caller's order; the caller's order on ties, when no
-// face is active or with windows off.
+// axes by increasing fraction of their planes that hold an active face (the
+// x window of the passes the narrowest, the y window next); the caller's
+// order on ties, when no face is active or with windows off.
 void choose_frame(const vie_ctx* c, const unsigned char* flags, int* perm)
 {
     perm[0] = 0;
@@ -1417,19 +1430,9 @@ void choose_frame(const vie_ctx* c, const unsigned char* flags, int* perm)
                 hi[2] = std::max(hi[2], k1);
             }
     if (hi[0] < 0) return;
-    int best = 0;
-    double fb = 2.0;
-    for (int b = 0; b < 3; ++b) {
-        const double fr = (double)(hi[b] - lo[b] + 1) / c->nr[b];
-        if (fr < fb) {
-            fb = fr;
-            best = b;
-        }
-    }
-    int k = 1;
-    perm[0] = best;
-    for (int b = 0; b < 3; ++b)
-        if (b != best) perm[k++] = b;
+    double fr[3];
+    for (int b = 0; b < 3; ++b) fr[b] = (double)(hi[b] - lo[b] + 1) / c->nr[b];
+    std::stable_sort(perm, perm + 3, [&](int a, int b) { return fr[a] < fr[b]; });
 }
 
 using cd = std::complex<double>;
@@ -2394,7 +2397,7 @@ int vie_kernel_time(vie_ctx* c, int* launches, double* mean_ms)
 double vie_model_bytes_matvec(const vie_ctx* c)
 {
     if (!c) return 0.0;
-    int p[9];
+    int p[11];
     if (vie_plan(c, p) != VIE_OK) return 0.0;
     const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
     const double Vw = (double)(p[4] - p[3]) * p[7] * p[8];
@@ -2406,14 +2409,18 @@ int vie_plan(const vie_ctx* c, int* out)
 {
     if (!c || !out) return VIE_ERR_ARG;
     const int* n = c->desc.n;
-    int x0 = 0, x1 = n[0], ns = 0;
+    int x0 = 0, x1 = n[0], y0 = 0, y1 = n[1], ns = 0;
     if (c->op64) {
         x0 = c->op64->wx0;
         x1 = c->op64->wx1;
+        y0 = c->op64->wy0;
+        y1 = c->op64->wy1;
         ns = c->op64->nxs;
     } else if (c->op32) {
         x0 = c->op32->wx0;
         x1 = c->op32->wx1;
+        y0 = c->op32->wy0;
+        y1 = c->op32->wy1;
         ns = c->op32->nxs;
     }
     for (int a = 0; a < 3; ++a) {
@@ -2423,6 +2430,8 @@ int vie_plan(const vie_ctx* c, int* out)
     out[3] = x0;
     out[4] = x1;
     out[5] = ns ? ns : n[0];
+    out[9] = y0;
+    out[10] = y1;
     return VIE_OK;
 }
 

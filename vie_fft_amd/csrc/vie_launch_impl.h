@@ -61,7 +61,7 @@ template <class R, int n>
 static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
-    const long rows = (long)(g.x1 - g.x0) * g.ny;  // the x window's rows
+    const long rows = win_rows(g);  // the rows of the x and y windows
     if constexpr (YZReg<R, n>::ok) {
         if (zfwd_reg() && !yz_lds_forced()) {
             using X = YZReg<R, n>;
@@ -124,7 +124,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
                               const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st, long* nblk)
 {
-    const long rows = (long)(g.x1 - g.x0) * g.ny;  // the x window's rows
+    const long rows = win_rows(g);  // the rows of the x and y windows
     if constexpr (ZInvReg<R, n>::ok) {
         if (!yz_lds_forced() && dg.d) {
             using X = ZInvReg<R, n>;

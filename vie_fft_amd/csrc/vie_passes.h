@@ -17,14 +17,25 @@ namespace vie {
 // [x0, x1) is the x window the passes cover (0 <= x0 < x1 <= nx, set by the
 // owner of the Geo): the x planes outside it carry no active face, so a
 // product whose input vanishes there needs neither its z/y transforms on
-// those planes nor its x-pencil entries there.  The x pass's pencil entry n
-// is plane n + xs (xs = x0 when the pencils have the short period).
+// those planes nor its x-pencil entries there.  [y0, y1) (0 <= y0 < y1 <= ny)
+// likewise bounds the z rows (x, y) the z passes transform and the y entries
+// the y passes load and store.  The x pass's pencil entry n is plane n + xs
+// (xs = x0 when the pencils have the short period).
 struct Geo {
     int nx, ny, nz;
     long V;
     int x0 = 0, x1 = 0;
+    int y0 = 0, y1 = 0;
     int xs = 0;
 };
+
+// z row of window row r, r < (x1 - x0)(y1 - y0): x0 + r / wy, y0 + r % wy
+__device__ __forceinline__ long win_row(const Geo& g, int r)
+{
+    const int wy = g.y1 - g.y0;
+    return (long)(g.x0 + r / wy) * g.ny + g.y0 + r % wy;
+}
+__host__ __device__ __forceinline__ int win_rows(const Geo& g) { return (g.x1 - g.x0) * (g.y1 - g.y0); }
 
 // Tables of the spectral product, per axis a in split order t = e*N_a + m
 // (frequency k = 2m + e, theta = 2 pi k / (2 N_a)):
@@ -156,15 +167,19 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     if (done && *done) return;
     const int s = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
-    const long rend = (long)g.x1 * g.ny;  // rows of the x window: [x0 Ny, x1 Ny)
-    const long row0 = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B;
+    const int nw = win_rows(g);  // the window's rows (column col: window row r0 + col)
+    const int r0 = (int)xcd_tile(blockIdx.x, gridDim.x) * B;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
     __shared__ unsigned zrs[B];  // active range of each row (the whole row without lin.zr)
-    for (int t = threadIdx.x; t < B; t += NT)
-        zrs[t] = lin.zr && row0 + t < rend ? lin.zr[s * rows + row0 + t] : ((unsigned)N << 16);
+    __shared__ long rw[B];       // z row of each column
+    for (int t = threadIdx.x; t < B; t += NT) {
+        const bool ok = r0 + t < nw;
+        rw[t] = ok ? win_row(g, r0 + t) : 0;
+        zrs[t] = lin.zr && ok ? lin.zr[s * rows + rw[t]] : ((unsigned)N << 16);
+    }
     T c[3];
     load_coef<R>(lin.coef, c);
     __syncthreads();
@@ -176,11 +191,12 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
         for (int it = 0; it < IT; ++it) {
             const int e = threadIdx.x + it * NT;
             const int col = e / (N / 2), n = 2 * (e % (N / 2));
-            const long row = row0 + col;
+            const bool rok = e < B * N / 2 && r0 + col < nw;
+            const long row = rok ? rw[col] : 0;
             x0[it][0] = x0[it][1] = x1[it][0] = x1[it][1] = x2[it][0] = x2[it][1] = czero<T>();
             // with row ranges, pairs that miss the row's active range are zero
             const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
-            if (e < B * N / 2 && row < rend && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
+            if (rok && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
                 const long idx = s * g.V + row * N + n;
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
@@ -192,9 +208,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int e = threadIdx.x + it * NT;
             if (e >= B * N / 2) break;
             const int col = e / (N / 2), n = 2 * (e % (N / 2));
-            const long row = row0 + col;
+            const long row = rw[col];
             T v0 = czero<T>(), v1 = czero<T>();
-            if (row < rend) {
+            if (r0 + col < nw) {
                 const long idx = s * g.V + row * N + n;
                 v0 = cmul(c[0], x0[it][0]);
                 v1 = cmul(c[0], x0[it][1]);
@@ -220,9 +236,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     } else {
         for (int e = threadIdx.x; e < B * N; e += NT) {
             const int col = e / N, n = e % N;
-            const long row = row0 + col;
+            const long row = rw[col];
             T v = czero<T>();
-            if (row < rend) {
+            if (r0 + col < nw) {
                 const long idx = s * g.V + row * N + n;
                 const unsigned rg = zrs[col];
                 if (n >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
@@ -245,8 +261,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     // t and t+1 share the half e when N is even (pairs never straddle N)
     for (int e = threadIdx.x; e < B * N; e += NT) {
         const int col = e / N, t = 2 * (e % N);
-        const long row = row0 + col;
-        if (row < rend) {
+        const long row = rw[col];
+        if (r0 + col < nw) {
             const int ep = t >= N, m = t - ep * N;
             T* dst = W1 + (s * rows + row) * (2 * N) + t;
             if constexpr (N % 2 == 0)
@@ -291,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
     for (int e = threadIdx.x; e < N * B / 2; e += NT) {
         const int j = e / (B / 2), col = 2 * (e % (B / 2));
         T v0 = czero<T>(), v1 = czero<T>();
-        if (z0 + col < P2) ld2(W1 + in0 + (long)j * P2 + col, v0, v1);
+        if (z0 + col < P2 && j >= g.y0 && j < g.y1) ld2(W1 + in0 + (long)j * P2 + col, v0, v1);
         buf[j * LD + col] = v0;
         buf[j * LD + col + 1] = v1;
         buf[j * LD + B + col] = cmul(v0, tws[j]);
@@ -431,7 +447,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
     const long out0 = ((long)s * g.nx + i) * N * P2 + z0;
     for (int e = threadIdx.x; e < N * B / 2; e += NT) {
         const int j = e / (B / 2), col = 2 * (e % (B / 2));
-        if (z0 + col < P2)
+        if (z0 + col < P2 && j >= g.y0 && j < g.y1)
             st2(W1 + out0 + (long)j * P2 + col, cadd(buf[j * LD + col], cmulc(buf[j * LD + B + col], tws[j])),
                 cadd(buf[j * LD + col + 1], cmulc(buf[j * LD + B + col + 1], tws[j])));
     }
@@ -455,17 +471,20 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     if (done && *done) return;
     const int f = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
-    const long rend = (long)g.x1 * g.ny;  // rows of the x window: [x0 Ny, x1 Ny)
-    const long row0 = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B;
+    const int nw = win_rows(g);  // the window's rows (column col: window row r0 + col)
+    const int r0 = (int)xcd_tile(blockIdx.x, gridDim.x) * B;
+    __shared__ long rw[B];       // z row of each column
+    for (int t = threadIdx.x; t < B; t += NT) rw[t] = r0 + t < nw ? win_row(g, r0 + t) : 0;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
+    __syncthreads();
     for (int e = threadIdx.x; e < B * N; e += NT) {
         const int col = e / N, t = 2 * (e % N);
-        const long row = row0 + col;
+        const long row = rw[col];
         T v0 = czero<T>(), v1 = czero<T>();
-        if (row < rend) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
+        if (r0 + col < nw) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
         const int ep = t >= N, m = t - ep * N;
         buf[m * LD + ep * B + col] = v0;
         const int t1 = t + 1, ep1 = t1 >= N, m1 = t1 - ep1 * N;
@@ -489,8 +508,8 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     if constexpr (N % 2 == 0) {
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {
             const int col = e / (N / 2), k = 2 * (e % (N / 2));
-            const long row = row0 + col;
-            if (row < rend) {
+            const long row = rw[col];
+            if (r0 + col < nw) {
                 const long idx = f * g.V + row * N + k;
                 term(idx, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
                 term(idx + 1, cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1])));
@@ -499,8 +518,8 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     } else {
         for (int e = threadIdx.x; e < B * N; e += NT) {
             const int col = e / N, k = e % N;
-            const long row = row0 + col;
-            if (row < rend) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
+            const long row = rw[col];
+            if (r0 + col < nw) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
         }
     }
     if (dots.n > 0) {

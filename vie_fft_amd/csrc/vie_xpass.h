@@ -413,9 +413,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     const int s = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;  // consecutive lanes: consecutive cells
-    // rows of the x window: [x0 Ny, x1 Ny)
-    const long row = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
-    const bool rowok = row < (long)g.x1 * g.ny;
+    // window row r of the x and y windows
+    const int r = (int)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = r < win_rows(g);
+    const long row = rowok ? win_row(g, r) : 0;
     T_ c[3];
     load_coef<R>(lin.coef, c);
     const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
@@ -489,9 +490,10 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     const int f = blockIdx.y;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;
-    // rows of the x window: [x0 Ny, x1 Ny)
-    const long row = (long)g.x0 * g.ny + (long)xcd_tile(blockIdx.x, gridDim.x) * B + col;
-    const bool rowok = row < (long)g.x1 * g.ny;
+    // window row r of the x and y windows
+    const int r = (int)xcd_tile(blockIdx.x, gridDim.x) * B + col;
+    const bool rowok = r < win_rows(g);
+    const long row = rowok ? win_row(g, r) : 0;
     T_ v[E];
     {
         const T_* in = W1 + (f * rows + (rowok ? row : 0)) * (2 * N);
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
         for (int q = 0; q < r1; ++q) {
             const int n = ii + q * (P / r1);
-            v[b * r1 + q] = (n < N && colok) ? in[(long)n * P2] : czero<T_>();
+            v[b * r1 + q] = (n < N && colok && n >= g.y0 && n < g.y1) ? in[(long)n * P2] : czero<T_>();
         }
     }
     for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
@@ -652,7 +654,7 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = ii + q * (P / r1);
-                if (n < N) out[(long)n * P2] = v[b * r1 + q];
+                if (n < N && n >= g.y0 && n < g.y1) out[(long)n * P2] = v[b * r1 + q];
             }
         }
     }

@@ -321,10 +321,12 @@ class Solver:
 
     def plan(self):
         """Internal plan of the resident medium: axis order, x window of the
-        products inside solves, pencil length of their x pass, internal grid."""
-        a = (ct.c_int * 9)()
+        products inside solves, pencil length of their x pass, internal grid,
+        y window of the BiCGSTAB products."""
+        a = (ct.c_int * 11)()
         self._check(self.lib.vie_plan(self.h, a))
-        return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9])}
+        return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9]),
+                "y_window": (a[9], a[10])}
 
 
 def supported_n(n):
