@@ -114,6 +114,29 @@ def test_slab_operator_matches_oracle(axis, lo, hi):
         assert rel(s.apply(x), op.apply(x)) <= 3e-6
 
 
+def test_slab_per_voxel_eps_matches_oracle():
+    """vie_set_medium_eps (per-voxel permittivity) on a z slab: reordered axes."""
+    mat, _, flags = slab_scene(2, 6, 15, 16)
+    rng = np.random.default_rng(17)
+    eps = rng.uniform(1.5, 6.0, N) - 1j * rng.uniform(0.0, 1.0, N)
+    op = oracle(N, eps, flags)
+    x = random_field(N, 18) * flags
+    b = O.incident(N, D, K0, 40.0, 20.0, 10.0) * flags
+    ref = op.solve(b, 3, tol=1e-300, maxit=8)
+    with VF.Solver(N, D, K0) as s:
+        s.set_medium_eps(eps, flags)
+        s.green_compute()
+        assert s.plan()["axis_order"] == (2, 0, 1)
+        assert rel(s.apply(x), op.apply(x)) <= 1e-12
+        assert rel(s.apply(x, True), op.apply(x, True)) <= 1e-12
+        xs, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+        assert rel(xs, ref["x"]) <= 1e-9
+        s.set_rhs(b)
+        s.solve_resident(VF.BICGSTAB, tol=1e-10)
+        J = s.solution()
+        assert abs(s.absorption() - O.absorption(N, D, K0, J, eps, flags)) <= 1e-12 * s.absorption()
+
+
 @pytest.mark.parametrize("precision", [VF.FP64, VF.FP32])
 def test_slab_apply_async_equals_apply(precision):
     import torch
