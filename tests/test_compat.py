@@ -31,11 +31,17 @@ def _write_case(path, sc, eps_table, material, flags, angles):
 
 
 @pytest.mark.gpu
-def test_compat_driver_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("case", ["sphere", "layered"])
+def test_compat_driver_matches_c_abi(tmp_path, case):
+    """The layered medium (bands along z) runs on the library's reordered
+    internal axes; the sphere keeps the caller's order."""
     from vie_fft_amd import viefft as VF
 
     drv = BD.OUT if os.path.exists(BD.OUT) else BD.build(str(tmp_path / "driver.bin"))
-    sc = G.sphere_case(n=20, radius=6.0, eps_phys=-4.0 + 0.7j, wavelength_nm=450.0)
+    if case == "sphere":
+        sc = G.sphere_case(n=20, radius=6.0, eps_phys=-4.0 + 0.7j, wavelength_nm=450.0)
+    else:
+        sc = G.layered_case(n=(24, 20, 32), band_start=4, sphere_z=14.0)
     mat, flags = sc.material(), sc.flags()
     angles = (0.0, 0.0, 30.0)
     case, out = tmp_path / "case.bin", tmp_path / "out.bin"
