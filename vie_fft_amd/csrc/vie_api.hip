@@ -430,14 +430,14 @@ struct vie_ctx {
     long Vu = 0;
     bool emb = false;         // the caller's grid is embedded in a larger compiled one
     // Internal axis order: internal axis a is the caller's axis perm[a]
-    // (desc.n and desc.d are in internal order); set_medium puts the axis
-    // whose active faces span the fewest planes first, so the passes' x
-    // window is the narrowest.
+    // (desc.n and desc.d are in internal order); set_medium orders the axes
+    // by the fraction of their planes holding an active face, narrowest
+    // first, so the passes' x window is the narrowest and the y window next.
     int perm[3] = {0, 1, 2};
     int nr[3] = {0, 0, 0};     // compiled grid in the caller's axis order
     double dr[3] = {0, 0, 0};  // cell sizes in the caller's axis order
     bool remap = false;        // caller layout != internal layout (embedding or axis order)
-    bool windows = true;       // x windows and the axis order that favours them (VIE_WINDOW=0: off)
+    bool windows = true;       // x/y windows and the axis order that favours them (VIE_WINDOW=0: off)
     vie_desc desc{};
     hipStream_t stream = nullptr;
     std::string err;
@@ -660,8 +660,8 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
                        cim[2], c->eps_dev, op->dg);
     HIPCHK(hipGetLastError());
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
-    // x window: the x planes holding an active face of any component (all of
-    // them when windows are off or no face is active)
+    // x and y windows: the x and y planes holding an active face of any
+    // component (all of them when windows are off or no face is active)
     int lo = n[0], hi = 0, ylo = n[1], yhi = 0;
     for (int f = 0; f < 3; ++f)
         for (int i = 0; i < n[0]; ++i)
@@ -691,10 +691,11 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
 // Length of the short-period x pass for a window of w planes: the
 // shortest length >= w below Nx with a register-resident x pass in both
 // precisions, else the shortest compiled one below Nx, else none (also with
-// windows off).  At 128^3 with w = 48 (MI355X, per launch): 64 register
-// 60 us FP32 / 105 us FP64, 48 LDS-staged 83 / 153 us, 48 register (4-stage
-// plan) 67 / 155 us, the full 128 84 / 196 us.  VIE_XSHORT=0 turns it off,
-// VIE_XSHORT=m asks for length m.
+// windows off).  At 128^3 with w = 48 (MI355X, per launch, FP32 / FP64):
+// the 64-length register pass (60 / 105 us) beat the 48-length LDS-staged
+// (83 / 153 us) and register (4-stage plan, 67 / 155 us) ones and the full
+// 128 (84 / 196 us).  VIE_XSHORT=0 turns it off, VIE_XSHORT=m asks for
+// length m.
 int short_len(const vie_ctx* c, int w)
 {
     const int nx = c->desc.n[0];
