@@ -11,7 +11,7 @@ oracle's, which works on the caller's grid with every plane:
   far-field, absorbed-power and gradient outputs, and the mixed solve;
 * the x pass on a shorter period than the grid's (register-resident and
   LDS-staged pencils);
-* boxes limited in two or three axes (x and y windows);
+* boxes limited in two or three axes (x and y windows, short x and y periods);
 * an embedded grid (lengths without compiled kernels) with a slab;
 * the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
 """
@@ -99,7 +99,8 @@ def test_plan_puts_the_slab_axis_first(axis, lo, hi, order, pencil):
             p = s.plan()
     finally:
         os.environ.pop("VIE_WINDOW", None)
-    assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N, "y_window": (0, N[1])}
+    assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N, "y_window": (0, N[1]),
+                 "y_pencil": N[1]}
 
 
 @pytest.mark.parametrize("axis,lo,hi", SLABS)
@@ -259,6 +260,29 @@ def test_box_converged_and_mixed_match_oracle(lo, hi, order, xw, yw):
         m = s.solve_mixed(tol=1e-9)
         assert m["converged"] and m["true_relres"] < 1e-9
         assert rel(s.solution(), ref["x"]) <= 1e-7
+
+
+@pytest.mark.parametrize("n,lo,hi,ypen", [((16, 64, 12), (4, 20, 0), (8, 45, 12), 32),
+                                          ((12, 128, 10), (3, 30, 0), (7, 90, 10), 64)])
+def test_short_y_period_matches_oracle(n, lo, hi, ypen):
+    """Boxes whose second-narrowest axis spans 25 of 64 and 60 of 128 planes:
+    BiCGSTAB products run their y passes on register-resident 32- and
+    64-length pencils (the y window shifted to entry 0)."""
+    mat, eps, flags = box_scene(lo, hi, 19, n)
+    b = O.incident(n, D, K0, 30.0, 40.0, 50.0) * flags
+    op = oracle(n, eps, flags)
+    ref = op.solve(b, 3, tol=1e-300, maxit=8)
+    with solver(n, mat, flags) as s:
+        p = s.plan()
+        assert p["axis_order"] == (0, 1, 2) and p["y_pencil"] == ypen
+        x, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+        assert rel(x, ref["x"]) <= 1e-9, rel(x, ref["x"])
+        s.set_rhs(b)
+        m = s.solve_mixed(tol=1e-9)
+        assert m["converged"] and m["true_relres"] < 1e-9
+    with solver(n, mat, flags, VF.FP32) as s:
+        x32, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+    assert rel(x32, ref["x"]) <= 1e-4, rel(x32, ref["x"])
 
 
 def test_embedded_slab_matches_oracle():

@@ -132,16 +132,16 @@ template <class R> struct Op {
     Geo g{};
     int n[3];
     double d[3], k0;
-    // per-axis tables, slots 0-2 the grid's axes; slot 3 the x axis of the
-    // short-period x pass (length nxs)
-    T* tw[4] = {};
-    T* twist[4] = {};
-    T* twP[4] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
-    T* U[4] = {};
-    T* Vv[4] = {};
-    T* Ua[4] = {};
-    T* Va[4] = {};
-    int* fold[4] = {};
+    // per-axis tables, slots 0-2 the grid's axes; slots 3 and 4 the x and y
+    // axes of the short periods (lengths nxs, nys)
+    T* tw[5] = {};
+    T* twist[5] = {};
+    T* twP[5] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
+    T* U[5] = {};
+    T* Vv[5] = {};
+    T* Ua[5] = {};
+    T* Va[5] = {};
+    int* fold[5] = {};
     T* G = nullptr;
     T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
@@ -154,13 +154,17 @@ template <class R> struct Op {
     // nxs >= w reproduces (the window shifted to plane 0).  nxs = 0: the
     // x pass runs at the grid's length.  Gs is the G~ octant of the
     // (nxs, Ny, Nz) grid scaled by Nx / nxs (the tables keep the grid's
-    // normalisation 1/(8 Nx Ny Nz)).
-    int nxs = 0;
+    // normalisation 1/(8 Nx Ny Nz)).  Products whose outputs the solver keeps
+    // zero (wskip) also use the y window [wy0, wy1): with nys > 0 their y
+    // passes and W2 have the short y period nys >= wy1 - wy0 and their x pass
+    // the octant Gy of the (nxs or Nx, nys, Nz) grid, scaled likewise.
+    int nxs = 0, nys = 0;
     T* Gs = nullptr;
+    T* Gy = nullptr;
     T* W1 = nullptr;
     T* W2 = nullptr;
     bool own_w = true;
-    MixTabs<R> mf{}, ma{}, mfs{}, mas{};
+    MixTabs<R> mf{}, ma{}, mfs{}, mas{}, mfy{}, may{};
 
     // Length-N tables of slot a: LDS-pass twiddles, the register passes'
     // twP, U, the adjoint's V and the G~ fold (the k0-dependent Vv and Ua
@@ -230,6 +234,7 @@ template <class R> struct Op {
         g.x1 = wx1 = n[0];
         g.y0 = wy0 = 0;
         g.y1 = wy1 = n[1];
+        g.nyp = n[1];
         for (int a = 0; a < 3; ++a) build_axis(a, n[a], st);
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
@@ -257,31 +262,58 @@ template <class R> struct Op {
         set_k0(kk, st);
     }
 
-    // Short-period x pass of length ns (0: none): slot-3 tables, Gs (filled
-    // by the caller from green_octant) and the MixTabs that use them.
-    void set_short(int ns, hipStream_t st)
+    // Short periods of lengths ns (x) and nsy (y), 0: none: slot-3/4 tables,
+    // the octants Gs and Gy (filled by the caller from green_octant) and the
+    // MixTabs that use them.
+    void set_short(int ns, int nsy, hipStream_t st)
     {
-        if (ns == nxs) return;
-        free_axis(3);
+        if (ns == nxs && nsy == nys) return;
+        if (ns != nxs) {
+            free_axis(3);
+            nxs = ns;
+            if (ns) {
+                build_axis(3, ns, st);
+                axis_k0(3, ns, d[0], st);
+            }
+        }
+        if (nsy != nys) {
+            free_axis(4);
+            nys = nsy;
+            if (nsy) {
+                build_axis(4, nsy, st);
+                axis_k0(4, nsy, d[1], st);
+            }
+        }
         dfree(Gs);
-        nxs = ns;
-        if (!ns) return;
-        build_axis(3, ns, st);
-        Gs = dalloc<T>((long)(ns + 1) * (n[1] + 1) * (n[2] + 1));
-        axis_k0(3, ns, d[0], st);
+        dfree(Gy);
+        if (nxs) Gs = dalloc<T>((long)(nxs + 1) * (n[1] + 1) * (n[2] + 1));
+        if (nys) Gy = dalloc<T>((long)((nxs ? nxs : n[0]) + 1) * (nys + 1) * (n[2] + 1));
         short_tabs();
     }
     void short_tabs()
     {
         mfs = mf;
         mas = ma;
-        mfs.U[0] = U[3];
-        mfs.Vv[0] = Vv[3];
-        mfs.fold[0] = fold[3];
-        mas.U[0] = Ua[3];
-        mas.Vv[0] = Va[3];
-        mas.fold[0] = fold[3];
-        mfs.G = mas.G = Gs;
+        if (nxs) {
+            mfs.U[0] = U[3];
+            mfs.Vv[0] = Vv[3];
+            mfs.fold[0] = fold[3];
+            mas.U[0] = Ua[3];
+            mas.Vv[0] = Va[3];
+            mas.fold[0] = fold[3];
+            mfs.G = mas.G = Gs;
+        }
+        mfy = mfs;
+        may = mas;
+        if (nys) {
+            mfy.U[1] = U[4];
+            mfy.Vv[1] = Vv[4];
+            mfy.fold[1] = fold[4];
+            may.U[1] = Ua[4];
+            may.Vv[1] = Va[4];
+            may.fold[1] = fold[4];
+            mfy.G = may.G = Gy;
+        }
     }
 
     // v_s = (j Z0/(k0 delta_s NP)) (1 - e^{-j theta_s}) of slot a (its
@@ -319,16 +351,16 @@ template <class R> struct Op {
             ma.D[a][0] = (R)D.real();
             ma.D[a][1] = (R)-D.imag();
         }
-        if (nxs) {
-            axis_k0(3, nxs, d[0], st);
-            short_tabs();
-        }
+        if (nxs) axis_k0(3, nxs, d[0], st);
+        if (nys) axis_k0(4, nys, d[1], st);
+        short_tabs();
     }
     ~Op()
     {
-        for (int a = 0; a < 4; ++a) free_axis(a);
+        for (int a = 0; a < 5; ++a) free_axis(a);
         dfree(G);
         dfree(Gs);
+        dfree(Gy);
         dfree(dg);
         dfree(flags);
         dfree(zr);
@@ -371,8 +403,14 @@ template <class R> struct Op {
         t0(1);
         HIPCHK(run_zfwd<R>(gw, lin, W1, tw[2], twist[2], twP[2], done, st));
         t0(1);
+        const bool ysh = lin.zr && lin.wskip && nys;  // the y window's pencils on the short y period
+        const int ay = ysh ? 4 : 1;
+        if (ysh) {
+            gw.nyp = nys;
+            gw.ys = wy0;
+        }
         t0(2);
-        HIPCHK(run_yfwd<R>(gw, W1, W2, tw[1], twist[1], twP[1], done, st));
+        HIPCHK(run_yfwd<R>(gw, W1, W2, tw[ay], twist[ay], twP[ay], done, st));
         t0(2);
         Geo gx = gw;
         int ax = 0, nxp = g.nx;
@@ -381,12 +419,12 @@ template <class R> struct Op {
             ax = 3;
             nxp = nxs;
         }
-        const MixTabs<R>& mt = ax ? (adj ? mas : mfs) : (adj ? ma : mf);
+        const MixTabs<R>& mt = ysh ? (adj ? may : mfy) : ax ? (adj ? mas : mfs) : (adj ? ma : mf);
         t0(3);
         HIPCHK(run_xmix<R>(gx, nxp, W2, tw[ax], twist[ax], twP[ax], mt, done, st));
         t0(3);
         t0(4);
-        HIPCHK(run_yinv<R>(gw, W2, W1, tw[1], twist[1], twP[1], done, st));
+        HIPCHK(run_yinv<R>(gw, W2, W1, tw[ay], twist[ay], twP[ay], done, st));
         t0(4);
         Diag<R> dd{};
         dd.d = dg;
@@ -696,48 +734,62 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
 // (83 / 153 us) and register (4-stage plan, 67 / 155 us) ones and the full
 // 128 (84 / 196 us).  VIE_XSHORT=0 turns it off, VIE_XSHORT=m asks for
 // length m.
-int short_len(const vie_ctx* c, int w)
+int short_len(const vie_ctx* c, int w, int axis)
 {
-    const int nx = c->desc.n[0];
-    if (!c->windows || w >= nx) return 0;
-    if (const char* e = std::getenv("VIE_XSHORT")) {
+    const int nn = c->desc.n[axis];
+    if (!c->windows || w >= nn) return 0;
+    if (const char* e = std::getenv(axis == 0 ? "VIE_XSHORT" : "VIE_YSHORT")) {
         const int m = std::atoi(e);
-        return (m >= w && m < nx && size_supported(m)) ? m : 0;
+        return (m >= w && m < nn && size_supported(m)) ? m : 0;
     }
-    for (int m = w; m < nx; ++m)
-        if (size_supported(m) && x_register<double>(m) && x_register<float>(m)) return m;
-    for (int m = w; m < nx; ++m)
+    auto reg = [axis](int m) {
+        return axis == 0 ? x_register<double>(m) && x_register<float>(m) : y_register<double>(m) && y_register<float>(m);
+    };
+    for (int m = w; m < nn; ++m)
+        if (size_supported(m) && reg(m)) return m;
+    for (int m = w; m < nn; ++m)
         if (size_supported(m)) return m;
     return 0;
 }
 
-// Short-period x pass of every operator for the current window, with its
-// G~ octant (scaled by Nx / nxs) once G~ is computed.
+// G~ octant of the (px, py, Nz) grid, scaled by (Nx Ny) / (px py), into
+// the FP64 staging buffer Gshort and the operators' G64 / G32 copies.
+void short_octant(vie_ctx* c, int px, int py, double2* g64, float2* g32)
+{
+    const int* n = c->desc.n;
+    const int n3[3] = {px, py, n[2]};
+    const long tot = (long)(px + 1) * (py + 1) * (n[2] + 1);
+    if (!c->Gshort) c->Gshort = dalloc<double2>((long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1));
+    green_octant(c, n3, c->Gshort);
+    hipLaunchKernelGGL(k_scale_c64, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                       (double)n[0] * n[1] / ((double)px * py), c->Gshort);
+    HIPCHK(hipGetLastError());
+    if (g64) HIPCHK(hipMemcpyAsync(g64, c->Gshort, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    if (g32) {
+        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
+                           c->Gshort, g32);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// Short periods of every operator for the current windows, with their G~
+// octants once G~ is computed.
 void short_sync(vie_ctx* c)
 {
     Op<double>* o64 = c->op64;
     Op<float>* o32 = c->op32;
     if (!o64 && !o32) return;
-    const int w = o64 ? o64->wx1 - o64->wx0 : o32->wx1 - o32->wx0;
-    const int ns = c->medium_ready ? short_len(c, w) : 0;
-    if (o64) o64->set_short(ns, c->stream);
-    if (o32) o32->set_short(ns, c->stream);
-    if (!ns || !c->green_ready) return;
+    const int wx = o64 ? o64->wx1 - o64->wx0 : o32->wx1 - o32->wx0;
+    const int wy = o64 ? o64->wy1 - o64->wy0 : o32->wy1 - o32->wy0;
+    const int ns = c->medium_ready ? short_len(c, wx, 0) : 0;
+    const int nsy = c->medium_ready ? short_len(c, wy, 1) : 0;
+    if (o64) o64->set_short(ns, nsy, c->stream);
+    if (o32) o32->set_short(ns, nsy, c->stream);
+    if (!c->green_ready) return;
     const int* n = c->desc.n;
-    const int n3[3] = {ns, n[1], n[2]};
-    const long tot = (long)(ns + 1) * (n[1] + 1) * (n[2] + 1);
-    if (!c->Gshort) c->Gshort = dalloc<double2>((long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1));
-    green_octant(c, n3, c->Gshort);
-    hipLaunchKernelGGL(k_scale_c64, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
-                       (double)n[0] / ns, c->Gshort);
-    HIPCHK(hipGetLastError());
-    if (o64) HIPCHK(hipMemcpyAsync(o64->Gs, c->Gshort, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
-    if (o32) {
-        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
-                           c->Gshort, o32->Gs);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ns) short_octant(c, ns, n[1], o64 ? o64->Gs : nullptr, o32 ? o32->Gs : nullptr);
+    if (nsy) short_octant(c, ns ? ns : n[0], nsy, o64 ? o64->Gy : nullptr, o32 ? o32->Gy : nullptr);
 }
 
 template <> Op<double>* make_op<double>(vie_ctx* c)
@@ -2398,7 +2450,7 @@ int vie_kernel_time(vie_ctx* c, int* launches, double* mean_ms)
 double vie_model_bytes_matvec(const vie_ctx* c)
 {
     if (!c) return 0.0;
-    int p[11];
+    int p[12];
     if (vie_plan(c, p) != VIE_OK) return 0.0;
     const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
     const double Vw = (double)(p[4] - p[3]) * p[7] * p[8];
@@ -2410,19 +2462,21 @@ int vie_plan(const vie_ctx* c, int* out)
 {
     if (!c || !out) return VIE_ERR_ARG;
     const int* n = c->desc.n;
-    int x0 = 0, x1 = n[0], y0 = 0, y1 = n[1], ns = 0;
+    int x0 = 0, x1 = n[0], y0 = 0, y1 = n[1], ns = 0, nsy = 0;
     if (c->op64) {
         x0 = c->op64->wx0;
         x1 = c->op64->wx1;
         y0 = c->op64->wy0;
         y1 = c->op64->wy1;
         ns = c->op64->nxs;
+        nsy = c->op64->nys;
     } else if (c->op32) {
         x0 = c->op32->wx0;
         x1 = c->op32->wx1;
         y0 = c->op32->wy0;
         y1 = c->op32->wy1;
         ns = c->op32->nxs;
+        nsy = c->op32->nys;
     }
     for (int a = 0; a < 3; ++a) {
         out[a] = c->perm[a];
@@ -2433,6 +2487,7 @@ int vie_plan(const vie_ctx* c, int* out)
     out[5] = ns ? ns : n[0];
     out[9] = y0;
     out[10] = y1;
+    out[11] = nsy ? nsy : n[1];
     return VIE_OK;
 }
 

@@ -24,8 +24,9 @@ hipError_t run_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx<R>* tw, c
 template <class R>
 hipError_t run_xmix(const Geo& g, int nxp, cx<R>* W2, const cx<R>* tw, const cx<R>* twist, const cx<R>* twP,
                     const MixTabs<R>& mt, const int* done, hipStream_t st);
-// whether length n has a register-resident x pass (else the LDS-staged one)
+// whether length n has a register-resident x (y) pass (else the LDS-staged one)
 template <class R> bool x_register(int n);
+template <class R> bool y_register(int n);
 template <class R>
 hipError_t run_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                     const cx<R>* twP, const int* done, hipStream_t st);

@@ -172,7 +172,7 @@ template <>
 hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>* tw,
                         const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
 {
-    switch (g.ny) {
+    switch (g.nyp) {
 #define CASE(n) \
     case n: return launch_yfwd<R_, n>(g, W1, W2, tw, twist, twP, done, st);
         VIE_SIZES(CASE)
@@ -238,7 +238,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
             using X = XReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nz + B - 1) / B;
-            dim3 grid((unsigned)(2L * g.ny * nzt));
+            dim3 grid((unsigned)(2L * g.nyp * nzt));
             hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP>), grid, dim3(NT), 0, st, g, W2, twP, mt,
                                done);
             return hipGetLastError();
@@ -246,9 +246,22 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
     }
     constexpr int B = b_x<R>(n);
     const long nzt = (2L * g.nz + B - 1) / B;
-    dim3 grid((unsigned)(2L * g.ny * nzt));
+    dim3 grid((unsigned)(2L * g.nyp * nzt));
     hipLaunchKernelGGL((k_xmix<R, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done);
     return hipGetLastError();
+}
+
+template <> bool y_register<R_>(int n)
+{
+    if (yz_lds_forced()) return false;
+    switch (n) {
+#define CASE(m) \
+    case m: return YZReg<R_, m>::ok;
+        VIE_SIZES(CASE)
+#undef CASE
+    default:
+        return false;
+    }
 }
 
 template <> bool x_register<R_>(int n)
@@ -282,7 +295,7 @@ template <>
 hipError_t run_yinv<R_>(const Geo& g, const cx<R_>* W2, cx<R_>* W1, const cx<R_>* tw,
                         const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
 {
-    switch (g.ny) {
+    switch (g.nyp) {
 #define CASE(n) \
     case n: return launch_yinv<R_, n>(g, W2, W1, tw, twist, twP, done, st);
         VIE_SIZES(CASE)

@@ -20,13 +20,16 @@ namespace vie {
 // those planes nor its x-pencil entries there.  [y0, y1) (0 <= y0 < y1 <= ny)
 // likewise bounds the z rows (x, y) the z passes transform and the y entries
 // the y passes load and store.  The x pass's pencil entry n is plane n + xs
-// (xs = x0 when the pencils have the short period).
+// (xs = x0 when the pencils have the short period).  W2 holds nyp y pencils
+// of length 2 nyp per (x, kz): nyp = ny, or the short y period, whose entry
+// n is y plane n + ys (ys = y0).
 struct Geo {
     int nx, ny, nz;
     long V;
     int x0 = 0, x1 = 0;
     int y0 = 0, y1 = 0;
     int xs = 0;
+    int nyp = 0, ys = 0;
 };
 
 // z row of window row r, r < (x1 - x0)(y1 - y0): x0 + r / wy, y0 + r % wy
@@ -302,12 +305,12 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
         tws[t] = twistg[t];
     }
     __syncthreads();
-    const long in0 = ((long)s * g.nx + i) * N * P2 + z0;
+    const long in0 = ((long)s * g.nx + i) * g.ny * P2 + z0;
     // column pairs (col, col+1): P2 and z0 are even, so a pair is in or out together
     for (int e = threadIdx.x; e < N * B / 2; e += NT) {
-        const int j = e / (B / 2), col = 2 * (e % (B / 2));
+        const int j = e / (B / 2), col = 2 * (e % (B / 2)), pl = j + g.ys;  // entry j: y plane pl
         T v0 = czero<T>(), v1 = czero<T>();
-        if (z0 + col < P2 && j >= g.y0 && j < g.y1) ld2(W1 + in0 + (long)j * P2 + col, v0, v1);
+        if (z0 + col < P2 && pl >= g.y0 && pl < g.y1) ld2(W1 + in0 + (long)pl * P2 + col, v0, v1);
         buf[j * LD + col] = v0;
         buf[j * LD + col + 1] = v1;
         buf[j * LD + B + col] = cmul(v0, tws[j]);
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     __shared__ T vz[B];
     __shared__ int fz[B];
     if (done && *done) return;
-    const int P1 = 2 * g.ny, P2 = 2 * g.nz;
+    const int P1 = 2 * g.nyp, P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     }
     const T uy = mt.U[1][ty], vy = mt.Vv[1][ty];
     const long fyoff = (long)mt.fold[1][ty] * (g.nz + 1);
-    const long gs = (long)(g.ny + 1) * (g.nz + 1);
+    const long gs = (long)(g.nyp + 1) * (g.nz + 1);
     __syncthreads();
     // x-pencils of the three components: rows i < Nx, columns z0..z0+B
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
@@ -444,11 +447,11 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
-    const long out0 = ((long)s * g.nx + i) * N * P2 + z0;
+    const long out0 = ((long)s * g.nx + i) * g.ny * P2 + z0;
     for (int e = threadIdx.x; e < N * B / 2; e += NT) {
-        const int j = e / (B / 2), col = 2 * (e % (B / 2));
-        if (z0 + col < P2 && j >= g.y0 && j < g.y1)
-            st2(W1 + out0 + (long)j * P2 + col, cadd(buf[j * LD + col], cmulc(buf[j * LD + B + col], tws[j])),
+        const int j = e / (B / 2), col = 2 * (e % (B / 2)), pl = j + g.ys;  // entry j: y plane pl
+        if (z0 + col < P2 && pl >= g.y0 && pl < g.y1)
+            st2(W1 + out0 + (long)pl * P2 + col, cadd(buf[j * LD + col], cmulc(buf[j * LD + B + col], tws[j])),
                 cadd(buf[j * LD + col + 1], cmulc(buf[j * LD + B + col + 1], tws[j])));
     }
 }

@@ -258,7 +258,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
     __shared__ T_ gsl[GSL ? (N + 1) * B : 1];    // G octant slice [fx][col]
     if (done && *done) return;
-    const int P1 = 2 * g.ny, P2 = 2 * g.nz;
+    const int P1 = 2 * g.nyp, P2 = 2 * g.nz;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
@@ -286,8 +286,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             }
         }
     if constexpr (GSL) {
-        const long gs = (long)(g.ny + 1) * (g.nz + 1);
-        const long gy = (long)fold_split(ty, g.ny) * (g.nz + 1);
+        const long gs = (long)(g.nyp + 1) * (g.nz + 1);
+        const long gy = (long)fold_split(ty, g.nyp) * (g.nz + 1);
         for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
             const int fxe = e / B, ce = e % B;
             const int tze = z0 + ce < P2 ? z0 + ce : z0;
@@ -305,8 +305,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const int tz = colok ? z0 + col : z0;
     const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
     // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
-    const T_* gcol = mt.G + (long)fold_split(ty, g.ny) * (g.nz + 1) + fold_split(tz, g.nz);
-    const unsigned gs = (unsigned)(g.ny + 1) * (g.nz + 1);
+    const T_* gcol = mt.G + (long)fold_split(ty, g.nyp) * (g.nz + 1) + fold_split(tz, g.nz);
+    const unsigned gs = (unsigned)(g.nyp + 1) * (g.nz + 1);
     auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
         const int kf = kx < P - kx ? kx : P - kx;
         return (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
@@ -593,7 +593,8 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
     const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
     const int col = threadIdx.x % B, t = threadIdx.x / B;
     const bool colok = z0 + col < P2;
-    const T_* in = W1 + ((long)s * g.nx + i) * N * P2 + z0 + col;
+    // pencil entry n is y plane n + ys of W1 (Ny planes)
+    const T_* in = W1 + (((long)s * g.nx + i) * g.ny + g.ys) * P2 + z0 + col;
     T_ v[E];
 #pragma unroll
     for (int b = 0; b < E / r1; ++b) {
@@ -601,7 +602,8 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
         for (int q = 0; q < r1; ++q) {
             const int n = ii + q * (P / r1);
-            v[b * r1 + q] = (n < N && colok && n >= g.y0 && n < g.y1) ? in[(long)n * P2] : czero<T_>();
+            v[b * r1 + q] = (n < N && colok && n + g.ys >= g.y0 && n + g.ys < g.y1) ? in[(long)n * P2]
+                                                                                    : czero<T_>();
         }
     }
     for (int e = threadIdx.x; e < P; e += NT) tw[e] = twP[e];
@@ -647,14 +649,15 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
     __syncthreads();
     r_inverse<T_, C, PL>(v, lds, tw, t, col);
     if (colok) {
-        T_* out = W1 + ((long)s * g.nx + i) * N * P2 + z0 + col;
+        // pencil entry n is y plane n + ys of W1 (Ny planes)
+        T_* out = W1 + (((long)s * g.nx + i) * g.ny + g.ys) * P2 + z0 + col;
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {
             const int ii = t + T * b;
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = ii + q * (P / r1);
-                if (n < N && n >= g.y0 && n < g.y1) out[(long)n * P2] = v[b * r1 + q];
+                if (n < N && n + g.ys >= g.y0 && n + g.ys < g.y1) out[(long)n * P2] = v[b * r1 + q];
             }
         }
     }

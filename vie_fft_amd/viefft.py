@@ -322,11 +322,11 @@ class Solver:
     def plan(self):
         """Internal plan of the resident medium: axis order, x window of the
         products inside solves, pencil length of their x pass, internal grid,
-        y window of the BiCGSTAB products."""
-        a = (ct.c_int * 11)()
+        y window and y pencil length of the BiCGSTAB products."""
+        a = (ct.c_int * 12)()
         self._check(self.lib.vie_plan(self.h, a))
         return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9]),
-                "y_window": (a[9], a[10])}
+                "y_window": (a[9], a[10]), "y_pencil": a[11]}
 
 
 def supported_n(n):
