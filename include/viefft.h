@@ -231,10 +231,9 @@ double vie_model_bytes_matvec(const vie_ctx* ctx);
    caller's axis out[a]); out[3], out[4] the x window [x0, x1) of the products
    inside solves, on the internal grid; out[5] the pencil length of their x
    pass (the short period, or the internal Nx); out[6..8] the internal grid;
-   out[9], out[10] the y window [y0, y1) of the products whose outputs the
-   solver keeps zero on inactive faces (BiCGSTAB); out[11] the y pencil
-   length of those products (the short y period, or the internal Ny).  out
-   holds 12 ints. */
+   out[9], out[10] the y window [y0, y1) of the products inside solves;
+   out[11] their y pencil length (the short y period, or the internal Ny).
+   out holds 12 ints. */
 int vie_plan(const vie_ctx* ctx, int* out);
 
 #ifdef __cplusplus

@@ -454,6 +454,16 @@ template <class R> Lin<R> lin_k(const uint32_t* zr, const cx<R>* x)
     return l;
 }
 
+// The same for a solver that zeroed its vectors at the start and writes only
+// active faces from then on: the product leaves inactive output entries
+// unwritten and uses the y window and short periods too.
+template <class R> Lin<R> lin_kw(const uint32_t* zr, const cx<R>* x)
+{
+    Lin<R> l = lin_k<R>(zr, x);
+    l.wskip = 1;
+    return l;
+}
+
 template <class R> Dots<R> nodots()
 {
     Dots<R> d{};
@@ -928,25 +938,34 @@ template <class R> KState solve_cg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>*
     cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP);
     const int* done = &st->done;
     state_init(c, st, tol, maxit);
+    // every vector starts at zero and is written only on active faces from here
+    // on (wskip): products and updates skip the rest and run over the windows
+    fill0<R>(c, x, n);
+    fill0<R>(c, r, n);
+    fill0<R>(c, p, n);
+    fill0<R>(c, q, n);
+    fill0<R>(c, Ap, n);
     {   // x = 0 ; r = -b
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = b;
         L.hc2[0][0] = -1.0;
         L.out = x;
         L.out2 = r;
         lincomb<R>(c, L, done);
         // p = A^H b ; ||p||^2, ||b||^2
-        long nb = op->apply(lin_k<R>(op->zr, b), b, p, dots_with<R>(c, 2, {0, 0, 1, 1}, b, nullptr), true, done, c->stream,
+        long nb = op->apply(lin_kw<R>(op->zr, b), b, p, dots_with<R>(c, 2, {0, 0, 1, 1}, b, nullptr), true, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CG_INIT, nb, 2);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin_k<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_kw<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CG_ALPHA, nb, 1);
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = x; L.x[1] = p; L.x[2] = r; L.x[3] = Ap;
         L.coef = &st->cf[CF_CG_U];
         L.out = x;
@@ -954,11 +973,12 @@ template <class R> KState solve_cg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>*
         L.dots = dots_with<R>(c, 1, {3, 3}, nullptr, nullptr);
         nb = lincomb<R>(c, L, done);
         scalar(c, st, SC_CG_V4, nb, 1);
-        nb = op->apply(lin_k<R>(op->zr, r), r, q, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), true, done, c->stream,
+        nb = op->apply(lin_kw<R>(op->zr, r), r, q, dots_with<R>(c, 1, {0, 0}, nullptr, nullptr), true, done, c->stream,
                        &c->timer);
         scalar(c, st, SC_CG_BETA, nb, 1);
         LinComb<R> P{};
         P.n = n;
+        P.wskip = 1;
         P.x[0] = p; P.x[1] = q;
         P.coef = &st->cf[CF_CG_P];
         P.out = p;
@@ -975,9 +995,16 @@ template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
     cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *Ap = vec<R>(c, VAP);
     const int* done = &st->done;
     state_init(c, st, tol, maxit);
+    // every vector starts at zero and is written only on active faces from here
+    // on (wskip): products and updates skip the rest and run over the windows
+    fill0<R>(c, x, n);
+    fill0<R>(c, r, n);
+    fill0<R>(c, p, n);
+    fill0<R>(c, Ap, n);
     {
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = b;
         L.hc2[0][0] = 1.0;
         L.out = x;
@@ -987,6 +1014,7 @@ template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         scalar(c, st, SC_BICG_INIT, nb, 1);
         LinComb<R> S{};  // p = r / (r^T r)
         S.n = n;
+        S.wskip = 1;
         S.x[0] = r;
         S.coef = &st->cf[CF_BI_S];
         S.out = p;
@@ -994,11 +1022,12 @@ template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
     }
     auto iter = [&]() {
         // Ap ; Ap^T p (unconjugated)
-        long nb = op->apply(lin_k<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 1}, p, nullptr, {1}), false, done, c->stream,
+        long nb = op->apply(lin_kw<R>(op->zr, p), p, Ap, dots_with<R>(c, 1, {0, 1}, p, nullptr, {1}), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_BICG_ALPHA, nb, 1);
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = x; L.x[1] = p; L.x[2] = r; L.x[3] = Ap;
         L.coef = &st->cf[CF_BI_U];
         L.out = x;
@@ -1008,6 +1037,7 @@ template <class R> KState solve_bicg(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         scalar(c, st, SC_BICG_END, nb, 1);
         LinComb<R> P{};
         P.n = n;
+        P.wskip = 1;
         P.x[0] = p; P.x[1] = r;
         P.coef = &st->cf[CF_BI_P];
         P.out = p;
@@ -1025,12 +1055,21 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
     cx<R>*rc = vec<R>(c, VRC), *qc = vec<R>(c, VQC), *pc = vec<R>(c, VPC);
     const int* done = &st->done;
     state_init(c, st, tol, maxit);
+    // every vector starts at zero and is written only on active faces from here
+    // on (wskip): products and updates skip the rest and run over the windows
+    fill0<R>(c, x, n);
+    fill0<R>(c, r, n);
+    fill0<R>(c, p, n);
     fill0<R>(c, q, n);
+    fill0<R>(c, Ap, n);
+    fill0<R>(c, Aq, n);
     fill0<R>(c, rc, n);
+    fill0<R>(c, qc, n);
     fill0<R>(c, pc, n);
     {
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = b;
         L.hc2[0][0] = 1.0;
         L.out = x;
@@ -1040,11 +1079,12 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         scalar(c, st, SC_BNRM, nb, 1);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin_k<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_kw<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_CORS_RHO, nb, 1);
         LinComb<R> L1{};  // p = r + b pc ; qc = Ap + b rc
         L1.n = n;
+        L1.wskip = 1;
         L1.x[0] = r; L1.x[1] = pc; L1.x[2] = Ap; L1.x[3] = rc;
         L1.coef = &st->cf[CF_CO_L1];
         L1.out = p;
@@ -1052,6 +1092,7 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         lincomb<R>(c, L1, done);
         Lin<R> lq{};  // q = qc + b rc + b^2 q, then Aq
         lq.zr = op->zr;
+        lq.wskip = 1;
         lq.x[0] = qc; lq.x[1] = rc; lq.x[2] = q;
         lq.coef = &st->cf[CF_CO_Q];
         lq.out = q;
@@ -1059,6 +1100,7 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         scalar(c, st, SC_CORS_ALPHA, nb, 1);
         LinComb<R> L3{};  // pc = p - a q ; rc = qc - a Aq
         L3.n = n;
+        L3.wskip = 1;
         L3.x[0] = p; L3.x[1] = q; L3.x[2] = qc; L3.x[3] = Aq;
         L3.coef = &st->cf[CF_CO_L3];
         L3.out = pc;
@@ -1066,12 +1108,14 @@ template <class R> KState solve_cors(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R
         lincomb<R>(c, L3, done);
         LinComb<R> LX{};  // x += 2a p - a^2 q
         LX.n = n;
+        LX.wskip = 1;
         LX.x[0] = x; LX.x[1] = p; LX.x[2] = q;
         LX.coef = &st->cf[CF_CO_X];
         LX.out = x;
         lincomb<R>(c, LX, done);
         LinComb<R> LR{};  // r += -2a qc + a^2 Aq ; ||r||^2
         LR.n = n;
+        LR.wskip = 1;
         LR.x[0] = r; LR.x[1] = qc; LR.x[2] = Aq;
         LR.coef = &st->cf[CF_CO_R];
         LR.out = r;
@@ -1091,12 +1135,20 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
     cx<R>*qc = vec<R>(c, VQC), *pc = vec<R>(c, VPC);
     const int* done = &st->done;
     state_init(c, st, tol, maxit);
+    // every vector starts at zero and is written only on active faces from here
+    // on (wskip): products and updates skip the rest and run over the windows
+    fill0<R>(c, x, n);
+    fill0<R>(c, r, n);
     fill0<R>(c, p, n);
     fill0<R>(c, q, n);
+    fill0<R>(c, Ap, n);
     fill0<R>(c, Aq, n);
+    fill0<R>(c, qc, n);
+    fill0<R>(c, pc, n);
     {
         LinComb<R> L{};
         L.n = n;
+        L.wskip = 1;
         L.x[0] = b;
         L.hc2[0][0] = 1.0;
         L.out = x;
@@ -1106,17 +1158,19 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
         scalar(c, st, SC_BNRM, nb, 1);
     }
     auto iter = [&]() {
-        long nb = op->apply(lin_k<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
+        long nb = op->apply(lin_kw<R>(op->zr, r), r, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream,
                             &c->timer);
         scalar(c, st, SC_BICOR_RHO, nb, 1);
         LinComb<R> L1{};  // p = r + b p - b w q
         L1.n = n;
+        L1.wskip = 1;
         L1.x[0] = r; L1.x[1] = p; L1.x[2] = q;
         L1.coef = &st->cf[CF_BC_P];
         L1.out = p;
         lincomb<R>(c, L1, done);
         Lin<R> lq{};  // q = Ap + b q - b w Aq, then Aq
         lq.zr = op->zr;
+        lq.wskip = 1;
         lq.x[0] = Ap; lq.x[1] = q; lq.x[2] = Aq;
         lq.coef = &st->cf[CF_BC_Q];
         lq.out = q;
@@ -1124,6 +1178,7 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
         scalar(c, st, SC_BICOR_ALPHA, nb, 1);
         LinComb<R> L2{};  // pc = r - a q ; qc = Ap - a Aq ; qc^H pc, qc^H qc
         L2.n = n;
+        L2.wskip = 1;
         L2.x[0] = r; L2.x[1] = q; L2.x[2] = Ap; L2.x[3] = Aq;
         L2.coef = &st->cf[CF_BC_L2];
         L2.out = pc;
@@ -1133,12 +1188,14 @@ template <class R> KState solve_bicorstab(vie_ctx* c, Op<R>* op, const cx<R>* b,
         scalar(c, st, SC_BICOR_OMEGA, nb, 2);
         LinComb<R> LX{};  // x += a p + w pc
         LX.n = n;
+        LX.wskip = 1;
         LX.x[0] = x; LX.x[1] = p; LX.x[2] = pc;
         LX.coef = &st->cf[CF_BC_X];
         LX.out = x;
         lincomb<R>(c, LX, done);
         LinComb<R> LR{};  // r = pc - w qc ; ||r||^2
         LR.n = n;
+        LR.wskip = 1;
         LR.x[0] = pc; LR.x[1] = qc;
         LR.coef = &st->cf[CF_BC_R];
         LR.out = r;

@@ -322,7 +322,7 @@ class Solver:
     def plan(self):
         """Internal plan of the resident medium: axis order, x window of the
         products inside solves, pencil length of their x pass, internal grid,
-        y window and y pencil length of the BiCGSTAB products."""
+        y window and y pencil length of the products inside solves."""
         a = (ct.c_int * 12)()
         self._check(self.lib.vie_plan(self.h, a))
         return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9]),
