@@ -243,8 +243,11 @@ __device__ __forceinline__ int fold_split(int t, int n)
 // allocation aims for (0: compiler default).  GM: how the G octant is read --
 // 0 per frequency from memory in the product, 1 a slice staged in LDS, 2 per
 // frequency into registers before the forward transform.  OPQ: per-component
-// lane coordinates (x_forward).
-template <class R, int N, int T, class PL, int WPE, int GM = (sizeof(R) == 8 ? 1 : 0), bool OPQ = false>
+// lane coordinates (x_forward).  SUP: only pencil entries n < SUP can hold
+// data (the caller's window lies within them), so the loads and stores of the
+// rest, and the butterflies that only feed them, drop out at compile time.
+template <class R, int N, int T, class PL, int WPE, int GM = (sizeof(R) == 8 ? 1 : 0), bool OPQ = false,
+          int SUP = N>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTabs<R> mt, const int* done)
 {
@@ -280,7 +283,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1);
-                v[c][b * r1 + q] = (n < N && colok && n + g.xs >= g.x0 && n + g.xs < g.x1)
+                v[c][b * r1 + q] = (n < SUP && colok && n + g.xs >= g.x0 && n + g.xs < g.x1)
                                        ? base[c * cst + (unsigned)n * rowst]
                                        : czero<T_>();
             }
@@ -356,7 +359,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
                 for (int q = 0; q < r1; ++q) {
                     const int n = i + q * (P / r1);
-                    if (n < N && n + g.xs >= g.x0 && n + g.xs < g.x1)
+                    if (n < SUP && n + g.xs >= g.x0 && n + g.xs < g.x1)
                         base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
                 }
             }
