@@ -41,13 +41,27 @@ XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil":
 # VALU instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, median launch):
 # profiles/k_xreg_f32_128cube_valu.md.  The pass is VALU-bound; MI355X issues at most
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles = 614.4e9 wave64 VALU instructions per second.
-XPASS_F32_128_VALU = 25878528
+XPASS_F32_128_VALU = 25059328
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
-# register-resident x-pass kernels by pencil length (FP32)
-XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false>", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,0,false>",
-            96: "k_xreg<float,96,16,XPlan<4,4,4,3>,0,2,true>", 128: "k_xreg<float,128,16,XPlan<16,16>,0,2,true>",
-            256: "k_xreg<float,256,32,XPlan<16,16,2>,0,0,true>"}
+# register-resident x-pass kernels by pencil length (FP32), without the last
+# template argument: the pencil entries that can hold data (the launcher picks
+# n/2 or 3n/4 when the window fits, else n)
+XREG_F32 = {32: "k_xreg<float,32,8,XPlan<8,8>,0,2,false", 64: "k_xreg<float,64,16,XPlan<8,4,4>,0,0,false",
+            96: "k_xreg<float,96,16,XPlan<4,4,4,3>,0,2,true", 128: "k_xreg<float,128,16,XPlan<16,16>,0,2,true",
+            256: "k_xreg<float,256,32,XPlan<16,16,2>,0,0,true"}
+
+
+def xreg_support(plan):
+    """Pencil entries the FP32 x pass's instantiation covers (launch_x)."""
+    n = plan["x_pencil"]
+    x0, x1 = plan["x_window"]
+    need = x1 - x0 if n < plan["grid"][0] else x1
+    if n % 4 == 0 and need <= n // 2:
+        return n // 2
+    if n % 4 == 0 and need <= 3 * n // 4:
+        return 3 * n // 4
+    return n
 
 
 def product_bytes(plan, s):
@@ -139,7 +153,7 @@ def main():
     xb = xpass_bytes(plan, 8.0)
     achieved = xb / (ms13 * 1e-3) if ms13 > 0 else 0.0
     nxp = plan["x_pencil"]
-    kernel = XREG_F32.get(nxp, f"k_xmix<float,{nxp},B>")
+    kernel = XREG_F32[nxp] + f",{xreg_support(plan)}>" if nxp in XREG_F32 else f"k_xmix<float,{nxp},B>"
     w = plan["x_window"][1] - plan["x_window"][0]
 
     out = {
