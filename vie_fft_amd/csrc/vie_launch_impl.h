@@ -44,17 +44,18 @@ static bool yz_lds_forced()
     return v;
 }
 
-// The z forward pass defaults to the LDS version: its loads of the three
-// Krylov vectors and the fused p update stream at near copy bandwidth there
-// (128^3: 52 us FP32 / 96 us FP64 against 63 / 100 us for the register
-// version).  VIE_ZFWD_REG=1 selects the register version.
-static bool zfwd_reg()
+// z forward pass: the LDS version for FP32, the register version for FP64.
+// Over the 48-plane window of the 128^3 layered workload the LDS version
+// takes 26.2 us FP32 / 46.8 us FP64 per launch, the register one 29.3 / 42.7
+// us (on every plane, before windows: 52 / 96 against 63 / 100 us).
+// VIE_ZFWD_REG=1 (=0) selects the register (LDS) version for both.
+template <class R> static bool zfwd_reg()
 {
-    static const bool v = [] {
+    static const int v = [] {
         const char* e = std::getenv("VIE_ZFWD_REG");
-        return e && e[0] == '1';
+        return e ? (e[0] == '1' ? 1 : 0) : -1;
     }();
-    return v;
+    return v < 0 ? sizeof(R) == 8 : v == 1;
 }
 
 template <class R, int n>
@@ -63,7 +64,7 @@ static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const 
 {
     const long rows = win_rows(g);  // the rows of the x and y windows
     if constexpr (YZReg<R, n>::ok) {
-        if (zfwd_reg() && !yz_lds_forced()) {
+        if (zfwd_reg<R>() && !yz_lds_forced()) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3);
