@@ -75,6 +75,25 @@ def product_bytes(plan, s):
     return 81.0 * vw * s + oct_ * s + 4.0 * vw
 
 
+def solve_models(plan, s, iters, outer=0):
+    """Solve-level algorithmic bytes of a BiCGSTAB solve (the survey's model):
+    per iteration 2 products + 16 fused 3-component vector streams; a mixed
+    solve adds, per residual replacement, one FP64 product and 12 V FP64
+    vector entries.  `dense` charges every cell of the grid (81 V s per
+    product, 48 V s of vectors); `window` charges the cells of the x window
+    that the products and updates actually cover."""
+    nx, ny, nz = plan["grid"]
+    V = nx * ny * nz
+    vw = (plan["x_window"][1] - plan["x_window"][0]) * ny * nz
+    oct_full = (nx + 1) * (ny + 1) * (nz + 1)
+    dense = iters * (2 * (81.0 * V * s + oct_full * s + 4.0 * V) + 48.0 * V * s)
+    window = iters * (2 * product_bytes(plan, s) + 48.0 * vw * s)
+    if outer:
+        dense += outer * (81.0 * V * 16 + oct_full * 16 + 4.0 * V + 12.0 * V * 16)
+        window += outer * (product_bytes(plan, 16.0) + 12.0 * V * 16)
+    return dense, window
+
+
 def xpass_bytes(plan, s):
     """x pass per launch: reads and writes the 3 components of the x window's
     planes of the padded (2Ny x 2Nz) spectrum in place, and reads the G~ octant
@@ -182,6 +201,7 @@ def main():
                  "xpass_ms": ms3_64, "xpass_frac": xpass_bytes(plan, 16.0) / (ms3_64 * 1e-3) / HBM_PEAK
                  if ms3_64 > 0 else None},
         "speedup_mixed_vs_fp64": t64 / tm,
+        "solve_roofline": solve_roofline(plan, tm, itm, outer, t64, it64),
         "plan": {"axis_order": list(plan["axis_order"]), "x_window": list(plan["x_window"]),
                  "y_window": list(plan["y_window"]), "y_pencil": plan["y_pencil"],
                  "x_pencil": nxp, "grid": list(plan["grid"]),
@@ -204,6 +224,18 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
     print(json.dumps(out))
     s.close()
+
+
+def solve_roofline(plan, tm, itm, outer, t64, it64):
+    """Whole-solve % of the HBM roofline: model bytes / solve time / 8 TB/s,
+    for the dense model (every cell; above 1 means faster than a dense solve
+    could be at peak bandwidth) and for the window the solve covers."""
+    dm, wm = solve_models(plan, 8.0, itm, outer)
+    d6, w6 = solve_models(plan, 16.0, it64)
+    return {"mixed": {"dense_bytes": dm, "dense_frac": dm / tm / HBM_PEAK,
+                      "window_bytes": wm, "window_frac": wm / tm / HBM_PEAK},
+            "fp64": {"dense_bytes": d6, "dense_frac": d6 / t64 / HBM_PEAK,
+                     "window_bytes": w6, "window_frac": w6 / t64 / HBM_PEAK}}
 
 
 def copy_rate(torch, n):
