@@ -195,7 +195,9 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // the product (148 registers) 264 us.  OP is on where it adds a wave per SIMD
 // (FP32 96: 211 -> 159 registers, 128); FP32 256 also reads G~ in the
 // product (240 -> 159 registers: 1578 against 1653 us at 256^3, where FP64
-// that way takes 3884 against 2983 us).  FP32 64 (the short x period of the
+// that way takes 3884 against 2983 us); T64 <8,8,8> without OP (124
+// registers, four waves, 15% fewer instructions per pencil) takes 2045
+// against 1513 us.  FP32 64 (the short x period of the
 // 128^3 layered workload, 48-plane window): G~ read in the product without
 // OP is 128 registers, four waves, and 407 VALU instructions per pencil
 // against 503 with OP: 56.9-58.0 us per launch against 60.8 us (G~
