@@ -76,7 +76,7 @@ def product_bytes(plan, s):
 
 
 def solve_models(plan, s, iters, outer=0):
-    """Solve-level algorithmic bytes of a BiCGSTAB solve (the survey's model):
+    """Solve-level algorithmic bytes of a BiCGSTAB solve (the path's byte model):
     per iteration 2 products + 16 fused 3-component vector streams; a mixed
     solve adds, per residual replacement, one FP64 product and 12 V FP64
     vector entries.  `dense` charges every cell of the grid (81 V s per
@@ -116,7 +116,7 @@ def main():
                     help="only time a plain device copy of the x pass's working set (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it "
+        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not part of it "
                                    "(independent wavelengths/polarisations shard with no communication)"}))
         sys.exit(2)
 
