@@ -37,7 +37,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # profiles/k_xreg_f32_128cube_hbm_traffic.md
 XPASS_F32_128_TRAFFIC = 168.5e6
 XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil": 64, "grid": (128, 128, 128),
-                      "y_window": (0, 128), "y_pencil": 128}
+                      "y_window": (0, 128), "y_pencil": 128, "z_window": (0, 128), "z_pencil": 128}
 # VALU instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, median launch):
 # profiles/k_xreg_f32_128cube_valu.md.  The pass is VALU-bound; MI355X issues at most
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles = 614.4e9 wave64 VALU instructions per second.
@@ -204,6 +204,7 @@ def main():
         "solve_roofline": solve_roofline(plan, tm, itm, outer, t64, it64),
         "plan": {"axis_order": list(plan["axis_order"]), "x_window": list(plan["x_window"]),
                  "y_window": list(plan["y_window"]), "y_pencil": plan["y_pencil"],
+                 "z_window": list(plan["z_window"]), "z_pencil": plan["z_pencil"],
                  "x_pencil": nxp, "grid": list(plan["grid"]),
                  "product_bytes_f32": product_bytes(plan, 8.0), "product_bytes_f64": product_bytes(plan, 16.0)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

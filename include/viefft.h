@@ -232,8 +232,10 @@ double vie_model_bytes_matvec(const vie_ctx* ctx);
    inside solves, on the internal grid; out[5] the pencil length of their x
    pass (the short period, or the internal Nx); out[6..8] the internal grid;
    out[9], out[10] the y window [y0, y1) of the products inside solves;
-   out[11] their y pencil length (the short y period, or the internal Ny).
-   out holds 12 ints. */
+   out[11] their y pencil length (the short y period, or the internal Ny);
+   out[12], out[13] their z window [z0, z1) (z0 even); out[14] their z
+   pencil length (the short z period, or the internal Nz).  out holds 15
+   ints. */
 int vie_plan(const vie_ctx* ctx, int* out);
 
 #ifdef __cplusplus

@@ -11,7 +11,8 @@ oracle's, which works on the caller's grid with every plane:
   far-field, absorbed-power and gradient outputs, and the mixed solve;
 * the x pass on a shorter period than the grid's (register-resident and
   LDS-staged pencils);
-* boxes limited in two or three axes (x and y windows, short x and y periods);
+* boxes limited in two or three axes (x, y and z windows, short x, y and z
+  periods);
 * an embedded grid (lengths without compiled kernels) with a slab;
 * the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
 """
@@ -100,7 +101,7 @@ def test_plan_puts_the_slab_axis_first(axis, lo, hi, order, pencil):
     finally:
         os.environ.pop("VIE_WINDOW", None)
     assert p == {"axis_order": (0, 1, 2), "x_window": (0, N[0]), "x_pencil": N[0], "grid": N, "y_window": (0, N[1]),
-                 "y_pencil": N[1]}
+                 "y_pencil": N[1], "z_window": (0, N[2]), "z_pencil": N[2]}
 
 
 @pytest.mark.parametrize("axis,lo,hi", SLABS)
@@ -283,6 +284,33 @@ def test_short_y_period_matches_oracle(n, lo, hi, ypen):
     with solver(n, mat, flags, VF.FP32) as s:
         x32, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
     assert rel(x32, ref["x"]) <= 1e-4, rel(x32, ref["x"])
+
+
+@pytest.mark.parametrize("n,lo,hi,order,zw,zpen", [((16, 12, 20), (4, 2, 5), (12, 9, 16), (0, 2, 1), (2, 9), 8),
+                                                  ((12, 16, 128), (3, 2, 20), (7, 10, 100), (0, 1, 2), (20, 100), 96),
+                                                  ((12, 16, 128), (3, 2, 21), (7, 10, 99), (0, 1, 2), (20, 99), 96)])
+def test_short_z_period_matches_oracle(n, lo, hi, order, zw, zpen):
+    """Boxes limited in all three axes: the z passes of the products inside
+    solves run on the short z period (the z window, rounded down to an even
+    start, shifted to entry 0), LDS-staged (8) and register-resident (96)."""
+    mat, eps, flags = box_scene(lo, hi, 20, n)
+    b = O.incident(n, D, K0, 35.0, 15.0, 25.0) * flags
+    op = oracle(n, eps, flags)
+    for method in (3, 1, 5):
+        ref = op.solve(b, method, tol=1e-300, maxit=8)
+        with solver(n, mat, flags) as s:
+            p = s.plan()
+            assert p["axis_order"] == order and p["z_window"] == zw and p["z_pencil"] == zpen
+            x, _ = s.solve(b, method, tol=1e-300, maxit=8)
+        assert rel(x, ref["x"]) <= 1e-9, (method, rel(x, ref["x"]))
+    with solver(n, mat, flags) as s:
+        s.set_rhs(b)
+        m = s.solve_mixed(tol=1e-9)
+        assert m["converged"] and m["true_relres"] < 1e-9
+        assert rel(s.solution(), op.solve(b, 3, tol=1e-10, maxit=2000)["x"]) <= 1e-7
+    with solver(n, mat, flags, VF.FP32) as s:
+        x32, _ = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+    assert rel(x32, op.solve(b, 3, tol=1e-300, maxit=8)["x"]) <= 1e-4
 
 
 def test_embedded_slab_matches_oracle():

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -132,22 +133,23 @@ template <class R> struct Op {
     Geo g{};
     int n[3];
     double d[3], k0;
-    // per-axis tables, slots 0-2 the grid's axes; slots 3 and 4 the x and y
-    // axes of the short periods (lengths nxs, nys)
-    T* tw[5] = {};
-    T* twist[5] = {};
-    T* twP[5] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
-    T* U[5] = {};
-    T* Vv[5] = {};
-    T* Ua[5] = {};
-    T* Va[5] = {};
-    int* fold[5] = {};
+    // per-axis tables, slots 0-2 the grid's axes; slots 3, 4 and 5 the x, y
+    // and z axes of the short periods (lengths nxs, nys, nzs)
+    T* tw[6] = {};
+    T* twist[6] = {};
+    T* twP[6] = {};    // exp(-2 pi j e / (2 N_a)), e < 2 N_a, per axis
+    T* U[6] = {};
+    T* Vv[6] = {};
+    T* Ua[6] = {};
+    T* Va[6] = {};
+    int* fold[6] = {};
     T* G = nullptr;
     T* dg = nullptr;           // per-face diagonal d_f (3V)
     uint8_t* flags = nullptr;
     uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
     int wx0 = 0, wx1 = 0;      // x window: the x planes with an active face
     int wy0 = 0, wy1 = 0;      // y window: the y planes with an active face
+    int wz0 = 0, wz1 = 0;      // z window: the z planes with an active face (wz0 even)
     // Short-period x pass: a product whose input and output live on the x
     // window [wx0, wx1) is a linear convolution along x of width
     // w = wx1 - wx0, which a padded transform of any length 2 nxs with
@@ -155,12 +157,13 @@ template <class R> struct Op {
     // x pass runs at the grid's length.  Gs is the G~ octant of the
     // (nxs, Ny, Nz) grid scaled by Nx / nxs (the tables keep the grid's
     // normalisation 1/(8 Nx Ny Nz)).  Products whose outputs the solver keeps
-    // zero (wskip) also use the y window [wy0, wy1): with nys > 0 their y
-    // passes and W2 have the short y period nys >= wy1 - wy0 and their x pass
-    // the octant Gy of the (nxs or Nx, nys, Nz) grid, scaled likewise.
-    int nxs = 0, nys = 0;
+    // zero (wskip) also use the y window [wy0, wy1) and the z window [wz0,
+    // wz1): with nys > 0 (nzs > 0) their y (z) transforms have the short
+    // period nys >= wy1 - wy0 (nzs >= wz1 - wz0), and their x pass reads the
+    // octant Gw of the (nxs or Nx, nys or Ny, nzs or Nz) grid, scaled likewise.
+    int nxs = 0, nys = 0, nzs = 0;
     T* Gs = nullptr;
-    T* Gy = nullptr;
+    T* Gw = nullptr;
     T* W1 = nullptr;
     T* W2 = nullptr;
     bool own_w = true;
@@ -235,6 +238,9 @@ template <class R> struct Op {
         g.y0 = wy0 = 0;
         g.y1 = wy1 = n[1];
         g.nyp = n[1];
+        wz0 = 0;
+        wz1 = n[2];
+        g.nzp = n[2];
         for (int a = 0; a < 3; ++a) build_axis(a, n[a], st);
         const long gsz = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
         G = dalloc<T>(gsz);
@@ -262,12 +268,12 @@ template <class R> struct Op {
         set_k0(kk, st);
     }
 
-    // Short periods of lengths ns (x) and nsy (y), 0: none: slot-3/4 tables,
-    // the octants Gs and Gy (filled by the caller from green_octant) and the
-    // MixTabs that use them.
-    void set_short(int ns, int nsy, hipStream_t st)
+    // Short periods of lengths ns (x), nsy (y) and nsz (z), 0: none:
+    // slot-3/4/5 tables, the octants Gs and Gw (filled by the caller from
+    // green_octant) and the MixTabs that use them.
+    void set_short(int ns, int nsy, int nsz, hipStream_t st)
     {
-        if (ns == nxs && nsy == nys) return;
+        if (ns == nxs && nsy == nys && nsz == nzs) return;
         if (ns != nxs) {
             free_axis(3);
             nxs = ns;
@@ -284,10 +290,19 @@ template <class R> struct Op {
                 axis_k0(4, nsy, d[1], st);
             }
         }
+        if (nsz != nzs) {
+            free_axis(5);
+            nzs = nsz;
+            if (nsz) {
+                build_axis(5, nsz, st);
+                axis_k0(5, nsz, d[2], st);
+            }
+        }
         dfree(Gs);
-        dfree(Gy);
+        dfree(Gw);
         if (nxs) Gs = dalloc<T>((long)(nxs + 1) * (n[1] + 1) * (n[2] + 1));
-        if (nys) Gy = dalloc<T>((long)((nxs ? nxs : n[0]) + 1) * (nys + 1) * (n[2] + 1));
+        if (nys || nzs)
+            Gw = dalloc<T>((long)((nxs ? nxs : n[0]) + 1) * ((nys ? nys : n[1]) + 1) * ((nzs ? nzs : n[2]) + 1));
         short_tabs();
     }
     void short_tabs()
@@ -305,15 +320,16 @@ template <class R> struct Op {
         }
         mfy = mfs;
         may = mas;
-        if (nys) {
-            mfy.U[1] = U[4];
-            mfy.Vv[1] = Vv[4];
-            mfy.fold[1] = fold[4];
-            may.U[1] = Ua[4];
-            may.Vv[1] = Va[4];
-            may.fold[1] = fold[4];
-            mfy.G = may.G = Gy;
+        for (int a = 1; a < 3; ++a) {
+            if (!(a == 1 ? nys : nzs)) continue;
+            mfy.U[a] = U[3 + a];
+            mfy.Vv[a] = Vv[3 + a];
+            mfy.fold[a] = fold[3 + a];
+            may.U[a] = Ua[3 + a];
+            may.Vv[a] = Va[3 + a];
+            may.fold[a] = fold[3 + a];
         }
+        if (nys || nzs) mfy.G = may.G = Gw;
     }
 
     // v_s = (j Z0/(k0 delta_s NP)) (1 - e^{-j theta_s}) of slot a (its
@@ -353,14 +369,15 @@ template <class R> struct Op {
         }
         if (nxs) axis_k0(3, nxs, d[0], st);
         if (nys) axis_k0(4, nys, d[1], st);
+        if (nzs) axis_k0(5, nzs, d[2], st);
         short_tabs();
     }
     ~Op()
     {
-        for (int a = 0; a < 5; ++a) free_axis(a);
+        for (int a = 0; a < 6; ++a) free_axis(a);
         dfree(G);
         dfree(Gs);
-        dfree(Gy);
+        dfree(Gw);
         dfree(dg);
         dfree(flags);
         dfree(zr);
@@ -401,14 +418,19 @@ template <class R> struct Op {
         const int off = sizeof(R) == 8 ? 0 : 10;  // timing classes 1-5 FP64, 11-15 FP32
         auto t0 = [&](int c) { if (tm && tm->cls == c + off) tm->mark(st); };
         t0(1);
-        HIPCHK(run_zfwd<R>(gw, lin, W1, tw[2], twist[2], twP[2], done, st));
-        t0(1);
-        const bool ysh = lin.zr && lin.wskip && nys;  // the y window's pencils on the short y period
-        const int ay = ysh ? 4 : 1;
+        // the y (z) window's pencils on the short y (z) period
+        const bool ysh = lin.zr && lin.wskip && nys, zsh = lin.zr && lin.wskip && nzs;
+        const int ay = ysh ? 4 : 1, az = zsh ? 5 : 2;
         if (ysh) {
             gw.nyp = nys;
             gw.ys = wy0;
         }
+        if (zsh) {
+            gw.nzp = nzs;
+            gw.zs = wz0;
+        }
+        HIPCHK(run_zfwd<R>(gw, lin, W1, tw[az], twist[az], twP[az], done, st));
+        t0(1);
         t0(2);
         HIPCHK(run_yfwd<R>(gw, W1, W2, tw[ay], twist[ay], twP[ay], done, st));
         t0(2);
@@ -419,7 +441,7 @@ template <class R> struct Op {
             ax = 3;
             nxp = nxs;
         }
-        const MixTabs<R>& mt = ysh ? (adj ? may : mfy) : ax ? (adj ? mas : mfs) : (adj ? ma : mf);
+        const MixTabs<R>& mt = (ysh || zsh) ? (adj ? may : mfy) : ax ? (adj ? mas : mfs) : (adj ? ma : mf);
         t0(3);
         HIPCHK(run_xmix<R>(gx, nxp, W2, tw[ax], twist[ax], twP[ax], mt, done, st));
         t0(3);
@@ -431,7 +453,7 @@ template <class R> struct Op {
         dd.conj = adj ? 1 : 0;
         dd.skip = lin.wskip;
         t0(5);
-        HIPCHK(run_zinv<R>(gw, W1, src, y, dd, flags, dots, tw[2], twist[2], twP[2], done, st, &nblk));
+        HIPCHK(run_zinv<R>(gw, W1, src, y, dd, flags, dots, tw[az], twist[az], twP[az], done, st, &nblk));
         t0(5);
         return nblk;
     }
@@ -708,29 +730,38 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
                        cim[2], c->eps_dev, op->dg);
     HIPCHK(hipGetLastError());
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
-    // x and y windows: the x and y planes holding an active face of any
-    // component (all of them when windows are off or no face is active)
-    int lo = n[0], hi = 0, ylo = n[1], yhi = 0;
+    // x, y and z windows: the planes holding an active face of any component
+    // (all of them when windows are off or no face is active); the z window
+    // starts on an even plane
+    int lo = n[0], hi = 0, ylo = n[1], yhi = 0, zlo = n[2], zhi = 0;
+    auto nz0 = [](uint8_t v) { return v != 0; };
     for (int f = 0; f < 3; ++f)
         for (int i = 0; i < n[0]; ++i)
             for (int j = 0; j < n[1]; ++j) {
                 const uint8_t* p = c->flags_h.data() + f * c->V + ((long)i * n[1] + j) * n[2];
-                if (std::any_of(p, p + n[2], [](uint8_t v) { return v != 0; })) {
-                    lo = std::min(lo, i);
-                    hi = std::max(hi, i + 1);
-                    ylo = std::min(ylo, j);
-                    yhi = std::max(yhi, j + 1);
-                }
+                const uint8_t* first = std::find_if(p, p + n[2], nz0);
+                if (first == p + n[2]) continue;
+                const uint8_t* last = std::find_if(std::make_reverse_iterator(p + n[2]),
+                                                   std::make_reverse_iterator(p), nz0).base();
+                lo = std::min(lo, i);
+                hi = std::max(hi, i + 1);
+                ylo = std::min(ylo, j);
+                yhi = std::max(yhi, j + 1);
+                zlo = std::min(zlo, (int)(first - p));
+                zhi = std::max(zhi, (int)(last - p));
             }
     if (hi <= lo || !c->windows) {
-        lo = ylo = 0;
+        lo = ylo = zlo = 0;
         hi = n[0];
         yhi = n[1];
+        zhi = n[2];
     }
     op->wx0 = lo;
     op->wx1 = hi;
     op->wy0 = ylo;
     op->wy1 = yhi;
+    op->wz0 = zlo & ~1;
+    op->wz1 = zhi;
     const long rows = 3L * n[0] * n[1];
     hipLaunchKernelGGL(k_zrange, dim3(grid1(rows)), dim3(NT), 0, c->stream, rows, n[2], op->flags, op->zr);
     HIPCHK(hipGetLastError());
@@ -748,10 +779,12 @@ int short_len(const vie_ctx* c, int w, int axis)
 {
     const int nn = c->desc.n[axis];
     if (!c->windows || w >= nn) return 0;
-    if (const char* e = std::getenv(axis == 0 ? "VIE_XSHORT" : "VIE_YSHORT")) {
+    if (axis == 2 && nn % 2) return 0;  // z pairs must not straddle rows
+    if (const char* e = std::getenv(axis == 0 ? "VIE_XSHORT" : axis == 1 ? "VIE_YSHORT" : "VIE_ZSHORT")) {
         const int m = std::atoi(e);
         return (m >= w && m < nn && size_supported(m)) ? m : 0;
     }
+    // the z passes use the y passes' register configurations
     auto reg = [axis](int m) {
         return axis == 0 ? x_register<double>(m) && x_register<float>(m) : y_register<double>(m) && y_register<float>(m);
     };
@@ -762,17 +795,17 @@ int short_len(const vie_ctx* c, int w, int axis)
     return 0;
 }
 
-// G~ octant of the (px, py, Nz) grid, scaled by (Nx Ny) / (px py), into
-// the FP64 staging buffer Gshort and the operators' G64 / G32 copies.
-void short_octant(vie_ctx* c, int px, int py, double2* g64, float2* g32)
+// G~ octant of the (px, py, pz) grid, scaled by (Nx Ny Nz) / (px py pz),
+// into the FP64 staging buffer Gshort and the operators' G64 / G32 copies.
+void short_octant(vie_ctx* c, int px, int py, int pz, double2* g64, float2* g32)
 {
     const int* n = c->desc.n;
-    const int n3[3] = {px, py, n[2]};
-    const long tot = (long)(px + 1) * (py + 1) * (n[2] + 1);
+    const int n3[3] = {px, py, pz};
+    const long tot = (long)(px + 1) * (py + 1) * (pz + 1);
     if (!c->Gshort) c->Gshort = dalloc<double2>((long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1));
     green_octant(c, n3, c->Gshort);
     hipLaunchKernelGGL(k_scale_c64, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
-                       (double)n[0] * n[1] / ((double)px * py), c->Gshort);
+                       (double)n[0] * n[1] * n[2] / ((double)px * py * pz), c->Gshort);
     HIPCHK(hipGetLastError());
     if (g64) HIPCHK(hipMemcpyAsync(g64, c->Gshort, tot * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
     if (g32) {
@@ -792,14 +825,22 @@ void short_sync(vie_ctx* c)
     if (!o64 && !o32) return;
     const int wx = o64 ? o64->wx1 - o64->wx0 : o32->wx1 - o32->wx0;
     const int wy = o64 ? o64->wy1 - o64->wy0 : o32->wy1 - o32->wy0;
+    const int wz = o64 ? o64->wz1 - o64->wz0 : o32->wz1 - o32->wz0;
     const int ns = c->medium_ready ? short_len(c, wx, 0) : 0;
     const int nsy = c->medium_ready ? short_len(c, wy, 1) : 0;
-    if (o64) o64->set_short(ns, nsy, c->stream);
-    if (o32) o32->set_short(ns, nsy, c->stream);
-    if (!c->green_ready) return;
+    const int nsz = c->medium_ready ? short_len(c, wz, 2) : 0;
     const int* n = c->desc.n;
-    if (ns) short_octant(c, ns, n[1], o64 ? o64->Gs : nullptr, o32 ? o32->Gs : nullptr);
-    if (nsy) short_octant(c, ns ? ns : n[0], nsy, o64 ? o64->Gy : nullptr, o32 ? o32->Gy : nullptr);
+    if (nsz) {  // partial sums of the z inverse at the short z length
+        Geo gz{n[0], n[1], nsz, c->V};
+        ensure_partial(c, 3 * std::max(zinv_blocks<double>(gz), zinv_blocks<float>(gz)));
+    }
+    if (o64) o64->set_short(ns, nsy, nsz, c->stream);
+    if (o32) o32->set_short(ns, nsy, nsz, c->stream);
+    if (!c->green_ready) return;
+    if (ns) short_octant(c, ns, n[1], n[2], o64 ? o64->Gs : nullptr, o32 ? o32->Gs : nullptr);
+    if (nsy || nsz)
+        short_octant(c, ns ? ns : n[0], nsy ? nsy : n[1], nsz ? nsz : n[2], o64 ? o64->Gw : nullptr,
+                     o32 ? o32->Gw : nullptr);
 }
 
 template <> Op<double>* make_op<double>(vie_ctx* c)
@@ -2507,7 +2548,7 @@ int vie_kernel_time(vie_ctx* c, int* launches, double* mean_ms)
 double vie_model_bytes_matvec(const vie_ctx* c)
 {
     if (!c) return 0.0;
-    int p[12];
+    int p[15];
     if (vie_plan(c, p) != VIE_OK) return 0.0;
     const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
     const double Vw = (double)(p[4] - p[3]) * p[7] * p[8];
@@ -2519,22 +2560,22 @@ int vie_plan(const vie_ctx* c, int* out)
 {
     if (!c || !out) return VIE_ERR_ARG;
     const int* n = c->desc.n;
-    int x0 = 0, x1 = n[0], y0 = 0, y1 = n[1], ns = 0, nsy = 0;
-    if (c->op64) {
-        x0 = c->op64->wx0;
-        x1 = c->op64->wx1;
-        y0 = c->op64->wy0;
-        y1 = c->op64->wy1;
-        ns = c->op64->nxs;
-        nsy = c->op64->nys;
-    } else if (c->op32) {
-        x0 = c->op32->wx0;
-        x1 = c->op32->wx1;
-        y0 = c->op32->wy0;
-        y1 = c->op32->wy1;
-        ns = c->op32->nxs;
-        nsy = c->op32->nys;
-    }
+    int x0 = 0, x1 = n[0], y0 = 0, y1 = n[1], z0 = 0, z1 = n[2], ns = 0, nsy = 0, nsz = 0;
+    auto take = [&](auto* o) {
+        x0 = o->wx0;
+        x1 = o->wx1;
+        y0 = o->wy0;
+        y1 = o->wy1;
+        z0 = o->wz0;
+        z1 = o->wz1;
+        ns = o->nxs;
+        nsy = o->nys;
+        nsz = o->nzs;
+    };
+    if (c->op64)
+        take(c->op64);
+    else if (c->op32)
+        take(c->op32);
     for (int a = 0; a < 3; ++a) {
         out[a] = c->perm[a];
         out[6 + a] = n[a];
@@ -2545,6 +2586,9 @@ int vie_plan(const vie_ctx* c, int* out)
     out[9] = y0;
     out[10] = y1;
     out[11] = nsy ? nsy : n[1];
+    out[12] = z0;
+    out[13] = z1;
+    out[14] = nsz ? nsz : n[2];
     return VIE_OK;
 }
 

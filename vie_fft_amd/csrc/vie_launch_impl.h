@@ -44,6 +44,12 @@ static bool yz_lds_forced()
     return v;
 }
 
+// Whether the z pencils are a short period (entry n: plane n + zs of rows of
+// nz > nzp cells); the full-row instantiations fix nz = N and zs = 0, which
+// keeps the 128-point z passes at their earlier cost (one more compare and a
+// run-time row stride cost the FP32 z inverse pass 7%).
+static inline bool short_z(const Geo& g) { return g.nzp != g.nz || g.zs != 0; }
+
 // z forward pass: the LDS version for FP32, the register version for FP64.
 // Over the 48-plane window of the 128^3 layered workload the LDS version
 // takes 26.2 us FP32 / 46.8 us FP64 per launch, the register one 29.3 / 42.7
@@ -58,7 +64,7 @@ template <class R> static bool zfwd_reg()
     return v < 0 ? sizeof(R) == 8 : v == 1;
 }
 
-template <class R, int n>
+template <class R, int n, bool ZS>
 static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
@@ -68,13 +74,13 @@ static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const 
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3);
-            hipLaunchKernelGGL((k_zfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, lin, W1, twP, done);
+            hipLaunchKernelGGL((k_zfwd_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, lin, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_z<R>(n);
     dim3 grid((unsigned)((rows + B - 1) / B), 3);
-    hipLaunchKernelGGL((k_zfwd<R, n, B>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done);
+    hipLaunchKernelGGL((k_zfwd<R, n, B, ZS>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -86,14 +92,14 @@ static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx
         if (!yz_lds_forced()) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
-            const long nzt = (2L * g.nz + B - 1) / B;
+            const long nzt = (2L * g.nzp + B - 1) / B;
             dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
             hipLaunchKernelGGL((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, W2, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
-    const long nzt = (2L * g.nz + B - 1) / B;
+    const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
     hipLaunchKernelGGL((k_yfwd<R, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done);
     return hipGetLastError();
@@ -107,20 +113,20 @@ static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx
         if (!yz_lds_forced()) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
-            const long nzt = (2L * g.nz + B - 1) / B;
+            const long nzt = (2L * g.nzp + B - 1) / B;
             dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
             hipLaunchKernelGGL((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
-    const long nzt = (2L * g.nz + B - 1) / B;
+    const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
     hipLaunchKernelGGL((k_yinv<R, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done);
     return hipGetLastError();
 }
 
-template <class R, int n>
+template <class R, int n, bool ZS>
 static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, cx<R>* y, const Diag<R>& dg,
                               const uint8_t* flags, const Dots<R>& dots, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st, long* nblk)
@@ -132,7 +138,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3);
             if (nblk) *nblk = (long)grid.x * grid.y;
-            hipLaunchKernelGGL((k_zinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, src, y, dg,
+            hipLaunchKernelGGL((k_zinv_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg,
                                flags, dots, twP, done);
             return hipGetLastError();
         }
@@ -140,7 +146,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
     constexpr int B = b_z<R>(n);
     dim3 grid((unsigned)((rows + B - 1) / B), 3);
     if (nblk) *nblk = (long)grid.x * grid.y;
-    hipLaunchKernelGGL((k_zinv<R, n, B>), grid, dim3(NT), 0, st, g, W1, src, y, dg, flags, dots, tw, twist, done);
+    hipLaunchKernelGGL((k_zinv<R, n, B, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg, flags, dots, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -159,9 +165,11 @@ template <>
 hipError_t run_zfwd<R_>(const Geo& g, const Lin<R_>& lin, cx<R_>* W1, const cx<R_>* tw,
                         const cx<R_>* twist, const cx<R_>* twP, const int* done, hipStream_t st)
 {
-    switch (g.nz) {
+    switch (g.nzp) {
 #define CASE(n) \
-    case n: return launch_zfwd<R_, n>(g, lin, W1, tw, twist, twP, done, st);
+    case n:                                                                                     \
+        return short_z(g) ? launch_zfwd<R_, n, true>(g, lin, W1, tw, twist, twP, done, st)        \
+                          : launch_zfwd<R_, n, false>(g, lin, W1, tw, twist, twP, done, st);
         VIE_SIZES(CASE)
 #undef CASE
     default:
@@ -240,7 +248,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
         if (!xpass_lds_forced()) {
             using X = XReg<R, n>;
             constexpr int B = NT / X::T;
-            const long nzt = (2L * g.nz + B - 1) / B;
+            const long nzt = (2L * g.nzp + B - 1) / B;
             dim3 grid((unsigned)(2L * g.nyp * nzt));
             // FP32: pencils whose data lies in the first half or three quarters
             // (a window narrower than the pencil) use a kernel that knows it
@@ -265,7 +273,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
         }
     }
     constexpr int B = b_x<R>(n);
-    const long nzt = (2L * g.nz + B - 1) / B;
+    const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)(2L * g.nyp * nzt));
     hipLaunchKernelGGL((k_xmix<R, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done);
     return hipGetLastError();
@@ -343,9 +351,11 @@ hipError_t run_zinv<R_>(const Geo& g, const cx<R_>* W1, const cx<R_>* src, cx<R_
                         const cx<R_>* tw, const cx<R_>* twist, const cx<R_>* twP, const int* done,
                         hipStream_t st, long* nblk)
 {
-    switch (g.nz) {
+    switch (g.nzp) {
 #define CASE(n) \
-    case n: return launch_zinv<R_, n>(g, W1, src, y, dg, flags, dots, tw, twist, twP, done, st, nblk);
+    case n:                                                                                           \
+        return short_z(g) ? launch_zinv<R_, n, true>(g, W1, src, y, dg, flags, dots, tw, twist, twP, done, st, nblk) \
+                          : launch_zinv<R_, n, false>(g, W1, src, y, dg, flags, dots, tw, twist, twP, done, st, nblk);
         VIE_SIZES(CASE)
 #undef CASE
     default:

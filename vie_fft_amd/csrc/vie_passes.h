@@ -22,7 +22,9 @@ namespace vie {
 // the y passes load and store.  The x pass's pencil entry n is plane n + xs
 // (xs = x0 when the pencils have the short period).  W2 holds nyp y pencils
 // of length 2 nyp per (x, kz): nyp = ny, or the short y period, whose entry
-// n is y plane n + ys (ys = y0).
+// n is y plane n + ys (ys = y0).  W1 and W2 hold 2 nzp z frequencies: nzp =
+// nz, or the short z period, whose entry n is z plane n + zs (nz and zs
+// even, so pairs of entries stay 16-byte aligned and never straddle nz).
 struct Geo {
     int nx, ny, nz;
     long V;
@@ -30,6 +32,7 @@ struct Geo {
     int y0 = 0, y1 = 0;
     int xs = 0;
     int nyp = 0, ys = 0;
+    int nzp = 0, zs = 0;
 };
 
 // z row of window row r, r < (x1 - x0)(y1 - y0): x0 + r / wy, y0 + r % wy
@@ -157,7 +160,7 @@ __device__ __forceinline__ void st2(double2* p, double2 a, double2 b)
 
 // ------------------------------------------------------------------ zfwd
 // grid (ceil(Nx*Ny / B), 3 components); N == Nz.
-template <class R, int N, int B>
+template <class R, int N, int B, bool ZS = false>
 __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restrict__ W1,
                                              const cx<R>* __restrict__ twg,
                                              const cx<R>* __restrict__ twistg, const int* done)
@@ -169,6 +172,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     __shared__ T tws[N];
     if (done && *done) return;
     const int s = blockIdx.y;
+    // z row length and the z plane of entry 0: without ZS the pencil is the
+    // whole row (N = nz, entry n is plane n), fixed at compile time
+    const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
     const long rows = (long)g.nx * g.ny;
     const int nw = win_rows(g);  // the window's rows (column col: window row r0 + col)
     const int r0 = (int)xcd_tile(blockIdx.x, gridDim.x) * B;
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     for (int t = threadIdx.x; t < B; t += NT) {
         const bool ok = r0 + t < nw;
         rw[t] = ok ? win_row(g, r0 + t) : 0;
-        zrs[t] = lin.zr && ok ? lin.zr[s * rows + rw[t]] : ((unsigned)N << 16);
+        zrs[t] = lin.zr && ok ? lin.zr[s * rows + rw[t]] : ((unsigned)nzr << 16);
     }
     T c[3];
     load_coef<R>(lin.coef, c);
@@ -199,8 +205,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             x0[it][0] = x0[it][1] = x1[it][0] = x1[it][1] = x2[it][0] = x2[it][1] = czero<T>();
             // with row ranges, pairs that miss the row's active range are zero
             const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
-            if (rok && n + 1 >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
-                const long idx = s * g.V + row * N + n;
+            const int z = n + zs;  // z plane of entry n
+            if (rok && z + 1 >= (int)(rg & 0xffffu) && z < (int)(rg >> 16)) {
+                const long idx = s * g.V + row * nzr + z;
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
                 if (lin.x[2]) ld2(lin.x[2] + idx, x2[it][0], x2[it][1]);
@@ -213,8 +220,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int col = e / (N / 2), n = 2 * (e % (N / 2));
             const long row = rw[col];
             T v0 = czero<T>(), v1 = czero<T>();
+            const int z = n + zs;
             if (r0 + col < nw) {
-                const long idx = s * g.V + row * N + n;
+                const long idx = s * g.V + row * nzr + z;
                 v0 = cmul(c[0], x0[it][0]);
                 v1 = cmul(c[0], x0[it][1]);
                 if (lin.x[1]) {
@@ -225,11 +233,11 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                     v0 = cadd(v0, cmul(c[2], x2[it][0]));
                     v1 = cadd(v1, cmul(c[2], x2[it][1]));
                 }
-                if (lin.out && (!lin.wskip || (n + 1 >= (int)(zrs[col] & 0xffffu) && n < (int)(zrs[col] >> 16))))
+                if (lin.out && (!lin.wskip || (z + 1 >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     st2(lin.out + idx, v0, v1);
                 const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
                 if (dropall) v0 = v1 = czero<T>();
-                if (s == 2 && n + 1 == N - 1) v1 = czero<T>();
+                if (s == 2 && z + 1 == nzr - 1) v1 = czero<T>();
             }
             buf[n * LD + col] = v0;
             buf[n * LD + B + col] = cmul(v0, tws[n]);
@@ -241,17 +249,18 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const int col = e / N, n = e % N;
             const long row = rw[col];
             T v = czero<T>();
+            const int z = n + zs;
             if (r0 + col < nw) {
-                const long idx = s * g.V + row * N + n;
+                const long idx = s * g.V + row * nzr + z;
                 const unsigned rg = zrs[col];
-                if (n >= (int)(rg & 0xffffu) && n < (int)(rg >> 16)) {
+                if (z >= (int)(rg & 0xffffu) && z < (int)(rg >> 16)) {
                     v = cmul(c[0], lin.x[0][idx]);
                     if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
                     if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
                 }
-                if (lin.out && (!lin.wskip || (n >= (int)(zrs[col] & 0xffffu) && n < (int)(zrs[col] >> 16))))
+                if (lin.out && (!lin.wskip || (z >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     lin.out[idx] = v;
-                const bool drop = (s == 2 && n == N - 1) || (s == 0 && row >= rows - g.ny) ||
+                const bool drop = (s == 2 && z == nzr - 1) || (s == 0 && row >= rows - g.ny) ||
                                   (s == 1 && (row % g.ny) == g.ny - 1);
                 if (drop) v = czero<T>();
             }
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
     __shared__ T tws[N];
     if (done && *done) return;
     const int s = blockIdx.y;
-    const int P2 = 2 * g.nz;
+    const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
@@ -348,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     __shared__ T vz[B];
     __shared__ int fz[B];
     if (done && *done) return;
-    const int P1 = 2 * g.nyp, P2 = 2 * g.nz;
+    const int P1 = 2 * g.nyp, P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
@@ -368,8 +377,8 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
         fz[t] = ok ? mt.fold[2][z0 + t] : 0;
     }
     const T uy = mt.U[1][ty], vy = mt.Vv[1][ty];
-    const long fyoff = (long)mt.fold[1][ty] * (g.nz + 1);
-    const long gs = (long)(g.nyp + 1) * (g.nz + 1);
+    const long fyoff = (long)mt.fold[1][ty] * (g.nzp + 1);
+    const long gs = (long)(g.nyp + 1) * (g.nzp + 1);
     __syncthreads();
     // x-pencils of the three components: rows i < Nx, columns z0..z0+B
     for (int e = threadIdx.x; e < 3 * N * B; e += NT) {
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
     __shared__ T tws[N];
     if (done && *done) return;
     const int s = blockIdx.y;
-    const int P2 = 2 * g.nz;
+    const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
 
 // ------------------------------------------------------------------ zinv
 // grid (ceil(Nx*Ny / B), 3); N == Nz.  y = flag (trunc(IDFT) + d x), dots.
-template <class R, int N, int B>
+template <class R, int N, int B, bool ZS = false>
 __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1,
                                              const cx<R>* __restrict__ src, cx<R>* __restrict__ y,
                                              Diag<R> dg, const uint8_t* __restrict__ flags, Dots<R> dots,
@@ -473,6 +482,9 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     __shared__ double2 red[NT / 64];
     if (done && *done) return;
     const int f = blockIdx.y;
+    // z row length and the z plane of entry 0: without ZS the pencil is the
+    // whole row (N = nz, entry n is plane n), fixed at compile time
+    const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
     const long rows = (long)g.nx * g.ny;
     const int nw = win_rows(g);  // the window's rows (column col: window row r0 + col)
     const int r0 = (int)xcd_tile(blockIdx.x, gridDim.x) * B;
@@ -512,8 +524,8 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {
             const int col = e / (N / 2), k = 2 * (e % (N / 2));
             const long row = rw[col];
-            if (r0 + col < nw) {
-                const long idx = f * g.V + row * N + k;
+            if (r0 + col < nw && k + zs < nzr) {  // entry k: z plane k + zs (pairs in or out together)
+                const long idx = f * g.V + row * nzr + zs + k;
                 term(idx, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
                 term(idx + 1, cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1])));
             }
@@ -522,7 +534,8 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         for (int e = threadIdx.x; e < B * N; e += NT) {
             const int col = e / N, k = e % N;
             const long row = rw[col];
-            if (r0 + col < nw) term(f * g.V + row * N + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
+            if (r0 + col < nw && k + zs < nzr)
+                term(f * g.V + row * nzr + zs + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
         }
     }
     if (dots.n > 0) {

@@ -261,7 +261,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
     __shared__ T_ gsl[GSL ? (N + 1) * B : 1];    // G octant slice [fx][col]
     if (done && *done) return;
-    const int P1 = 2 * g.nyp, P2 = 2 * g.nz;
+    const int P1 = 2 * g.nyp, P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
@@ -289,12 +289,12 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             }
         }
     if constexpr (GSL) {
-        const long gs = (long)(g.nyp + 1) * (g.nz + 1);
-        const long gy = (long)fold_split(ty, g.nyp) * (g.nz + 1);
+        const long gs = (long)(g.nyp + 1) * (g.nzp + 1);
+        const long gy = (long)fold_split(ty, g.nyp) * (g.nzp + 1);
         for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
             const int fxe = e / B, ce = e % B;
             const int tze = z0 + ce < P2 ? z0 + ce : z0;
-            T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tze, g.nz)];
+            T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tze, g.nzp)];
             gsl[e] = mt.gconj ? cconj(gv) : gv;
         }
     }
@@ -308,8 +308,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const int tz = colok ? z0 + col : z0;
     const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
     // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
-    const T_* gcol = mt.G + (long)fold_split(ty, g.nyp) * (g.nz + 1) + fold_split(tz, g.nz);
-    const unsigned gs = (unsigned)(g.nyp + 1) * (g.nz + 1);
+    const T_* gcol = mt.G + (long)fold_split(ty, g.nyp) * (g.nzp + 1) + fold_split(tz, g.nzp);
+    const unsigned gs = (unsigned)(g.nyp + 1) * (g.nzp + 1);
     auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
         const int kf = kx < P - kx ? kx : P - kx;
         return (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
@@ -402,7 +402,7 @@ template <int N> __device__ __forceinline__ int split_of(int k) { return (k & 1)
 
 // z forward: rows of Nz cells (optionally the Lin combination) -> W1 rows of
 // 2Nz in split order.  grid (ceil(Nx*Ny / B), 3); N == Nz.
-template <class R, int N, int T, class PL>
+template <class R, int N, int T, class PL, bool ZS = false>
 __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restrict__ W1,
                                                const cx<R>* __restrict__ twP, const int* done)
 {
@@ -414,6 +414,9 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
+    // z row length and the z plane of entry 0: without ZS the pencil is the
+    // whole row (N = nz, entry n is plane n), fixed at compile time
+    const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;  // consecutive lanes: consecutive cells
     // window row r of the x and y windows
@@ -423,9 +426,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     T_ c[3];
     load_coef<R>(lin.coef, c);
     const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
-    const long base = s * g.V + row * N;
-    // row range outside which the inputs vanish (the whole row without lin.zr)
-    int lo = 0, hi = N;
+    // entry n is z plane n + zs of the row
+    const long base = s * g.V + row * nzr + zs;
+    // z range outside which the inputs vanish (the whole row without lin.zr)
+    int lo = 0, hi = nzr;
     if (lin.zr && rowok) {
         const unsigned rg = lin.zr[s * rows + row];
         lo = (int)(rg & 0xffffu);
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
         for (int q = 0; q < r1; ++q) {
             const int n = i + q * (P / r1), e = b * r1 + q;
             v[e] = w1[e] = w2[e] = czero<T_>();
-            if (n < N && rowok && n >= lo && n < hi) {
+            if (n < N && rowok && n + zs >= lo && n + zs < hi) {
                 v[e] = lin.x[0][base + n];
                 if (lin.x[1]) w1[e] = lin.x[1][base + n];
                 if (lin.x[2]) w2[e] = lin.x[2][base + n];
@@ -454,9 +458,9 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
         for (int q = 0; q < r1; ++q) {
             const int n = i + q * (P / r1), e = b * r1 + q;
             T_ z = cadd(cadd(cmul(c[0], v[e]), cmul(c[1], w1[e])), cmul(c[2], w2[e]));
-            if (n < N && rowok) {
-                if (lin.out && (!lin.wskip || (n >= lo && n < hi))) lin.out[base + n] = z;
-                if (dropall || (s == 2 && n == N - 1)) z = czero<T_>();
+            if (n < N && rowok && n + zs < nzr) {
+                if (lin.out && (!lin.wskip || (n + zs >= lo && n + zs < hi))) lin.out[base + n] = z;
+                if (dropall || (s == 2 && n + zs == nzr - 1)) z = czero<T_>();
             } else {
                 z = czero<T_>();
             }
@@ -477,7 +481,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
 
 // z inverse: W1 rows (split order) -> y = mask (trunc(IDFT) + d x), dots.
 // grid (ceil(Nx*Ny / B), 3); N == Nz.
-template <class R, int N, int T, class PL>
+template <class R, int N, int T, class PL, bool ZS = false>
 __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ W1, const cx<R>* __restrict__ src,
                                                cx<R>* __restrict__ y, Diag<R> dg, const uint8_t* __restrict__ flags,
                                                Dots<R> dots, const cx<R>* __restrict__ twP, const int* done)
@@ -491,6 +495,9 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     __shared__ double2 red[NT / 64];
     if (done && *done) return;
     const int f = blockIdx.y;
+    // z row length and the z plane of entry 0: without ZS the pencil is the
+    // whole row (N = nz, entry n is plane n), fixed at compile time
+    const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
     const long rows = (long)g.nx * g.ny;
     const int col = threadIdx.x / T, t = threadIdx.x % T;
     // window row r of the x and y windows
@@ -512,7 +519,8 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     uint8_t fl[E];
     const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
     const bool e1src = dots.e1 == src;
-    const long base = f * g.V + (rowok ? row : 0) * N;
+    // entry n is z plane n + zs of the row (entries past Nz are not on the grid)
+    const long base = f * g.V + (rowok ? row : 0) * nzr + zs;
     if (rowok) {
         // flags first: x and d are read only on active faces (the output is
         // zero elsewhere), masked dot operands likewise
@@ -521,7 +529,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = t + T * b + q * (P / r1);
-                if (n < N) fl[b * r1 + q] = flags[base + n];
+                if (n < N && n + zs < nzr) fl[b * r1 + q] = flags[base + n];
             }
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {
@@ -529,7 +537,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1), e = b * r1 + q;
-                if (n < N) {
+                if (n < N && n + zs < nzr) {
                     const long idx = base + n;
                     sv[e] = dv[e] = e1v[e] = e2v[e] = czero<T_>();
                     if (fl[e] || (!dots.masked && need1 && e1src)) sv[e] = src[idx];
@@ -553,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1), e = b * r1 + q;
-                if (n < N) {
+                if (n < N && n + zs < nzr) {
                     const long idx = base + n;
                     T_ yv = czero<T_>();
                     if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
@@ -590,7 +598,7 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
-    const int P2 = 2 * g.nz;
+    const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;
@@ -635,7 +643,7 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
     __shared__ T_ tw[P];
     if (done && *done) return;
     const int s = blockIdx.y;
-    const int P2 = 2 * g.nz;
+    const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int i = g.x0 + (int)(tile / nzt), z0 = (tile % nzt) * B;

@@ -98,7 +98,8 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
     scene_at(wavelength_nm, psi) -> geometry.Scene (same grid and cells for
     every point).  Returns one dict per point: Q_ext, Q_abs (metal id
     `metal`), the active-material absorption density (id `active`, if given),
-    iterations, residual replacements, true residual and seconds.  With
+    iterations, residual replacements, true residual and seconds; the first
+    point of each wavelength also holds the context's plan (Solver.plan).  With
     out_dir the per-wavelength files are written (first polarisation of each
     wavelength, as the reference writes one file per process) and merged.
     """
@@ -122,8 +123,10 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
                 s.green_compute()
                 cur = lam
                 setup = time.perf_counter() - t0
+                plan = s.plan()
             else:
                 setup = 0.0
+                plan = None
             t0 = time.perf_counter()
             s.set_rhs_incident(sc.theta, sc.phi, psi)
             info = s.solve_mixed(tol=tol, maxit=maxit) if mixed else s.solve_resident(method, tol=tol, maxit=maxit)
@@ -135,6 +138,8 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
                  "active_density": act, "iterations": info["iterations"], "outer": info["outer"],
                  "true_relres": info["true_relres"], "converged": bool(info["converged"]),
                  "solve_s": solve_s, "setup_s": setup}
+            if plan is not None:
+                r["plan"] = plan
             results.append(r)
             if log:
                 log(r)
@@ -168,7 +173,7 @@ def main():
                       "solve_s": sum(r["solve_s"] for r in res), "setup_s": sum(r["setup_s"] for r in res),
                       "iterations_min": int(its.min()), "iterations_median": float(np.median(its)),
                       "iterations_max": int(its.max()),
-                      "all_converged": all(r["converged"] for r in res)}), flush=True)
+                      "all_converged": all(r["converged"] for r in res), "plan": res[0].get("plan")}), flush=True)
 
 
 if __name__ == "__main__":

@@ -322,11 +322,11 @@ class Solver:
     def plan(self):
         """Internal plan of the resident medium: axis order, x window of the
         products inside solves, pencil length of their x pass, internal grid,
-        y window and y pencil length of the products inside solves."""
-        a = (ct.c_int * 12)()
+        y and z windows and pencil lengths of the products inside solves."""
+        a = (ct.c_int * 15)()
         self._check(self.lib.vie_plan(self.h, a))
         return {"axis_order": tuple(a[0:3]), "x_window": (a[3], a[4]), "x_pencil": a[5], "grid": tuple(a[6:9]),
-                "y_window": (a[9], a[10]), "y_pencil": a[11]}
+                "y_window": (a[9], a[10]), "y_pencil": a[11], "z_window": (a[12], a[13]), "z_pencil": a[14]}
 
 
 def supported_n(n):
