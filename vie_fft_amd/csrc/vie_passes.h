@@ -233,7 +233,9 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                     v0 = cadd(v0, cmul(c[2], x2[it][0]));
                     v1 = cadd(v1, cmul(c[2], x2[it][1]));
                 }
-                if (lin.out && (!lin.wskip || (z + 1 >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
+                // entries past Nz (short periods only) are not on the grid
+                if (lin.out && (!ZS || z < nzr) &&
+                    (!lin.wskip || (z + 1 >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     st2(lin.out + idx, v0, v1);
                 const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
                 if (dropall) v0 = v1 = czero<T>();
@@ -258,7 +260,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                     if (lin.x[1]) v = cadd(v, cmul(c[1], lin.x[1][idx]));
                     if (lin.x[2]) v = cadd(v, cmul(c[2], lin.x[2][idx]));
                 }
-                if (lin.out && (!lin.wskip || (z >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
+                if (lin.out && (!ZS || z < nzr) &&
+                    (!lin.wskip || (z >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     lin.out[idx] = v;
                 const bool drop = (s == 2 && z == nzr - 1) || (s == 0 && row >= rows - g.ny) ||
                                   (s == 1 && (row % g.ny) == g.ny - 1);
