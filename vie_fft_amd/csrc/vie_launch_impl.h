@@ -211,7 +211,9 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // against 503 with OP: 56.9-58.0 us per launch against 60.8 us (G~
 // prefetched with OP), 61.9 us (prefetched, no OP) and 60.2 us (T8 <16,8>).
 // A five-wave register target (96 registers, 14 dwords spilled to scratch)
-// takes 94.5 against 55.7 us; FP64 at three waves spills likewise.
+// takes 94.5 against 55.7 us; FP64 at three waves spills likewise.  FP64 64
+// with OP and G~ read in the product reaches three waves without spilling
+// (147 registers) but issues 23% more instructions: 117.7 against 105.0 us.
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
 #define VIE_XREG(R, N, T_, GM_, OP_, ...)                                                                    \
     template <> struct XReg<R, N> {                                                                       \
