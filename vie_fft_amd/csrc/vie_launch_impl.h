@@ -210,6 +210,9 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // OP is 128 registers, four waves, and 407 VALU instructions per pencil
 // against 503 with OP: 56.9-58.0 us per launch against 60.8 us (G~
 // prefetched with OP), 61.9 us (prefetched, no OP) and 60.2 us (T8 <16,8>).
+// Of the T16 radix orders (window of 48) <8,4,4> issues the fewest VALU and
+// LDS instructions per wave: 1532 and 142, against 1533-1726 and 157-223 for
+// <8,8,2>, <8,2,8>, <4,8,4>, <4,4,8>, <2,8,8>, <4,4,4,2> and <2,4,4,4>.
 // A five-wave register target (96 registers, 14 dwords spilled to scratch)
 // takes 94.5 against 55.7 us; FP64 at three waves spills likewise.  FP64 64
 // with OP and G~ read in the product reaches three waves without spilling
