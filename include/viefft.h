@@ -142,14 +142,59 @@ int vie_set_rhs(vie_ctx* ctx, const double* b);
    wavelength): B and X hold nrhs fields of 3 V complex back to back.
    strategy VIE_BATCH_PLAIN solves each with `method` in the context's
    precision, VIE_BATCH_MIXED with mixed-precision BiCGSTAB (reliable updates;
-   needs an FP64 context).  The solves run one after the other on the
-   context's stream with the operator, G~ and work buffers resident; infos
-   (may be NULL) receives nrhs records.  Returns VIE_OK, or the first
-   non-OK solver status (VIE_NOT_CONVERGED / VIE_BREAKDOWN) after finishing
-   every right-hand side, or a hard error at once. */
+   needs an FP64 context).  The right-hand sides are solved together as
+   batches of as many as the device holds (vie_batch_solve; this reserves the
+   batch storage); infos (may be NULL) receives nrhs records.  Returns
+   VIE_OK, or the first non-OK solver status (VIE_NOT_CONVERGED /
+   VIE_BREAKDOWN) after finishing every right-hand side, or a hard error at
+   once. */
 typedef enum { VIE_BATCH_PLAIN = 0, VIE_BATCH_MIXED = 1 } vie_batch_strategy;
 int vie_solve_batched(vie_ctx* ctx, int nrhs, int strategy, int method, double tol, int maxit, const double* B,
                       double* X, vie_info* infos);
+
+/* ---- batches: many instances of one geometry in one set of launches ----
+   The reference's only parallelism is independent frequencies, one process
+   each (vie_fft/main.cpp:27-36, re-running vie_fft/CG_FFT_VIE.cpp:45-61 per
+   wavelength).  Here the instances of one medium's geometry -- wavelengths,
+   polarisations, incidence angles -- are solved together: every pass and
+   vector launch of a batched solve covers all instances still iterating,
+   each with its own solver scalars and done flag, so an instance that
+   converges (or reaches its limit) stops costing work without stopping the
+   others.  Instance 0 is the resident right-hand side / solution of the
+   single-instance calls.
+   vie_batch_reserve: device storage for nb instances (VIE_ERR_OOM if it does
+   not fit; vie_batch_capacity estimates the largest nb); drops resident
+   right-hand sides, solutions and per-instance operators, and returns to one
+   operator (the context's k0 and permittivity) for every instance. */
+int vie_batch_reserve(vie_ctx* ctx, int nb);
+int vie_batch_capacity(vie_ctx* ctx, int* nb_max);
+/* Per-instance operators for instances [i0, i0 + count): wavenumbers k0[]
+   and, for a medium set with material ids, permittivity tables eps_tables
+   (count x n_mat complex, engineering sign like Real_EPR/Imag_EPR; NULL: the
+   context's permittivity).  Green_Function_Compute (vie_fft/GreenFunction.h:
+   8-141) for all count wavenumbers runs in one set of launches; the
+   diagonal term (vie_fft/fft.h:405-588) follows each table.  Needs
+   vie_set_medium first (a new medium drops them). */
+int vie_batch_set_operators(vie_ctx* ctx, int i0, int count, const double* k0, const double* eps_tables,
+                            int n_mat);
+/* Instance i's right-hand side: the masked plane wave at its wavenumber
+   (Incident_Wave, vie_fft/GreenFunction.h:144-248), or a host field (3 V
+   complex, masked like vie_set_rhs). */
+int vie_batch_set_rhs_incident(vie_ctx* ctx, int i, double theta, double phi, double psi);
+int vie_batch_set_rhs(vie_ctx* ctx, int i, const double* b);
+/* Solve instances 0..nb-1 together: VIE_BATCH_PLAIN runs `method` in the
+   context's precision, VIE_BATCH_MIXED the mixed-precision BiCGSTAB of
+   vie_solve_mixed_resident (reliable updates, FP64 context).  tols / maxits
+   (may be NULL) give each instance its own tolerance and limit, else tol /
+   maxit (<= 0: V).  infos (may be NULL) receives nb records.  Returns VIE_OK,
+   or the first instance's non-OK solver status after every instance ended. */
+int vie_batch_solve(vie_ctx* ctx, int nb, int strategy, int method, double tol, int maxit, const double* tols,
+                    const int* maxits, vie_info* infos);
+int vie_batch_get_solution(vie_ctx* ctx, int i, double* x);
+/* vie_forward_scatter / vie_absorbed of instance i's solution, at its own
+   wavenumber and permittivity */
+int vie_batch_forward_scatter(vie_ctx* ctx, int i, double theta, double phi, double psi, double* value);
+int vie_batch_absorbed(vie_ctx* ctx, int i, int mat, double* value);
 
 /* Resident right-hand side computed on the device: Incident_Wave
    (vie_fft/GreenFunction.h:144-248) for angles theta, phi, psi (degrees)

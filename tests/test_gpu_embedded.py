@@ -112,3 +112,25 @@ def test_embedded_grid_rejects_active_last_plane():
     with VF.Solver(N_ODD, D, K0) as s:
         with pytest.raises(VF.VieError):
             s.set_medium_eps(eps, flags)
+
+
+def test_embedded_apply_of_unmasked_fields_matches_oracle():
+    """Caller fields that are not zero on inactive faces or on a component's
+    own-axis last plane: that plane is outside the operator on the caller's
+    grid (Current_Expansion, vie_fft/CG_Iteration.h:59-63), also where the
+    embedding makes it interior."""
+    import torch
+
+    n = (18, 14, 11)  # 18 and 11 run embedded (in 20 and 12)
+    assert not VF.supported_n(18) and not VF.supported_n(11)
+    eps, flags = _medium(n, 11)
+    op = O.Operator(n, D, K0, eps, flags, O.SCHED_FUSED)
+    x = random_field(n, 12)
+    with _solver(n, eps, flags) as s:
+        for adj in (False, True):
+            assert rel(s.apply(x, adjoint=adj), op.apply(x, adjoint=adj)) <= 1e-12
+        xt = torch.tensor(x, dtype=torch.complex128, device="cuda")
+        yt = torch.empty_like(xt)
+        s.apply_async(xt.data_ptr(), yt.data_ptr())
+        torch.cuda.synchronize()
+        assert rel(yt.cpu().numpy(), op.apply(x)) <= 1e-12

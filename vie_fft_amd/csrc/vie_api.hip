@@ -168,6 +168,23 @@ template <class R> struct Op {
     T* W2 = nullptr;
     bool own_w = true;
     MixTabs<R> mf{}, ma{}, mfs{}, mas{}, mfy{}, may{};
+    // Batches: the launch slots of batched products (the context's current
+    // slots) and, when the instances have their own operators (several
+    // wavenumbers), their tables for the windowed products' grid (pencil
+    // lengths bl[3] along x, y, z): G~ octants (gob values each), forward Vv
+    // and adjoint U tables (2 lb values per axis each), D_f forward and
+    // adjoint (3 each) and d_f (3V each), for nbt instances.
+    const Bat* bt = nullptr;
+    bool bmulti = false;
+    int nbt = 0, lb = 0, bl[3] = {0, 0, 0};
+    long gob = 0;
+    T* Gb = nullptr;
+    T* Vb[3] = {};
+    T* Uab[3] = {};
+    T* Db = nullptr;
+    T* Dab = nullptr;
+    T* dgb = nullptr;
+    MixTabs<R> mfb{}, mab{};
 
     // Length-N tables of slot a: LDS-pass twiddles, the register passes'
     // twP, U, the adjoint's V and the G~ fold (the k0-dependent Vv and Ua
@@ -222,7 +239,8 @@ template <class R> struct Op {
         dfree(fold[a]);
     }
 
-    Op(const int* nn, const double* dd, double kk, T* w1, T* w2, hipStream_t st)
+    // w1, w2: pass buffers shared with another operator, else cap slots' own
+    Op(const int* nn, const double* dd, double kk, T* w1, T* w2, hipStream_t st, int cap = 1)
     {
         for (int a = 0; a < 3; ++a) {
             n[a] = nn[a];
@@ -251,8 +269,8 @@ template <class R> struct Op {
             W2 = w2;
             own_w = false;
         } else {
-            W1 = dalloc<T>(3 * 2 * g.V);
-            W2 = dalloc<T>(3 * 4 * g.V);
+            W1 = dalloc<T>(3 * 2 * g.V * cap);
+            W2 = dalloc<T>(3 * 4 * g.V * cap);
         }
         for (int a = 0; a < 3; ++a) {
             mf.U[a] = U[a];
@@ -336,20 +354,100 @@ template <class R> struct Op {
     // conjugate is the adjoint's u), NP = 8 Nx Ny Nz of the grid.
     void axis_k0(int a, int N, double da, hipStream_t st)
     {
+        std::vector<T> h_v(2 * N), h_ua(2 * N);
+        k0_tables(N, da, k0, h_v.data(), h_ua.data());
+        copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+        copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+    }
+    // Vv (and its conjugate, the adjoint's U) of an axis of length N and cell
+    // size da for wavenumber kk
+    void k0_tables(int N, double da, double kk, T* v, T* ua) const
+    {
         const double NP = 8.0 * (double)g.V;
         const std::complex<double> J(0, 1);
-        std::vector<T> h_v(2 * N), h_ua(2 * N);
         for (int t = 0; t < 2 * N; ++t) {
             const int k = 2 * (t % N) + t / N;
             const long double th = (long double)kPi * k / N;
             const double s = (double)sinl(th), h = (double)sinl(th / 2);
             const std::complex<double> w(2.0 * h * h, s);  // 1 - e^{-j th}
-            const std::complex<double> vv = J * kZ0 / (k0 * da * NP) * w;
-            h_v[t] = hc<R>(vv);
-            h_ua[t] = hc<R>(std::conj(vv));
+            const std::complex<double> vv = J * kZ0 / (kk * da * NP) * w;
+            v[t] = hc<R>(vv);
+            ua[t] = hc<R>(std::conj(vv));
         }
-        copy_sync(Vv[a], h_v.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
-        copy_sync(Ua[a], h_ua.data(), 2 * N * sizeof(T), hipMemcpyHostToDevice, st);
+    }
+    // D_f = j kk Z0 delta_f / NP (forward; the adjoint's is its conjugate)
+    T d_of(int a, double kk) const
+    {
+        const std::complex<double> D = std::complex<double>(0, 1) * kk * kZ0 * d[a] / (8.0 * (double)g.V);
+        return hc<R>(D);
+    }
+
+    // Per-instance operator tables for cap instances on the current windows.
+    void batch_free()
+    {
+        dfree(Gb);
+        for (int a = 0; a < 3; ++a) {
+            dfree(Vb[a]);
+            dfree(Uab[a]);
+        }
+        dfree(Db);
+        dfree(Dab);
+        dfree(dgb);
+        nbt = 0;
+    }
+    void batch_alloc(int cap)
+    {
+        const int l[3] = {nxs ? nxs : n[0], nys ? nys : n[1], nzs ? nzs : n[2]};
+        if (cap <= nbt && l[0] == bl[0] && l[1] == bl[1] && l[2] == bl[2]) return;
+        batch_free();
+        for (int a = 0; a < 3; ++a) bl[a] = l[a];
+        lb = std::max(l[0], std::max(l[1], l[2]));
+        gob = (long)(l[0] + 1) * (l[1] + 1) * (l[2] + 1);
+        Gb = dalloc<T>(cap * gob);
+        for (int a = 0; a < 3; ++a) {
+            Vb[a] = dalloc<T>(2L * lb * cap);
+            Uab[a] = dalloc<T>(2L * lb * cap);
+        }
+        Db = dalloc<T>(3L * cap);
+        Dab = dalloc<T>(3L * cap);
+        dgb = dalloc<T>(3 * g.V * cap);
+        nbt = cap;
+        // the windowed products' slots of the shared (k0-free) tables
+        const int sl[3] = {nxs ? 3 : 0, nys ? 4 : 1, nzs ? 5 : 2};
+        for (int a = 0; a < 3; ++a) {
+            mfb.U[a] = U[sl[a]];
+            mfb.Vv[a] = Vb[a];
+            mfb.fold[a] = fold[sl[a]];
+            mab.U[a] = Uab[a];
+            mab.Vv[a] = Va[sl[a]];
+            mab.fold[a] = fold[sl[a]];
+        }
+        mfb.G = mab.G = Gb;
+        mfb.gconj = 0;
+        mab.gconj = 1;
+        mfb.gstr = mab.gstr = gob;
+        mfb.vstr = mab.ustr = 2 * lb;
+        mfb.ustr = mab.vstr = 0;
+        mfb.Dd = Db;
+        mab.Dd = Dab;
+    }
+    // instance i's k0 tables (host part: Vv, adjoint U, D)
+    void batch_k0(int i, double kk, hipStream_t st)
+    {
+        std::vector<T> v(2 * lb), ua(2 * lb);
+        for (int a = 0; a < 3; ++a) {
+            k0_tables(bl[a], d[a], kk, v.data(), ua.data());
+            copy_sync(Vb[a] + 2L * lb * i, v.data(), 2 * bl[a] * sizeof(T), hipMemcpyHostToDevice, st);
+            copy_sync(Uab[a] + 2L * lb * i, ua.data(), 2 * bl[a] * sizeof(T), hipMemcpyHostToDevice, st);
+        }
+        T df[3], da[3];
+        for (int a = 0; a < 3; ++a) {
+            df[a] = d_of(a, kk);
+            da[a] = df[a];
+            da[a].y = -da[a].y;
+        }
+        copy_sync(Db + 3L * i, df, sizeof(df), hipMemcpyHostToDevice, st);
+        copy_sync(Dab + 3L * i, da, sizeof(da), hipMemcpyHostToDevice, st);
     }
 
     // The wavenumber enters the tables through v_s and D_f = j k0 Z0
@@ -357,15 +455,13 @@ template <class R> struct Op {
     void set_k0(double kk, hipStream_t st)
     {
         k0 = kk;
-        const double NP = 8.0 * (double)g.V;
-        const std::complex<double> J(0, 1);
         for (int a = 0; a < 3; ++a) {
             axis_k0(a, n[a], d[a], st);
-            const std::complex<double> D = J * k0 * kZ0 * d[a] / NP;
-            mf.D[a][0] = (R)D.real();
-            mf.D[a][1] = (R)D.imag();
-            ma.D[a][0] = (R)D.real();
-            ma.D[a][1] = (R)-D.imag();
+            const T D = d_of(a, k0);
+            mf.D[a][0] = D.x;
+            mf.D[a][1] = D.y;
+            ma.D[a][0] = D.x;
+            ma.D[a][1] = -D.y;
         }
         if (nxs) axis_k0(3, nxs, d[0], st);
         if (nys) axis_k0(4, nys, d[1], st);
@@ -374,6 +470,7 @@ template <class R> struct Op {
     }
     ~Op()
     {
+        batch_free();
         for (int a = 0; a < 6; ++a) free_axis(a);
         dfree(G);
         dfree(Gs);
@@ -406,6 +503,14 @@ template <class R> struct Op {
                 gw.y1 = wy1;
             }
         }
+        // batched products are the solvers' (windowed, outputs kept zero)
+        const bool batched = bt && (bt->nb > 1 || bt->ilist);
+        if (batched && !(lin.zr && lin.wskip)) throw Err(VIE_ERR_ARG, "internal: unwindowed product in a batch");
+        if (batched) gw.bt = *bt;
+        const bool multi = batched && bmulti;
+        if (multi && (nbt == 0 || bl[0] != (nxs ? nxs : n[0]) || bl[1] != (nys ? nys : n[1]) ||
+                      bl[2] != (nzs ? nzs : n[2])))
+            throw Err(VIE_ERR_ARG, "the batch's operators were set for other windows: set them again");
         if ((gw.x0 > 0 || gw.x1 < g.nx) && !lin.wskip) {
             const long pl = (long)g.ny * g.nz;
             const unsigned gz = grid1(3 * (g.V - (gw.x1 - gw.x0) * pl));
@@ -441,7 +546,8 @@ template <class R> struct Op {
             ax = 3;
             nxp = nxs;
         }
-        const MixTabs<R>& mt = (ysh || zsh) ? (adj ? may : mfy) : ax ? (adj ? mas : mfs) : (adj ? ma : mf);
+        const MixTabs<R>& mt = multi ? (adj ? mab : mfb)
+                               : (ysh || zsh) ? (adj ? may : mfy) : ax ? (adj ? mas : mfs) : (adj ? ma : mf);
         t0(3);
         HIPCHK(run_xmix<R>(gx, nxp, W2, tw[ax], twist[ax], twP[ax], mt, done, st));
         t0(3);
@@ -449,7 +555,8 @@ template <class R> struct Op {
         HIPCHK(run_yinv<R>(gw, W2, W1, tw[ay], twist[ay], twP[ay], done, st));
         t0(4);
         Diag<R> dd{};
-        dd.d = dg;
+        dd.d = multi ? dgb : dg;
+        dd.dstr = multi ? 3 * g.V : 0;
         dd.conj = adj ? 1 : 0;
         dd.skip = lin.wskip;
         t0(5);
@@ -543,7 +650,31 @@ struct vie_ctx {
     int ahead = 4;                 // iterations enqueued per host check
     KState* snap = nullptr;        // two pinned state snapshots (one batch in flight while the other is read)
     hipEvent_t snap_ev[2] = {};
+    // batches of instances (one geometry, several right-hand sides or
+    // wavelengths): the vectors, states and pass buffers hold nbcap
+    // instances; bn instances take part in the solve being run and the
+    // batched launches serve the slots of `cur`
+    int nbcap = 1;
+    int bn = 1;
+    bool in_batch = false;         // a batched solve is being run (its records are per instance)
+    Bat cur{};
+    int* ilist = nullptr;          // launch slot -> instance (device, nbcap)
+    bool bmulti = false;           // instances have their own operators
+    std::vector<double> bk0;       // their wavenumbers
+    double* btol = nullptr;        // per-instance tolerance and limit (device)
+    int* bmaxit = nullptr;
+    std::vector<KState> bstate;    // per-instance states after a batched solve
+    std::vector<char> brhs;        // instance has a right-hand side
+    std::vector<char> bop;         // instance has its own operator (2: with its own eps table)
+    std::vector<double> btab;      // their eps tables (bnmat complex each)
+    int bnmat = 0;
+    double2* beps = nullptr;       // per-instance permittivity (V complex, post-processing scratch)
 };
+
+// Krylov state records of a batch are this many ints / complex values apart.
+static_assert(sizeof(KState) % sizeof(double2) == 0, "KState stride");
+constexpr int kStateInts = (int)(sizeof(KState) / sizeof(int));
+constexpr int kStateCx = (int)(sizeof(KState) / sizeof(double2));
 
 namespace {
 
@@ -566,7 +697,7 @@ template <class R> cx<R>* vec(vie_ctx* c, int k)
 {
     auto& v = vecs_of<R>(c);
     while ((int)v.size() <= k) v.push_back(nullptr);
-    if (!v[k]) v[k] = dalloc<cx<R>>(3 * c->V);
+    if (!v[k]) v[k] = dalloc<cx<R>>(3 * c->V * c->nbcap);
     return (cx<R>*)v[k];
 }
 
@@ -578,9 +709,11 @@ void ensure_partial(vie_ctx* c, long need)
     c->partial_cap = need;
 }
 
+// n entries per instance, for every instance of the batch being solved
 template <class R> void fill0(vie_ctx* c, cx<R>* x, long n)
 {
-    hipLaunchKernelGGL(k_fill_zero<R>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, x);
+    n *= c->bn;
+    hipLaunchKernelGGL(k_fill_zero<R>, dim3(grid1(n, NT, 16384)), dim3(NT), 0, c->stream, n, x);
     HIPCHK(hipGetLastError());
 }
 
@@ -603,27 +736,40 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
     const unsigned gsz = grid1(L.wV ? (L.whi - L.wlo) / 2 : (L.n + 1) / 2, NT, L.wV ? 1024 : 2048);
     const unsigned gy = L.wV ? 3 : 1;
     L.dots.partial = c->partial;
+    L.bt = c->cur;
     const int cls = sizeof(R) == 8 ? 6 : 16;
     if (c->timer.cls == cls) c->timer.mark(c->stream);
-    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz, gy), dim3(NT), 0, c->stream, L, done);
+    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz, gy, c->cur.nb), dim3(NT), 0, c->stream, L, done);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == cls) c->timer.mark(c->stream);
     return (long)gsz * gy;
 }
 
-void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd)
+// One workgroup per launch slot of the current batch (one outside batches);
+// the residual history is kept for single solves only.  mask (optional, one
+// flag per instance state record): instances whose flag is set are skipped.
+void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd, const int* mask = nullptr)
 {
     if (c->timer.cls == 7) c->timer.mark(c->stream);
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(NTS), 0, c->stream, op, s, c->partial, nblk, nd, c->hist);
+    const bool single = c->cur.nb == 1 && !c->cur.ilist;
+    hipLaunchKernelGGL(k_scalar, dim3(c->cur.nb), dim3(NTS), 0, c->stream, op, s, c->partial, nblk, nd,
+                       single ? c->hist : nullptr, c->cur, mask);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == 7) c->timer.mark(c->stream);
 }
 
 // pause > 0: the iterations also stop (done, not converged) once the
-// recursive residual falls below it (reliable updates)
-void state_init(vie_ctx* c, KState* s, double tol, int maxit, double pause = 0.0)
+// recursive residual falls below it (reliable updates).  In a batch every
+// instance's record is initialised; per_inst takes each instance's own
+// tolerance and limit.
+void state_init(vie_ctx* c, KState* s, double tol, int maxit, double pause = 0.0, bool per_inst = true)
 {
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit, pause);
+    if (c->in_batch) {
+        hipLaunchKernelGGL(k_state_init_n, dim3((c->bn + 63) / 64), dim3(64), 0, c->stream, s, c->bn,
+                           per_inst ? c->btol : nullptr, per_inst ? c->bmaxit : nullptr, tol, maxit, pause);
+    } else {
+        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit, pause);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -635,22 +781,32 @@ KState read_state(vie_ctx* c, KState* s)
     return h;
 }
 
+// the bn state records of a batch
+void read_states(vie_ctx* c, KState* s, std::vector<KState>& out)
+{
+    out.resize(c->bn);
+    HIPCHK(hipMemcpyAsync(out.data(), s, c->bn * sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 // Enqueue a copy of the state into pinned snapshot `slot`, read later with
 // wait_snap: the host checks one batch of iterations while the next runs.
+// Snapshots hold the bn records of a batch (slot k at snap + k nbcap).
 void snap_state(vie_ctx* c, const KState* s, int slot)
 {
     if (!c->snap) {
-        HIPCHK(hipHostMalloc((void**)&c->snap, 2 * sizeof(KState), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&c->snap, 2 * c->nbcap * sizeof(KState), hipHostMallocDefault));
         for (hipEvent_t& e : c->snap_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    HIPCHK(hipMemcpyAsync(&c->snap[slot], s, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&c->snap[slot * c->nbcap], s, c->bn * sizeof(KState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->snap_ev[slot], c->stream));
 }
 KState wait_snap(vie_ctx* c, int slot)
 {
     HIPCHK(hipEventSynchronize(c->snap_ev[slot]));
-    return c->snap[slot];
+    return c->snap[slot * c->nbcap];
 }
+const KState* snap_all(vie_ctx* c, int slot) { return &c->snap[slot * c->nbcap]; }
 
 template <class R> Dots<R> dots_with(vie_ctx* c, int n, std::initializer_list<int> ab, const cx<R>* e1,
                                      const cx<R>* e2, std::initializer_list<int> nc = {})
@@ -694,7 +850,7 @@ void green_octant(vie_ctx* c, const int* n, double2* out)
     double2* src = c->gwork;
     double2* dst = out;
     hipLaunchKernelGGL(k_green_offsets, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, n[0], n[1], n[2],
-                       d[0], d[1], d[2], c->desc.k0, src);
+                       d[0], d[1], d[2], c->desc.k0, nullptr, src);
     HIPCHK(hipGetLastError());
     // x: gwork -> out, y: out -> gwork, z: gwork -> out
     for (int ax = 0; ax < 3; ++ax) {
@@ -854,7 +1010,8 @@ template <> Op<double>* make_op<double>(vie_ctx* c)
         // an FP32 operator's work buffers are too small to share; allocate fresh
         (void)w1;
         (void)w2;
-        c->op64 = new Op<double>(c->desc.n, c->desc.d, c->desc.k0, nullptr, nullptr, c->stream);
+        c->op64 = new Op<double>(c->desc.n, c->desc.d, c->desc.k0, nullptr, nullptr, c->stream, c->nbcap);
+        c->op64->bt = &c->cur;
         if (c->green_ready)
             copy_sync(c->op64->G, c->Goff,
                       (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1) * sizeof(double2),
@@ -873,7 +1030,8 @@ template <> Op<float>* make_op<float>(vie_ctx* c)
         // the FP32 pass buffers fit in the FP64 ones: share them
         float2* w1 = c->op64 ? (float2*)c->op64->W1 : nullptr;
         float2* w2 = c->op64 ? (float2*)c->op64->W2 : nullptr;
-        c->op32 = new Op<float>(c->desc.n, c->desc.d, c->desc.k0, w1, w2, c->stream);
+        c->op32 = new Op<float>(c->desc.n, c->desc.d, c->desc.k0, w1, w2, c->stream, c->nbcap);
+        c->op32->bt = &c->cur;
         const long tot = (long)(c->desc.n[0] + 1) * (c->desc.n[1] + 1) * (c->desc.n[2] + 1);
         if (c->green_ready) {
             hipLaunchKernelGGL(k_convert<float>, dim3(grid1(tot, 256, 1L << 30)), dim3(256), 0, c->stream, tot,
@@ -904,8 +1062,11 @@ enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, NVEC }
 // batches of c->ahead iterations are in flight: the snapshot taken after one
 // is read while the next runs, so the device never waits for the host (the
 // iterations enqueued after convergence exit at once).
+template <class F> KState drive_batch(vie_ctx* c, KState* st, F iter);
+
 template <class F> KState drive(vie_ctx* c, KState* st, F iter)
 {
+    if (c->in_batch) return drive_batch(c, st, iter);
     int slot = 0;
     for (int k = 0; k < c->ahead; ++k) iter();
     snap_state(c, st, slot);
@@ -916,6 +1077,54 @@ template <class F> KState drive(vie_ctx* c, KState* st, F iter)
         slot ^= 1;
         if (h.done) return read_state(c, st);
     }
+}
+
+// Launch slots of a batch: all bn instances until the snapshots show that a
+// quarter of them have stopped, then the first m slots of a list that puts
+// the instances still iterating first (k_ilist, rebuilt on the device at the
+// start of every round, so it also places instances that resumed), m the
+// number of instances that may still iterate at the last snapshot.
+void batch_slots(vie_ctx* c, KState* st, int may_run, int* nact)
+{
+    if (c->cur.ilist || may_run <= *nact * 3 / 4) {
+        hipLaunchKernelGGL(k_ilist, dim3(1), dim3(NTS), 0, c->stream, st, c->bn, c->ilist);
+        HIPCHK(hipGetLastError());
+        c->cur.ilist = c->ilist;
+        c->cur.nb = std::max(1, may_run);
+        *nact = c->cur.nb;
+    }
+}
+// every instance of the batch in slot order (no list)
+void batch_all(vie_ctx* c)
+{
+    c->cur.nb = c->bn;
+    c->cur.ilist = nullptr;
+}
+
+// drive() for a batch: the same two rounds in flight; the solve ends when
+// every instance's record is done.  c->bstate receives the bn final records.
+template <class F> KState drive_batch(vie_ctx* c, KState* st, F iter)
+{
+    const int nb = c->bn;
+    int nact = nb, slot = 0;
+    batch_all(c);
+    for (int k = 0; k < c->ahead; ++k) iter();
+    snap_state(c, st, slot);
+    int live = nb;
+    for (;;) {
+        batch_slots(c, st, live, &nact);
+        for (int k = 0; k < c->ahead; ++k) iter();
+        snap_state(c, st, slot ^ 1);
+        wait_snap(c, slot);
+        const KState* h = snap_all(c, slot);
+        slot ^= 1;
+        live = 0;
+        for (int i = 0; i < nb; ++i) live += h[i].done ? 0 : 1;
+        if (live == 0) break;
+    }
+    batch_all(c);
+    read_states(c, st, c->bstate);
+    return c->bstate[0];
 }
 
 // BiCGSTAB (vie_fft/CG_Iteration.h:531-776) on b -> x, both of precision R.
@@ -1275,6 +1484,9 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_fill_zero<double>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->x64);
     HIPCHK(hipGetLastError());
+    // the FP64 products of x64 write only active faces (wskip): t64 starts at zero
+    hipLaunchKernelGGL(k_fill_zero<double>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->t64);
+    HIPCHK(hipGetLastError());
     // FP64 ||b||^2 for the true residual
     state_init(c, ax, 0.0, 1);
     {
@@ -1356,7 +1568,8 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
         HIPCHK(hipGetLastError());
         fill0<R>(c, x, n);
-        op64->apply(lin_k<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, &c->timer);
+        op64->apply(lin_kw<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream,
+                    &c->timer);
         LinComb<double> M{};
         M.n = n;
         M.x[0] = c->b64;
@@ -1394,6 +1607,151 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     *true_err = terr;
     *replacements = nrep;
     return h;
+}
+
+// solve_bicgstab_reliable for the bn instances of a batch at once.  The
+// decisions the single solve takes on the host -- when an instance's residual
+// is replaced, and whether it is then finished or resumes -- are taken per
+// instance on the device (k_mix_select, k_mix_decide), so one host loop
+// drives every instance: each round enqueues `ahead` FP32 iterations over the
+// instances that may still iterate and, when the last snapshot shows a
+// stopped instance that is not finished, one replacement step over the
+// stopped ones.  Each instance's arithmetic is that of its own single solve.
+// Per-instance results are left in c->bstate (terr, nrep, fin).
+void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
+{
+    using R = float;
+    Op<float>* op = make_op<float>(c);
+    Op<double>* op64 = make_op<double>(c);
+    const int nb = c->bn;
+    KState* st = c->state;
+    KState* ax = c->aux;
+    const long n = 3 * c->V, nt = n * nb;
+    float2 *b = vec<R>(c, VB), *x = vec<R>(c, VX), *r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ),
+           *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
+    const int* done = &st->done;
+    const int* sel = &ax->done;  // replacement steps: instances with ax.done == 0
+    batch_all(c);
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid1(nt, NT, 16384)), dim3(NT), 0, c->stream, nt, c->b64, b);
+    HIPCHK(hipGetLastError());
+    fill0<double>(c, c->x64, n);
+    fill0<double>(c, c->t64, n);
+    state_init(c, ax, 0.0, 1, 0.0, false);
+    {   // FP64 ||b||^2 per instance
+        LinComb<double> L{};
+        L.n = n;
+        L.x[0] = c->b64;
+        L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+        long nbk = lincomb<double>(c, L, nullptr);
+        scalar(c, ax, SC_BNRM, nbk, 1);
+    }
+    state_init(c, st, 0.0, 0, delta);  // each instance's tolerance and limit
+    fill0<R>(c, p, n);
+    fill0<R>(c, Ap, n);
+    fill0<R>(c, q, n);
+    fill0<R>(c, Aq, n);
+    {
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = b;
+        L.hc2[0][0] = 1.0;
+        L.out = x;
+        L.out2 = r;
+        L.dots = dots_with<R>(c, 1, {1, 1}, b, nullptr);
+        long nbk = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_INIT, nbk, 1);
+    }
+    auto iter = [&]() {
+        Lin<R> lp{};
+        lp.zr = op->zr;
+        lp.wskip = 1;
+        lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
+        lp.coef = &st->cf[CF_BS_P];
+        lp.out = p;
+        long nbk = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_ALPHA, nbk, 1);
+        Lin<R> lq{};
+        lq.zr = op->zr;
+        lq.wskip = 1;
+        lq.x[0] = r; lq.x[1] = Ap;
+        lq.coef = &st->cf[CF_BS_Q];
+        lq.out = q;
+        nbk = op->apply(lq, q, Aq, dots_with<R>(c, 2, {0, 1, 0, 0}, q, nullptr), false, done, c->stream, &c->timer);
+        scalar(c, st, SC_BICGSTAB_OMEGA, nbk, 2);
+        LinComb<R> L{};
+        L.n = n;
+        L.x[0] = x; L.x[1] = p; L.x[2] = q; L.x[3] = Aq;
+        L.coef = &st->cf[CF_BS_U];
+        L.out = x;
+        L.out2 = r;
+        L.wskip = 1;
+        L.dots = dots_with<R>(c, 2, {3, 3, 1, 3}, b, nullptr);
+        nbk = lincomb<R>(c, L, done);
+        scalar(c, st, SC_BICGSTAB_END, nbk, 2);
+    };
+    // residual replacement of the stopped, unfinished instances:
+    // x64 += x32 ; x32 = 0 ; r64 = b - A64 x64 ; then resume or finish
+    auto replace = [&]() {
+        const Bat keep = c->cur;
+        batch_all(c);
+        const unsigned gi = (unsigned)((nb + 63) / 64);
+        hipLaunchKernelGGL(k_mix_select, dim3(gi), dim3(64), 0, c->stream, st, ax, nb);
+        const Bat all = c->cur;
+        hipLaunchKernelGGL(k_fold64_b<float>, dim3(grid1(n, NT, 2048), nb), dim3(NT), 0, c->stream, n, all, sel, x,
+                           c->x64);
+        HIPCHK(hipGetLastError());
+        op64->apply(lin_kw<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, sel, c->stream,
+                    &c->timer);
+        LinComb<double> M{};
+        M.n = n;
+        M.x[0] = c->b64;
+        M.x[1] = c->t64;
+        M.hc[0][0] = 1.0;
+        M.hc[1][0] = -1.0;
+        M.out = c->r64;
+        M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+        long nbk = lincomb<double>(c, M, sel);
+        scalar(c, ax, SC_RELRES, nbk, 1);
+        hipLaunchKernelGGL(k_mix_decide, dim3(gi), dim3(64), 0, c->stream, st, ax, nb, delta);
+        HIPCHK(hipGetLastError());
+        // the resumed instances (ax.done still 0): load the replaced residual
+        hipLaunchKernelGGL(k_convert_b<float>, dim3(grid1(n, NT, 2048), nb), dim3(NT), 0, c->stream, n, all, sel,
+                           c->r64, r);
+        HIPCHK(hipGetLastError());
+        LinComb<R> Rr{};  // b^H r, r^H r of the replaced residual
+        Rr.n = n;
+        Rr.x[0] = r;
+        Rr.hc[0][0] = 1.0;
+        Rr.dots = dots_with<R>(c, 2, {1, 0, 0, 0}, b, nullptr);
+        nbk = lincomb<R>(c, Rr, sel);
+        scalar(c, st, SC_BICGSTAB_REPLACE, nbk, 2, sel);
+        c->cur = keep;
+    };
+    int nact = nb, slot = 0, may_run = nb;
+    bool need = false;
+    for (int k = 0; k < c->ahead; ++k) iter();
+    snap_state(c, st, slot);
+    for (;;) {
+        batch_slots(c, st, may_run, &nact);
+        for (int k = 0; k < c->ahead; ++k) iter();
+        if (need) replace();
+        snap_state(c, st, slot ^ 1);
+        wait_snap(c, slot);
+        const KState* h = snap_all(c, slot);
+        slot ^= 1;
+        int fin = 0, stopped = 0;
+        for (int i = 0; i < nb; ++i) {
+            fin += h[i].fin ? 1 : 0;
+            stopped += (h[i].done && !h[i].fin) ? 1 : 0;
+        }
+        if (fin == nb) break;
+        // an instance stopped at the older snapshot may already have been
+        // replaced by the step just enqueued; another step then selects nothing
+        need = stopped > 0;
+        may_run = nb - fin;
+    }
+    batch_all(c);
+    read_states(c, st, c->bstate);
 }
 
 template <class R> KState solve_method(vie_ctx* c, int method, const cx<R>* b, cx<R>* x, double tol, int maxit)
@@ -1608,6 +1966,395 @@ void check_last_planes(const vie_ctx* c, const unsigned char* flags)
                 }
 }
 
+// ================================================================ batches
+// Incident_Wave (vie_fft/GreenFunction.h:144-248) for wavenumber k0 and angles
+// in degrees, masked by the flags (vie_fft/CG_Iteration.h:559-562), into the
+// internal-layout field dst on the device.
+void rhs_incident(vie_ctx* c, double k0, double theta, double phi, double psi, double2* dst)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+    const double kv[3] = {k0 * std::sin(th) * std::cos(ph), k0 * std::sin(th) * std::sin(ph), k0 * std::cos(th)};
+    const double pol[3] = {std::cos(th) * std::cos(ph) * std::cos(ps) - std::sin(ph) * std::sin(ps),
+                           std::cos(th) * std::sin(ph) * std::cos(ps) + std::cos(ph) * std::sin(ps),
+                           -std::sin(th) * std::cos(ps)};
+    // wave vector and per-component factors along the internal axes
+    double ki[3];
+    double2 seg[3];
+    for (int a = 0; a < 3; ++a) {
+        const int b = c->perm[a];
+        ki[a] = kv[b];
+        const double sg = std::fabs(kv[b]) < kEps ? d[a] : 2.0 * std::sin(kv[b] * d[a] / 2.0) / kv[b];
+        seg[a] = make_double2(sg * pol[b], 0.0);
+    }
+    Op<double>* op = make_op<double>(c);
+    hipLaunchKernelGGL(k_incident, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2],
+                       ki[0], ki[1], ki[2], seg[0], seg[1], seg[2], op->flags, dst);
+    HIPCHK(hipGetLastError());
+}
+
+// A caller-layout host field b, masked by the flags, into dst (internal
+// layout); t64 and r64 are staging.
+void rhs_host(vie_ctx* c, const double* b, double2* dst)
+{
+    const long n = 3 * c->V;
+    for (long i = 0; i < 6 * c->Vu; ++i)
+        if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
+    if (c->remap) {  // caller layout staged in r64
+        copy_sync(c->r64, b, 3 * c->Vu * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_embed<double2>, dim3(grid1(n)), dim3(NT), 0, c->stream, remap_of(c), 3, c->r64, c->t64);
+        HIPCHK(hipGetLastError());
+    } else {
+        copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+    }
+    Op<double>* op = make_op<double>(c);
+    // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
+    hipLaunchKernelGGL(k_mask<double>, dim3(grid1(n, NT, 1L << 30)), dim3(NT), 0, c->stream, n, c->t64, op->flags,
+                       dst);
+    HIPCHK(hipGetLastError());
+}
+
+// FarField1's Forward_RCS (vie_fft/NearFar.h:16-109) of the field J (internal
+// layout, device) for wavenumber k0.
+double forward_scatter_of(vie_ctx* c, const double2* J, double k0, double theta, double phi, double psi)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+    const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+    const double vd[3] = {vc[c->perm[0]], vc[c->perm[1]], vc[c->perm[2]]};  // internal axes
+    const unsigned gsz = grid1(c->V, NT, 1024);
+    hipLaunchKernelGGL(k_farsum, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], k0, vd[0],
+                       vd[1], vd[2], J, c->partial);
+    HIPCHK(hipGetLastError());
+    std::vector<double2> h(3 * gsz);
+    HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::complex<double> S[3];
+    const double dV = d[0] * d[1] * d[2];
+    for (int cpt = 0; cpt < 3; ++cpt) {
+        std::complex<double> acc = 0;
+        for (unsigned b = 0; b < gsz; ++b) acc += std::complex<double>(h[cpt * gsz + b].x, h[cpt * gsz + b].y);
+        S[c->perm[cpt]] = std::polar(1.0, k0 * 0.5 * d[cpt] * vd[cpt]) * acc * dV;
+    }
+    const double tm = kZ0 * (S[0].real() * std::cos(th) * std::cos(ph) + S[1].real() * std::cos(th) * std::sin(ph) -
+                             S[2].real() * std::sin(th));
+    const double te = kZ0 * (S[0].real() * std::sin(ph) - S[1].real() * std::cos(ph));
+    return std::cos(ps) * tm - std::sin(ps) * te;
+}
+
+// NearField's Ene_Metal (vie_fft/NearFar.h:180-250) of the field J for
+// material `mat`, permittivity eps (device, V) and wavenumber k0.
+double absorbed_of(vie_ctx* c, const double2* J, const double2* eps, double k0, int mat)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    const unsigned gsz = grid1(c->V, NT, 1024);
+    const int* p = c->perm;  // the caller's extents along the internal axes
+    hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[p[0]], c->nu[p[1]],
+                       c->nu[p[2]], d[0] * d[1] * d[2], k0, kZ0, J, eps, c->mat_dev, mat, c->partial);
+    HIPCHK(hipGetLastError());
+    std::vector<double2> h(gsz);
+    HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double acc = 0;
+    for (unsigned b = 0; b < gsz; ++b) acc += h[b].x;
+    return acc;
+}
+
+// Bytes of device storage one more instance of a batch needs (solver
+// vectors of the mixed solve or of the longest plain method, pass buffers,
+// per-instance operator tables), for the capacity check.
+double batch_bytes_per_instance(const vie_ctx* c)
+{
+    const double V = (double)c->V;
+    const double s = c->desc.precision == VIE_FP64 ? 16.0 : 8.0;
+    const double fp64_fields = 4 * 3 * V * 16;           // b64, x64, t64, r64
+    const double krylov = std::max(7 * 3 * V * 8, 10 * 3 * V * s);
+    const double passes = 3 * 6 * V * (c->desc.precision == VIE_FP64 ? 16.0 : 8.0) + 3 * 6 * V * 8;
+    const double tables = 3 * V * (16 + 8);
+    return fp64_fields + krylov + passes + tables + 2 * sizeof(KState);
+}
+
+// Device storage for nb instances: solver records, resident right-hand sides
+// and solutions, Krylov vectors, pass buffers and partial sums.  Growing it
+// drops the resident right-hand sides, solutions and per-instance operators.
+void batch_reserve(vie_ctx* c, int nb)
+{
+    if (nb <= c->nbcap) return;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const double need = (nb - c->nbcap) * batch_bytes_per_instance(c);
+    if (need > 0.9 * (double)fr)
+        throw Err(VIE_ERR_OOM, "a batch of " + std::to_string(nb) + " instances needs about " +
+                                   std::to_string((long)(need / 1e6)) + " MB more device memory, " +
+                                   std::to_string((long)(fr / 1000000)) + " MB free");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long n = 3 * c->V;
+    dfree(c->state);
+    dfree(c->aux);
+    dfree(c->b64);
+    dfree(c->x64);
+    dfree(c->t64);
+    dfree(c->r64);
+    dfree(c->ilist);
+    dfree(c->btol);
+    dfree(c->bmaxit);
+    for (void*& v : c->vec64) if (v) { (void)hipFree(v); v = nullptr; }
+    for (void*& v : c->vec32) if (v) { (void)hipFree(v); v = nullptr; }
+    if (c->snap) {
+        (void)hipHostFree(c->snap);
+        c->snap = nullptr;
+    }
+    c->nbcap = nb;
+    c->rhs_ready = c->sol_ready = false;
+    c->state = dalloc<KState>(nb);
+    c->aux = dalloc<KState>(nb);
+    c->b64 = dalloc<double2>(n * nb);
+    c->x64 = dalloc<double2>(n * nb);
+    c->t64 = dalloc<double2>(n * nb);
+    c->r64 = dalloc<double2>(n * nb);
+    c->ilist = dalloc<int>(nb);
+    c->btol = dalloc<double>(nb);
+    c->bmaxit = dalloc<int>(nb);
+    // pass buffers of every slot (an FP32 operator shares the FP64 one's)
+    Op<double>* o64 = c->op64;
+    Op<float>* o32 = c->op32;
+    if (o64) {
+        dfree(o64->W1);
+        dfree(o64->W2);
+        o64->W1 = dalloc<double2>(3 * 2 * c->V * nb);
+        o64->W2 = dalloc<double2>(3 * 4 * c->V * nb);
+        o64->own_w = true;
+    }
+    if (o32) {
+        if (o32->own_w) {
+            dfree(o32->W1);
+            dfree(o32->W2);
+            o32->W1 = dalloc<float2>(3 * 2 * c->V * nb);
+            o32->W2 = dalloc<float2>(3 * 4 * c->V * nb);
+        } else {
+            o32->W1 = (float2*)o64->W1;
+            o32->W2 = (float2*)o64->W2;
+        }
+        o32->batch_free();
+    }
+    if (o64) o64->batch_free();
+    // partial sums: 3 per launch slot of the largest pass or vector launch
+    const int* ni = c->desc.n;
+    ensure_partial(c, 3L * nb * std::max<long>(std::max(zinv_blocks<double>(Geo{ni[0], ni[1], ni[2], c->V}),
+                                                          zinv_blocks<float>(Geo{ni[0], ni[1], ni[2], c->V})),
+                                                 4096));
+    c->brhs.assign(nb, 0);
+    c->bop.assign(nb, 0);
+    c->bk0.assign(nb, c->desc.k0);
+    c->bmulti = false;
+}
+
+// Operators of instances [i0, i0 + cnt): wavenumbers k0s and, for a medium
+// with material ids, permittivity tables (cnt x n_mat complex; nullptr: the
+// context's permittivity for every instance).  The G~ octants of all of them
+// come from one set of launches (one workgroup row per instance); each
+// instance's tables are computed as the single-instance path computes them.
+template <class R> void batch_tables(vie_ctx* c, Op<R>* op, int i0, int cnt, const double* k0s,
+                                     const double2* gall, const double2* epsv)
+{
+    op->batch_alloc(c->nbcap);
+    const long tot = op->gob;
+    if constexpr (sizeof(R) == 8) {
+        HIPCHK(hipMemcpyAsync(op->Gb + i0 * tot, gall, cnt * tot * sizeof(double2), hipMemcpyDeviceToDevice,
+                              c->stream));
+    } else {
+        hipLaunchKernelGGL(k_convert<float>, dim3(grid1(cnt * tot, 256, 1L << 30)), dim3(256), 0, c->stream,
+                           cnt * tot, gall, op->Gb + i0 * tot);
+        HIPCHK(hipGetLastError());
+    }
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    for (int i = 0; i < cnt; ++i) {
+        op->batch_k0(i0 + i, k0s[i], c->stream);
+        double cim[3];
+        for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / k0s[i];
+        const double2* e = epsv ? epsv + (long)i * c->V : c->eps_dev;
+        hipLaunchKernelGGL(k_diag<R>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                           cim[2], e, op->dgb + (long)(i0 + i) * 3 * c->V);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const double* eps_tables, int n_mat)
+{
+    require(c, c->medium_ready, "vie_set_medium has not been called");
+    // both operators: the mixed solve's inner FP32 one and the FP64 one of
+    // true residuals
+    Op<double>* o64 = make_op<double>(c);
+    Op<float>* o32 = make_op<float>(c);
+    o64->batch_alloc(c->nbcap);
+    o32->batch_alloc(c->nbcap);
+    const int* l = o32->bl;
+    const long tot = o32->gob;
+    // G~ octants of the windowed products' grid (l), scaled like short_octant
+    double2* w1 = dalloc<double2>(cnt * tot);
+    double2* w2 = nullptr;
+    double* kd = nullptr;
+    double2* ev = nullptr;
+    double2* tab = nullptr;
+    double* cd_ = nullptr;
+    try {
+        w2 = dalloc<double2>(cnt * tot);
+        kd = dalloc<double>(cnt);
+        copy_sync(kd, k0s, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        const int nmax = std::max(l[0], std::max(l[1], l[2]));
+        std::vector<double> ctab(3 * 2 * nmax);
+        for (int ax = 0; ax < 3; ++ax)
+            for (int j = 0; j < 2 * l[ax]; ++j) ctab[ax * 2 * nmax + j] = (double)cosl((long double)kPi * j / l[ax]);
+        cd_ = dalloc<double>(ctab.size());
+        copy_sync(cd_, ctab.data(), ctab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        const double* d = c->desc.d;
+        const dim3 gg(grid1(tot, 256, 1L << 30), cnt);
+        hipLaunchKernelGGL(k_green_offsets, gg, dim3(256), 0, c->stream, l[0], l[1], l[2], d[0], d[1], d[2], 0.0, kd,
+                           w1);
+        HIPCHK(hipGetLastError());
+        double2* src = w1;
+        double2* dst = w2;
+        for (int ax = 0; ax < 3; ++ax) {  // x: w1 -> w2, y: w2 -> w1, z: w1 -> w2
+            hipLaunchKernelGGL(k_green_cos, gg, dim3(256), 0, c->stream, l[0] + 1, l[1] + 1, l[2] + 1, ax,
+                               cd_ + ax * 2 * nmax, src, dst);
+            HIPCHK(hipGetLastError());
+            std::swap(src, dst);
+        }
+        const int* n = c->desc.n;
+        hipLaunchKernelGGL(k_scale_c64, dim3(grid1(cnt * tot, 256, 1L << 30)), dim3(256), 0, c->stream, cnt * tot,
+                           (double)n[0] * n[1] * n[2] / ((double)l[0] * l[1] * l[2]), src);
+        HIPCHK(hipGetLastError());
+        // per-instance permittivity from the tables
+        if (eps_tables) {
+            require(c, c->mat_dev != nullptr, "permittivity tables need a medium with material ids");
+            require(c, n_mat >= 1 && n_mat <= 256, "n_mat must be 1..256");
+            for (long v = 0; v < 2L * cnt * n_mat; ++v)
+                if (!std::isfinite(eps_tables[v])) throw Err(VIE_ERR_NAN, "non-finite permittivity");
+            ev = dalloc<double2>(cnt * c->V);
+            tab = dalloc<double2>(256);
+            for (int i = 0; i < cnt; ++i) {
+                std::vector<double2> t(256, make_double2(kEpsBackground, 0.0));
+                for (int m = 0; m < n_mat; ++m)
+                    t[m] = make_double2(eps_tables[2L * (i * n_mat + m)], eps_tables[2L * (i * n_mat + m) + 1]);
+                copy_sync(tab, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+                hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev, tab,
+                                   ev + (long)i * c->V);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        batch_tables<double>(c, o64, i0, cnt, k0s, src, ev);
+        batch_tables<float>(c, o32, i0, cnt, k0s, src, ev);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (eps_tables) {  // kept for the per-instance absorption
+            if ((int)c->btab.size() != 2 * c->nbcap * n_mat || c->bnmat != n_mat) {
+                c->btab.assign(2L * c->nbcap * n_mat, 0.0);
+                c->bnmat = n_mat;
+            }
+            std::copy(eps_tables, eps_tables + 2L * cnt * n_mat, c->btab.begin() + 2L * i0 * n_mat);
+        }
+    } catch (...) {
+        dfree(w1);
+        dfree(w2);
+        dfree(kd);
+        dfree(ev);
+        dfree(tab);
+        dfree(cd_);
+        throw;
+    }
+    dfree(w1);
+    dfree(w2);
+    dfree(kd);
+    dfree(ev);
+    dfree(tab);
+    dfree(cd_);
+    for (int i = 0; i < cnt; ++i) {
+        c->bk0[i0 + i] = k0s[i];
+        c->bop[i0 + i] = eps_tables ? 2 : 1;
+    }
+    c->bmulti = true;
+}
+
+// Instance i's permittivity into c->beps (device, V): its table, or the
+// context's permittivity.
+const double2* batch_eps(vie_ctx* c, int i)
+{
+    if (!c->bmulti || c->bop[i] != 2) return c->eps_dev;
+    if (!c->beps) c->beps = dalloc<double2>(c->V);
+    std::vector<double2> t(256, make_double2(kEpsBackground, 0.0));
+    for (int m = 0; m < c->bnmat; ++m)
+        t[m] = make_double2(c->btab[2L * (i * c->bnmat + m)], c->btab[2L * (i * c->bnmat + m) + 1]);
+    double2* tab = dalloc<double2>(256);
+    try {
+        copy_sync(tab, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev, tab, c->beps);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        dfree(tab);
+        throw;
+    }
+    dfree(tab);
+    return c->beps;
+}
+
+// ||b_i - A64 x_i|| / ||b_i|| for the bn instances (FP64 operators), into res.
+void true_relres_batch(vie_ctx* c, std::vector<double>& res)
+{
+    Op<double>* op = make_op<double>(c);
+    const long n = 3 * c->V;
+    KState* s = c->aux;
+    batch_all(c);
+    state_init(c, s, 0.0, 1, 0.0, false);
+    LinComb<double> L{};
+    L.n = n;
+    L.x[0] = c->b64;
+    L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+    long nb = lincomb<double>(c, L, nullptr);
+    scalar(c, s, SC_BNRM, nb, 1);
+    // the solutions vanish on inactive faces: windowed products into a zeroed t64
+    fill0<double>(c, c->t64, n);
+    op->apply(lin_kw<double>(op->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream, nullptr);
+    LinComb<double> M{};
+    M.n = n;
+    M.x[0] = c->b64;
+    M.x[1] = c->t64;
+    M.hc[0][0] = 1.0;
+    M.hc[1][0] = -1.0;
+    M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+    nb = lincomb<double>(c, M, nullptr);
+    scalar(c, s, SC_RELRES, nb, 1);
+    std::vector<KState> h;
+    read_states(c, s, h);
+    res.resize(c->bn);
+    for (int i = 0; i < c->bn; ++i) res[i] = h[i].err;
+}
+
+// Leaves batch mode on every exit path of a batched solve.
+struct BatchScope {
+    vie_ctx* c;
+    BatchScope(vie_ctx* cc, int nb) : c(cc)
+    {
+        c->bn = nb;
+        c->in_batch = true;
+        batch_all(c);
+        if (c->op64) c->op64->bmulti = c->bmulti;
+        if (c->op32) c->op32->bmulti = c->bmulti;
+    }
+    ~BatchScope()
+    {
+        c->bn = 1;
+        c->in_batch = false;
+        c->cur.nb = 1;
+        c->cur.ilist = nullptr;
+        if (c->op64) c->op64->bmulti = false;
+        if (c->op32) c->op32->bmulti = false;
+    }
+};
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -1636,6 +2383,8 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VIE_ERR_DEVICE;
     vie_ctx* c = new vie_ctx();
     c->desc = *desc;
+    c->cur.dstr = kStateInts;
+    c->cur.cstr = kStateCx;
     for (int a = 0; a < 3; ++a) {
         c->nu[a] = desc->n[a];
         c->nr[a] = c->desc.n[a] = nint[a];
@@ -1659,6 +2408,9 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
         c->t64 = dalloc<double2>(3 * c->V);
         c->r64 = dalloc<double2>(3 * c->V);
         if (c->desc.precision == VIE_FP64) make_op<double>(c); else make_op<float>(c);
+        c->brhs.assign(1, 0);
+        c->bop.assign(1, 0);
+        c->bk0.assign(1, c->desc.k0);
     } catch (const Err& e) {
         int code = e.code;
         vie_destroy(c);
@@ -1688,6 +2440,10 @@ int vie_destroy(vie_ctx* c)
     dfree(c->x64);
     dfree(c->t64);
     dfree(c->r64);
+    dfree(c->ilist);
+    dfree(c->btol);
+    dfree(c->bmaxit);
+    dfree(c->beps);
     if (c->snap) (void)hipHostFree(c->snap);
     for (hipEvent_t e : c->snap_ev)
         if (e) (void)hipEventDestroy(e);
@@ -1722,6 +2478,10 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
     if (!c->eps_dev) c->eps_dev = dalloc<double2>(c->V);
     copy_sync(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     c->medium_ready = true;
+    // per-instance operators follow the medium's windows: set them again
+    c->bop.assign(c->nbcap, 0);
+    c->bmulti = false;
+    c->brhs.assign(c->nbcap, 0);
     if (c->op64) upload_medium<double>(c, c->op64);
     if (c->op32) upload_medium<float>(c, c->op32);
     short_sync(c);
@@ -1974,30 +2734,11 @@ int vie_set_rhs_incident(vie_ctx* c, double theta, double phi, double psi)
     if (!c) return VIE_ERR_ARG;
     try {
         require(c, c->medium_ready, "vie_set_medium has not been called");
-        const int* n = c->desc.n;
-        const double* d = c->desc.d;
-        const double k0 = c->desc.k0;
-        const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
-        const double kv[3] = {k0 * std::sin(th) * std::cos(ph), k0 * std::sin(th) * std::sin(ph), k0 * std::cos(th)};
-        const double pol[3] = {std::cos(th) * std::cos(ph) * std::cos(ps) - std::sin(ph) * std::sin(ps),
-                               std::cos(th) * std::sin(ph) * std::cos(ps) + std::cos(ph) * std::sin(ps),
-                               -std::sin(th) * std::cos(ps)};
-        // wave vector and per-component factors along the internal axes
-        double ki[3];
-        double2 seg[3];
-        for (int a = 0; a < 3; ++a) {
-            const int b = c->perm[a];
-            ki[a] = kv[b];
-            const double s = std::fabs(kv[b]) < kEps ? d[a] : 2.0 * std::sin(kv[b] * d[a] / 2.0) / kv[b];
-            seg[a] = make_double2(s * pol[b], 0.0);
-        }
-        Op<double>* op = make_op<double>(c);
-        hipLaunchKernelGGL(k_incident, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2],
-                           ki[0], ki[1], ki[2], seg[0], seg[1], seg[2], op->flags, c->b64);
-        HIPCHK(hipGetLastError());
+        rhs_incident(c, c->desc.k0, theta, phi, psi, c->b64);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->rhs_ready = true;
         c->sol_ready = false;
+        c->brhs[0] = 1;
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -2009,25 +2750,11 @@ int vie_set_rhs(vie_ctx* c, const double* b)
     if (!c || !b) return VIE_ERR_ARG;
     try {
         require(c, c->medium_ready, "vie_set_medium has not been called");
-        const long n = 3 * c->V;
-        for (long i = 0; i < 6 * c->Vu; ++i)
-            if (!std::isfinite(b[i])) throw Err(VIE_ERR_NAN, "non-finite right-hand side");
-        if (c->remap) {  // caller layout staged in r64
-            copy_sync(c->r64, b, 3 * c->Vu * sizeof(double2), hipMemcpyHostToDevice, c->stream);
-            hipLaunchKernelGGL(k_embed<double2>, dim3(grid1(n)), dim3(NT), 0, c->stream, remap_of(c), 3, c->r64,
-                               c->t64);
-            HIPCHK(hipGetLastError());
-        } else {
-            copy_sync(c->t64, b, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
-        }
-        Op<double>* op = make_op<double>(c);
-        // Current_Reset of E_inc (vie_fft/CG_Iteration.h:559-562)
-        hipLaunchKernelGGL(k_mask<double>, dim3(grid1(n, NT, 1L << 30)), dim3(NT), 0, c->stream, n, c->t64, op->flags,
-                           c->b64);
-        HIPCHK(hipGetLastError());
+        rhs_host(c, b, c->b64);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->rhs_ready = true;
         c->sol_ready = false;
+        c->brhs[0] = 1;
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -2241,6 +2968,221 @@ int vie_get_solution(vie_ctx* c, double* x)
     }
 }
 
+int vie_batch_reserve(vie_ctx* c, int nb)
+{
+    if (!c || nb < 1) return VIE_ERR_ARG;
+    try {
+        batch_reserve(c, nb);
+        // back to one operator for every instance
+        c->bop.assign(c->nbcap, 0);
+        c->bmulti = false;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_capacity(vie_ctx* c, int* nb_max)
+{
+    if (!c || !nb_max) return VIE_ERR_ARG;
+    try {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const double more = 0.8 * (double)fr / batch_bytes_per_instance(c);
+        *nb_max = (int)std::min<double>(c->nbcap + std::max(0.0, more), 1 << 20);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_set_operators(vie_ctx* c, int i0, int count, const double* k0, const double* eps_tables, int n_mat)
+{
+    if (!c || i0 < 0 || count < 1 || !k0) return VIE_ERR_ARG;
+    try {
+        require(c, i0 + count <= c->nbcap, "instances beyond the reserved batch (vie_batch_reserve)");
+        for (int i = 0; i < count; ++i) require(c, k0[i] > 0 && std::isfinite(k0[i]), "k0 must be positive");
+        batch_operators(c, i0, count, k0, eps_tables, n_mat);
+        for (int i = 0; i < count; ++i) c->brhs[i0 + i] = 0;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+// the wavenumber instance i is solved at
+static double batch_k0(const vie_ctx* c, int i) { return c->bmulti ? c->bk0[i] : c->desc.k0; }
+
+int vie_batch_set_rhs_incident(vie_ctx* c, int i, double theta, double phi, double psi)
+{
+    if (!c || i < 0) return VIE_ERR_ARG;
+    try {
+        require(c, c->medium_ready, "vie_set_medium has not been called");
+        require(c, i < c->nbcap, "instance beyond the reserved batch (vie_batch_reserve)");
+        require(c, !c->bmulti || c->bop[i], "instance without an operator (vie_batch_set_operators)");
+        rhs_incident(c, batch_k0(c, i), theta, phi, psi, c->b64 + 3 * c->V * i);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->brhs[i] = 1;
+        if (i == 0) c->rhs_ready = true;
+        c->sol_ready = false;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_set_rhs(vie_ctx* c, int i, const double* b)
+{
+    if (!c || i < 0 || !b) return VIE_ERR_ARG;
+    try {
+        require(c, c->medium_ready, "vie_set_medium has not been called");
+        require(c, i < c->nbcap, "instance beyond the reserved batch (vie_batch_reserve)");
+        rhs_host(c, b, c->b64 + 3 * c->V * i);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->brhs[i] = 1;
+        if (i == 0) c->rhs_ready = true;
+        c->sol_ready = false;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_solve(vie_ctx* c, int nb, int strategy, int method, double tol, int maxit, const double* tols,
+                    const int* maxits, vie_info* infos)
+{
+    if (!c || nb < 1) return VIE_ERR_ARG;
+    try {
+        check_ready(c);
+        require(c, nb <= c->nbcap, "more instances than reserved (vie_batch_reserve)");
+        require(c, strategy == VIE_BATCH_PLAIN || strategy == VIE_BATCH_MIXED, "unknown batch strategy");
+        if (strategy == VIE_BATCH_MIXED) {
+            require(c, c->desc.precision == VIE_FP64, "mixed solve needs an FP64 context");
+        } else {
+            require(c, method >= VIE_CG && method <= VIE_BICORSTAB, "unknown method");
+        }
+        const int mdef = (int)std::min<long>(c->V, 1L << 30);
+        std::vector<double> tv(nb);
+        std::vector<int> mv(nb);
+        for (int i = 0; i < nb; ++i) {
+            require(c, c->brhs[i] != 0, "instance without a right-hand side");
+            require(c, !c->bmulti || c->bop[i], "instance without an operator (vie_batch_set_operators)");
+            tv[i] = tols ? tols[i] : tol;
+            mv[i] = maxits ? maxits[i] : maxit;
+            if (mv[i] <= 0) mv[i] = mdef;
+            require(c, tv[i] > 0, "tol must be positive");
+        }
+        copy_sync(c->btol, tv.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        copy_sync(c->bmaxit, mv.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream);
+        make_op<double>(c);  // the true FP64 residuals
+        BatchScope scope(c, nb);
+        const long n = 3 * c->V, nt = n * nb;
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<double> tres(nb);
+        if (strategy == VIE_BATCH_MIXED) {
+            solve_bicgstab_reliable_batch(c, 0.1);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < nb; ++i) tres[i] = c->bstate[i].terr;
+        } else {
+            if (c->desc.precision == VIE_FP64) {
+                solve_method<double>(c, method, c->b64, c->x64, tol, maxit);
+            } else {
+                float2* b32 = vec<float>(c, VB);
+                float2* x32 = vec<float>(c, VX);
+                hipLaunchKernelGGL(k_convert<float>, dim3(grid1(nt, NT, 16384)), dim3(NT), 0, c->stream, nt, c->b64,
+                                   b32);
+                solve_method<float>(c, method, b32, x32, tol, maxit);
+                hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(nt, NT, 16384)), dim3(NT), 0, c->stream, nt, x32,
+                                   c->x64);
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        const std::vector<KState> hs = c->bstate;
+        if (strategy != VIE_BATCH_MIXED) true_relres_batch(c, tres);
+        c->sol_ready = true;
+        int worst = VIE_OK;
+        std::string first_err;
+        for (int i = 0; i < nb; ++i) {
+            KState h = hs[i];
+            if (strategy == VIE_BATCH_MIXED) {
+                if (h.terr < tv[i]) {
+                    h.converged = 1;
+                    h.status = KS_OK;
+                } else if (h.status == KS_OK) {
+                    h.converged = 0;
+                }
+                h.err = h.terr;
+            }
+            if (infos) {
+                infos[i].iterations = h.iter;
+                infos[i].converged = h.converged;
+                infos[i].err = h.err;
+                infos[i].outer = strategy == VIE_BATCH_MIXED ? h.nrep : 0;
+                infos[i].seconds = std::chrono::duration<double>(t1 - t0).count();
+                infos[i].true_relres = tres[i];
+            }
+            const int rc = status_of(c, h);
+            if (rc != VIE_OK && worst == VIE_OK) {
+                worst = rc;
+                first_err = "instance " + std::to_string(i) + ": " + c->err;
+            }
+        }
+        if (worst != VIE_OK) c->err = first_err;
+        return worst;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_get_solution(vie_ctx* c, int i, double* x)
+{
+    if (!c || i < 0 || !x) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, i < c->nbcap, "instance beyond the reserved batch");
+        const double2* xi = c->x64 + 3 * c->V * i;
+        if (c->remap) {  // caller layout staged in t64
+            const long nu3 = 3 * c->Vu;
+            hipLaunchKernelGGL(k_crop<double2>, dim3(grid1(nu3)), dim3(NT), 0, c->stream, remap_of(c), 3, xi, c->t64);
+            HIPCHK(hipGetLastError());
+            copy_sync(x, c->t64, nu3 * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        } else {
+            copy_sync(x, xi, 3 * c->V * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+        }
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_forward_scatter(vie_ctx* c, int i, double theta, double phi, double psi, double* value)
+{
+    if (!c || i < 0 || !value) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, i < c->nbcap, "instance beyond the reserved batch");
+        *value = forward_scatter_of(c, c->x64 + 3 * c->V * i, batch_k0(c, i), theta, phi, psi);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_absorbed(vie_ctx* c, int i, int mat, double* value)
+{
+    if (!c || i < 0 || !value) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, i < c->nbcap, "instance beyond the reserved batch");
+        require(c, c->mat_dev != nullptr, "vie_batch_absorbed needs material ids (vie_set_medium)");
+        *value = absorbed_of(c, c->x64 + 3 * c->V * i, batch_eps(c, i), batch_k0(c, i), mat);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
 int vie_solve_batched(vie_ctx* c, int nrhs, int strategy, int method, double tol, int maxit, const double* B,
                       double* X, vie_info* infos)
 {
@@ -2249,21 +3191,42 @@ int vie_solve_batched(vie_ctx* c, int nrhs, int strategy, int method, double tol
         c->err = "unknown batch strategy";
         return VIE_ERR_ARG;
     }
+    if (nrhs == 0) return VIE_OK;
     const long n2 = 6 * c->Vu;  // doubles per right-hand side (caller's grid)
     int worst = VIE_OK;
-    for (int k = 0; k < nrhs; ++k) {
-        vie_info info{};
-        int rc = vie_set_rhs(c, B + k * n2);
-        if (rc == VIE_OK)
-            rc = strategy == VIE_BATCH_MIXED
-                     ? vie_solve_mixed_resident(c, VIE_MIXED_RELIABLE, VIE_BICGSTAB, tol, 0.0, maxit, &info)
-                     : vie_solve_resident(c, method, tol, maxit, &info);
-        if (rc != VIE_OK && rc != VIE_NOT_CONVERGED && rc != VIE_BREAKDOWN) return rc;  // hard error
-        const int rg = vie_get_solution(c, X + k * n2);
-        if (rg != VIE_OK) return rg;
-        if (infos) infos[k] = info;
-        if (worst == VIE_OK) worst = rc;
+    std::string first_err;
+    try {
+        // as many right-hand sides per batch as the device holds
+        int cap = 0;
+        int rc = vie_batch_capacity(c, &cap);
+        if (rc != VIE_OK) return rc;
+        const int chunk = std::max(1, std::min(nrhs, cap));
+        batch_reserve(c, chunk);
+        c->bop.assign(c->nbcap, 0);
+        c->bmulti = false;
+        for (int k0 = 0; k0 < nrhs; k0 += chunk) {
+            const int cnt = std::min(chunk, nrhs - k0);
+            for (int i = 0; i < cnt; ++i) {
+                rhs_host(c, B + (k0 + i) * n2, c->b64 + 3 * c->V * i);
+                c->brhs[i] = 1;
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+            rc = vie_batch_solve(c, cnt, strategy, method, tol, maxit, nullptr, nullptr, infos ? infos + k0 : nullptr);
+            if (rc != VIE_OK && rc != VIE_NOT_CONVERGED && rc != VIE_BREAKDOWN) return rc;  // hard error
+            if (rc != VIE_OK && worst == VIE_OK) {
+                worst = rc;
+                first_err = c->err;
+            }
+            for (int i = 0; i < cnt; ++i) {
+                const int rg = vie_batch_get_solution(c, i, X + (k0 + i) * n2);
+                if (rg != VIE_OK) return rg;
+            }
+        }
+        c->rhs_ready = true;  // instance 0 holds the last batch's first right-hand side
+    } catch (const Err& e) {
+        return fail(c, e);
     }
+    if (worst != VIE_OK) c->err = first_err;
     return worst;
 }
 
@@ -2283,29 +3246,7 @@ int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double
     if (!c || !value) return VIE_ERR_ARG;
     try {
         require(c, c->sol_ready, "no solution on the device");
-        const int* n = c->desc.n;
-        const double* d = c->desc.d;
-        const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
-        const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
-        const double vd[3] = {vc[c->perm[0]], vc[c->perm[1]], vc[c->perm[2]]};  // internal axes
-        const unsigned gsz = grid1(c->V, NT, 1024);
-        hipLaunchKernelGGL(k_farsum, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], c->desc.k0,
-                           vd[0], vd[1], vd[2], c->x64, c->partial);
-        HIPCHK(hipGetLastError());
-        std::vector<double2> h(3 * gsz);
-        HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        std::complex<double> S[3];
-        const double dV = d[0] * d[1] * d[2];
-        for (int cpt = 0; cpt < 3; ++cpt) {
-            std::complex<double> acc = 0;
-            for (unsigned b = 0; b < gsz; ++b) acc += std::complex<double>(h[cpt * gsz + b].x, h[cpt * gsz + b].y);
-            S[c->perm[cpt]] = std::polar(1.0, c->desc.k0 * 0.5 * d[cpt] * vd[cpt]) * acc * dV;
-        }
-        const double tm = kZ0 * (S[0].real() * std::cos(th) * std::cos(ph) + S[1].real() * std::cos(th) * std::sin(ph) -
-                                 S[2].real() * std::sin(th));
-        const double te = kZ0 * (S[0].real() * std::sin(ph) - S[1].real() * std::cos(ph));
-        *value = std::cos(ps) * tm - std::sin(ps) * te;
+        *value = forward_scatter_of(c, c->x64, c->desc.k0, theta, phi, psi);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
@@ -2414,20 +3355,7 @@ int vie_absorbed(vie_ctx* c, int mat, double* value)
     try {
         require(c, c->sol_ready, "no solution on the device");
         require(c, c->mat_dev != nullptr, "vie_absorbed needs material ids (vie_set_medium)");
-        const int* n = c->desc.n;
-        const double* d = c->desc.d;
-        const unsigned gsz = grid1(c->V, NT, 1024);
-        const int* p = c->perm;  // the caller's extents along the internal axes
-        hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[p[0]], c->nu[p[1]],
-                           c->nu[p[2]], d[0] * d[1] * d[2], c->desc.k0, kZ0, c->x64, c->eps_dev, c->mat_dev, mat,
-                           c->partial);
-        HIPCHK(hipGetLastError());
-        std::vector<double2> h(gsz);
-        HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        double acc = 0;
-        for (unsigned b = 0; b < gsz; ++b) acc += h[b].x;
-        *value = acc;
+        *value = absorbed_of(c, c->x64, c->eps_dev, c->desc.k0, mat);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

@@ -37,6 +37,9 @@ template <class R> struct LinComb {
     // the three components (length wV, all even) hold active faces; the
     // kernel then runs over them alone (grid.y = 3).  wV = 0: every entry.
     long wV, wlo, whi;
+    // batches (grid.z = launch slots): the vectors hold the instances n
+    // entries apart; mask and window are shared
+    Bat bt{};
 };
 
 // Active range of each z-row of the face flags (rows = 3 Nx Ny, component
@@ -62,7 +65,21 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
 {
     using T = cx<R>;
     __shared__ double2 red[NT / 64];
+    const int inst = bat_inst(L.bt, blockIdx.z);
+    done = bat_done(L.bt, done, inst);
     if (done && *done) return;
+    {   // the instance's vectors, coefficients and partial sums
+        const long o = (long)inst * L.n;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (L.x[q]) L.x[q] += o;
+        if (L.out) L.out += o;
+        if (L.out2) L.out2 += o;
+        if (L.dots.e1) L.dots.e1 += o;
+        if (L.dots.e2) L.dots.e2 += o;
+        if (L.coef) L.coef += (long)inst * L.bt.cstr;
+        L.dots.partial += (long)blockIdx.z * 3 * gridDim.x * gridDim.y;
+    }
     T c[4], c2[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -162,11 +179,19 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
 // loads one word of it alongside the partial sums (one memory latency for
 // both), thread 0 runs the step on the LDS copy, and every thread writes its
 // word back -- instead of a chain of dependent global accesses by one thread.
+// Batches: workgroup b runs the step of launch slot b's instance (partial
+// sums of slot b: 3 nblk entries after slot b - 1's), unless that instance's
+// flag in mask (nullptr: none; spaced like the done flags) is set.
 __global__ __launch_bounds__(NTS) void k_scalar(int op, KState* gs, const double2* part, long nblk,
-                                                 int nd, double* hist)
+                                                 int nd, double* hist, Bat bt, const int* mask)
 {
     constexpr int W = sizeof(KState) / sizeof(unsigned long long);
     static_assert(sizeof(KState) % sizeof(unsigned long long) == 0 && W <= NTS, "state words");
+    const int inst = bat_inst(bt, blockIdx.x);
+    mask = bat_done(bt, mask, inst);
+    if (mask && *mask) return;
+    gs += inst;
+    part += (long)blockIdx.x * 3 * nblk;
     __shared__ KState sh;
     __shared__ double2 v[4];
     __shared__ double2 red[4][NTS / 64];
@@ -254,15 +279,122 @@ __global__ void k_state_init(KState* s, double tol, int maxit, double pause)
         *s = z;
     }
 }
+// The same for nb instances: tolerance and limit tolv[i] and maxitv[i], or
+// tol and maxit for every instance when those are nullptr.
+__global__ void k_state_init_n(KState* s, int nb, const double* tolv, const int* maxitv, double tol, int maxit,
+                               double pause)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) {
+        KState z = {};
+        z.tol = tolv ? tolv[i] : tol;
+        z.maxit = maxitv ? maxitv[i] : maxit;
+        z.pause = pause;
+        s[i] = z;
+    }
+}
+
+// Launch slots: the instances whose records are not done first (in index
+// order), then the rest, so that a launch over the first m slots, m at least
+// the number not done, serves every instance still iterating.  One workgroup.
+__global__ __launch_bounds__(NTS) void k_ilist(const KState* s, int nb, int* ilist)
+{
+    __shared__ int pre[NTS + 1];
+    // thread t owns the instances [lo, hi)
+    const int per = (nb + NTS - 1) / NTS;
+    const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+    int mine = 0;
+    for (int i = lo; i < hi; ++i) mine += s[i].done ? 0 : 1;
+    pre[threadIdx.x + 1] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // prefix sums of the active counts
+        pre[0] = 0;
+        for (int t = 0; t < NTS; ++t) pre[t + 1] += pre[t];
+    }
+    __syncthreads();
+    const int nact = pre[NTS];
+    int a = pre[threadIdx.x];          // active instances before lo
+    int d = nact + (lo - a);           // slot of the first done instance at or after lo
+    for (int i = lo; i < hi; ++i) {
+        if (s[i].done) ilist[d++] = i;
+        else ilist[a++] = i;
+    }
+}
+// Mixed-precision batches (reliable updates, one FP32 BiCGSTAB per
+// instance): the instances whose FP32 iterations have stopped and that are
+// not finished get a residual replacement.  ax[i].done = 0 selects them for
+// the FP64 launches that follow (the others' workgroups exit at once).
+__global__ void k_mix_select(const KState* st, KState* ax, int nb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) ax[i].done = (st[i].done && !st[i].fin) ? 0 : 1;
+}
+// After the replacement residual of the selected instances is known
+// (ax[i].err): an instance whose true residual meets its tolerance, or whose
+// iterations ended for good (breakdown, limit), is finished; every other
+// selected one resumes its FP32 iterations with pause level delta * err, and
+// keeps ax[i].done = 0 for the launches that load its new residual.
+__global__ void k_mix_decide(KState* st, KState* ax, int nb, double delta)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb || ax[i].done) return;
+    const double t = ax[i].err;
+    st[i].nrep += 1;
+    st[i].terr = t;
+    const bool stop = st[i].status != KS_OK || st[i].iter >= st[i].maxit;
+    if (t < st[i].tol || stop || !isfinite(t)) {
+        st[i].fin = 1;
+        ax[i].done = 1;
+    } else {
+        st[i].done = 0;
+        st[i].converged = 0;
+        st[i].pause = delta * t;
+    }
+}
+
+// Batched element-wise updates over the instances of launch slots
+// (grid.y = slots, n entries per instance, done as in the pass kernels):
+// x64 += s e, e = 0 (the reliable update), and the conversions.
+template <class R>
+__global__ void k_fold64_b(long n, Bat bt, const int* done, cx<R>* e, double2* x)
+{
+    const int inst = bat_inst(bt, blockIdx.y);
+    done = bat_done(bt, done, inst);
+    if (done && *done) return;
+    e += (long)inst * n;
+    x += (long)inst * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        x[t].x += (double)e[t].x;
+        x[t].y += (double)e[t].y;
+        e[t] = czero<cx<R>>();
+    }
+}
+template <class R>
+__global__ void k_convert_b(long n, Bat bt, const int* done, const double2* in, cx<R>* out)
+{
+    const int inst = bat_inst(bt, blockIdx.y);
+    done = bat_done(bt, done, inst);
+    if (done && *done) return;
+    in += (long)inst * n;
+    out += (long)inst * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[t] = mkc((R)in[t].x, (R)in[t].y);
+}
 
 // ------------------------------------------------------------ Green's function
 // g on the (Nx+1)(Ny+1)(Nz+1) offsets (vie_fft/GreenFunction.h:53-88).
+// Several wavenumbers in one launch: with k0v, workgroup row y computes
+// k0 = k0v[y] into out + y * total.
 __global__ void k_green_offsets(int n0, int n1, int n2, double d0, double d1, double d2,
-                                double k0, double2* out)
+                                double k0, const double* k0v, double2* out)
 {
     const long total = (long)(n0 + 1) * (n1 + 1) * (n2 + 1);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
+    if (k0v) {
+        k0 = k0v[blockIdx.y];
+        out += blockIdx.y * total;
+    }
     const int k = t % (n2 + 1);
     const int j = (t / (n2 + 1)) % (n1 + 1);
     const int i = t / ((long)(n1 + 1) * (n2 + 1));
@@ -296,12 +428,15 @@ __global__ void k_green_offsets(int n0, int n1, int n2, double d0, double d1, do
 // Applied once per axis this is the forward DFT of the even extension of g
 // to 2N points, restricted to the octant 0 <= k <= N that evenness makes
 // unique (vie_fft/GreenFunction.h:91-139).  ctab[j] = cos(pi j / N), j < 2N.
+// Workgroup row y transforms array y of a batch (in, out + y * total).
 __global__ void k_green_cos(int L0, int L1, int L2, int axis, const double* ctab,
                             const double2* in, double2* out)
 {
     const long total = (long)L0 * L1 * L2;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
+    in += blockIdx.y * total;
+    out += blockIdx.y * total;
     const int c2 = t % L2, c1 = (t / L2) % L1, c0 = t / ((long)L1 * L2);
     const int L = axis == 0 ? L0 : axis == 1 ? L1 : L2;
     const int N = L - 1;
@@ -516,6 +651,10 @@ __global__ __launch_bounds__(NT) void k_embed(Remap m, int ncomp, const T* __res
             in_grid = in_grid && i[a] < m.u[m.perm[a]];
             off += i[a] * cs[m.perm[a]];
         }
+        // a field's component c leaves its own-axis last plane of the caller's
+        // grid out of the operator (Current_Expansion, vie_fft/CG_Iteration.h:
+        // 59-63); embedded, that plane is interior, so it is dropped here
+        if (ncomp == 3 && i[c] == m.u[m.perm[c]] - 1) in_grid = false;
         T r{};
         if (in_grid) r = in[off];
         out[e] = r;
@@ -535,6 +674,14 @@ __global__ __launch_bounds__(NT) void k_crop(Remap m, int ncomp, const T* __rest
 }
 
 // ------------------------------------------------------------ set-up kernels
+// Per-voxel permittivity from material ids and a table (GetMaterial's
+// Material with Real_EPR/Imag_EPR, vie_fft/Model.h:17,82-83).
+__global__ __launch_bounds__(NT) void k_eps_table(long V, const uint8_t* __restrict__ mat,
+                                                  const double2* __restrict__ tab, double2* __restrict__ eps)
+{
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) eps[v] = tab[mat[v]];
+}
+
 // Diagonal face term d_f[v] = c_f (1/(eps[v + e_f] - 1) + 1/(eps[v] - 1)) for
 // v_f < N_f - 1, else 0 (LXX_DO/LYY_DO/LZZ_DO, vie_fft/fft.h:405-588);
 // c_f = j cim[f].

@@ -73,13 +73,13 @@ static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const 
         if (zfwd_reg<R>() && !yz_lds_forced()) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
-            dim3 grid((unsigned)((rows + B - 1) / B), 3);
+            dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
             hipLaunchKernelGGL((k_zfwd_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, lin, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_z<R>(n);
-    dim3 grid((unsigned)((rows + B - 1) / B), 3);
+    dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
     hipLaunchKernelGGL((k_zfwd<R, n, B, ZS>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done);
     return hipGetLastError();
 }
@@ -93,14 +93,14 @@ static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
-            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
+            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
             hipLaunchKernelGGL((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, W2, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
-    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
+    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
     hipLaunchKernelGGL((k_yfwd<R, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done);
     return hipGetLastError();
 }
@@ -114,14 +114,14 @@ static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
-            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
+            dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
             hipLaunchKernelGGL((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
-    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3);
+    dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
     hipLaunchKernelGGL((k_yinv<R, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done);
     return hipGetLastError();
 }
@@ -136,16 +136,16 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
         if (!yz_lds_forced() && dg.d) {
             using X = ZInvReg<R, n>;
             constexpr int B = NT / X::T;
-            dim3 grid((unsigned)((rows + B - 1) / B), 3);
-            if (nblk) *nblk = (long)grid.x * grid.y;
+            dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
+            if (nblk) *nblk = 3L * grid.x;  // per launch slot
             hipLaunchKernelGGL((k_zinv_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg,
                                flags, dots, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_z<R>(n);
-    dim3 grid((unsigned)((rows + B - 1) / B), 3);
-    if (nblk) *nblk = (long)grid.x * grid.y;
+    dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
+    if (nblk) *nblk = 3L * grid.x;  // per launch slot
     hipLaunchKernelGGL((k_zinv<R, n, B, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg, flags, dots, tw, twist, done);
     return hipGetLastError();
 }
@@ -256,7 +256,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
             using X = XReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
-            dim3 grid((unsigned)(2L * g.nyp * nzt));
+            dim3 grid((unsigned)(2L * g.nyp * nzt), g.bt.nb);
             // FP32: pencils whose data lies in the first half or three quarters
             // (a window narrower than the pencil) use a kernel that knows it
             // (64-length pencils, 48-plane window: 1532 VALU instructions per
@@ -281,7 +281,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
     }
     constexpr int B = b_x<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
-    dim3 grid((unsigned)(2L * g.nyp * nzt));
+    dim3 grid((unsigned)(2L * g.nyp * nzt), g.bt.nb);
     hipLaunchKernelGGL((k_xmix<R, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done);
     return hipGetLastError();
 }

@@ -25,6 +25,25 @@ namespace vie {
 // n is y plane n + ys (ys = y0).  W1 and W2 hold 2 nzp z frequencies: nzp =
 // nz, or the short z period, whose entry n is z plane n + zs (nz and zs
 // even, so pairs of entries stay 16-byte aligned and never straddle nz).
+//
+// Batches: a launch may serve nb instances of one geometry (right-hand sides,
+// wavelengths, polarisations).  Launch slot b (workgroup row b; rows 3b..3b+2
+// for the per-component passes) serves instance ilist[b] (b without a list).
+// Field vectors hold the instances back to back (3V entries each), the pass
+// buffers W1/W2 hold the launch slots back to back, and every instance has its
+// own solver state record: its done flag is dstr ints and its coefficient
+// block cstr complex values after the previous instance's.
+struct Bat {
+    int nb = 1;
+    const int* ilist = nullptr;
+    int dstr = 0, cstr = 0;
+};
+__device__ __forceinline__ int bat_inst(const Bat& bt, int b) { return bt.ilist ? bt.ilist[b] : b; }
+__device__ __forceinline__ const int* bat_done(const Bat& bt, const int* done, int inst)
+{
+    return done ? done + (long)inst * bt.dstr : nullptr;
+}
+
 struct Geo {
     int nx, ny, nz;
     long V;
@@ -33,6 +52,7 @@ struct Geo {
     int xs = 0;
     int nyp = 0, ys = 0;
     int nzp = 0, zs = 0;
+    Bat bt{};
 };
 
 // z row of window row r, r < (x1 - x0)(y1 - y0): x0 + r / wy, y0 + r % wy
@@ -49,6 +69,9 @@ __host__ __device__ __forceinline__ int win_rows(const Geo& g) { return (g.x1 - 
 // (adjoint: U' = conj(Vv), Vv' = conj(U)), D[f] = j k0 Z0 delta_f / NP (adjoint
 // conj), fold[a][t] = position of min(k, P - k) in the parity order of the
 // Green's-function octant G (evens 0,2,..,N then odds 1,3,..).
+// Batches with one operator per instance (several wavelengths): instance i
+// reads G + i gstr, U[a] + i ustr, Vv[a] + i vstr and, with Dd set, D_f =
+// Dd[3 i + f] (strides 0: one operator for every instance).
 template <class R> struct MixTabs {
     const cx<R>* U[3];
     const cx<R>* Vv[3];
@@ -56,7 +79,30 @@ template <class R> struct MixTabs {
     const cx<R>* G;
     R D[3][2];
     int gconj;
+    long gstr = 0;
+    int ustr = 0, vstr = 0;
+    const cx<R>* Dd = nullptr;
 };
+
+// The instance's tables of a batched product.
+template <class R> struct InstTabs {
+    const cx<R>* U[3];
+    const cx<R>* Vv[3];
+    const cx<R>* G;
+    cx<R> D[3];
+};
+template <class R> __device__ __forceinline__ InstTabs<R> inst_tabs(const MixTabs<R>& mt, int inst)
+{
+    InstTabs<R> t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        t.U[a] = mt.U[a] + (long)inst * mt.ustr;
+        t.Vv[a] = mt.Vv[a] + (long)inst * mt.vstr;
+        t.D[a] = mt.Dd ? mt.Dd[3L * inst + a] : mkc(mt.D[a][0], mt.D[a][1]);
+    }
+    t.G = mt.G + inst * mt.gstr;
+    return t;
+}
 
 template <class T> __device__ __forceinline__ T czero() { return mkc(decltype(T().x)(0), decltype(T().x)(0)); }
 
@@ -67,6 +113,7 @@ template <class R> struct Diag {
     const cx<R>* d;       // per-face coefficients (3V)
     int conj;             // adjoint: use conj(d_f)
     int skip;             // y already holds zero on inactive faces: leave them unwritten
+    long dstr = 0;        // batches: instance i reads d + i dstr (0: shared)
 };
 
 // Block-wide sum of a double2; sh holds NT/64 entries.  Result valid in thread 0.
@@ -170,8 +217,11 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     __shared__ T buf[N * LD];
     __shared__ T tw[N];
     __shared__ T tws[N];
+    // component s of instance inst (field entries of component sb)
+    const int s = blockIdx.y % 3, inst = bat_inst(g.bt, blockIdx.y / 3);
+    const long sb = 3L * inst + s;
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
-    const int s = blockIdx.y;
     // z row length and the z plane of entry 0: without ZS the pencil is the
     // whole row (N = nz, entry n is plane n), fixed at compile time
     const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
@@ -190,7 +240,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
         zrs[t] = lin.zr && ok ? lin.zr[s * rows + rw[t]] : ((unsigned)nzr << 16);
     }
     T c[3];
-    load_coef<R>(lin.coef, c);
+    load_coef<R>(lin.coef ? lin.coef + (long)inst * g.bt.cstr : nullptr, c);
     __syncthreads();
     if constexpr (N % 2 == 0) {
         // all loads first, then the combination, stores and LDS writes
@@ -207,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
             const int z = n + zs;  // z plane of entry n
             if (rok && z + 1 >= (int)(rg & 0xffffu) && z < (int)(rg >> 16)) {
-                const long idx = s * g.V + row * nzr + z;
+                const long idx = sb * g.V + row * nzr + z;
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
                 if (lin.x[2]) ld2(lin.x[2] + idx, x2[it][0], x2[it][1]);
@@ -222,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             T v0 = czero<T>(), v1 = czero<T>();
             const int z = n + zs;
             if (r0 + col < nw) {
-                const long idx = s * g.V + row * nzr + z;
+                const long idx = sb * g.V + row * nzr + z;
                 v0 = cmul(c[0], x0[it][0]);
                 v1 = cmul(c[0], x0[it][1]);
                 if (lin.x[1]) {
@@ -253,7 +303,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             T v = czero<T>();
             const int z = n + zs;
             if (r0 + col < nw) {
-                const long idx = s * g.V + row * nzr + z;
+                const long idx = sb * g.V + row * nzr + z;
                 const unsigned rg = zrs[col];
                 if (z >= (int)(rg & 0xffffu) && z < (int)(rg >> 16)) {
                     v = cmul(c[0], lin.x[0][idx]);
@@ -279,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
         const long row = rw[col];
         if (r0 + col < nw) {
             const int ep = t >= N, m = t - ep * N;
-            T* dst = W1 + (s * rows + row) * (2 * N) + t;
+            T* dst = W1 + ((long)blockIdx.y * rows + row) * (2 * N) + t;
             if constexpr (N % 2 == 0)
                 st2(dst, buf[m * LD + ep * B + col], buf[(m + 1) * LD + ep * B + col]);
             else {
@@ -306,8 +356,9 @@ __global__ __launch_bounds__(NT) void k_yfwd(Geo g, const cx<R>* __restrict__ W1
     __shared__ T buf[N * LD];
     __shared__ T tw[N];
     __shared__ T tws[N];
+    done = bat_done(g.bt, done, bat_inst(g.bt, blockIdx.y / 3));
     if (done && *done) return;
-    const int s = blockIdx.y;
+    const int s = blockIdx.y;  // launch slot (component of one instance) of W1 and W2
     const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -359,27 +410,31 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     __shared__ T uz[B];
     __shared__ T vz[B];
     __shared__ int fz[B];
+    const int inst = bat_inst(g.bt, blockIdx.y);  // launch slot blockIdx.y of W2
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
+    const InstTabs<R> it = inst_tabs(mt, inst);
     const int P1 = 2 * g.nyp, P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
+    W2 += (long)blockIdx.y * 3 * g.nx * P1 * P2;
     for (int t = threadIdx.x; t < N; t += NT) {
         tw[t] = twg[t];
         tws[t] = twistg[t];
     }
     for (int t = threadIdx.x; t < 2 * N; t += NT) {
-        ux[t] = mt.U[0][t];
-        vx[t] = mt.Vv[0][t];
+        ux[t] = it.U[0][t];
+        vx[t] = it.Vv[0][t];
         fx[t] = mt.fold[0][t];
     }
     for (int t = threadIdx.x; t < B; t += NT) {
         const bool ok = z0 + t < P2;
-        uz[t] = ok ? mt.U[2][z0 + t] : czero<T>();
-        vz[t] = ok ? mt.Vv[2][z0 + t] : czero<T>();
+        uz[t] = ok ? it.U[2][z0 + t] : czero<T>();
+        vz[t] = ok ? it.Vv[2][z0 + t] : czero<T>();
         fz[t] = ok ? mt.fold[2][z0 + t] : 0;
     }
-    const T uy = mt.U[1][ty], vy = mt.Vv[1][ty];
+    const T uy = it.U[1][ty], vy = it.Vv[1][ty];
     const long fyoff = (long)mt.fold[1][ty] * (g.nzp + 1);
     const long gs = (long)(g.nyp + 1) * (g.nzp + 1);
     __syncthreads();
@@ -394,9 +449,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
     }
     __syncthreads();
     fft_lds<T, N, COLS, LD, -1>(buf, tw);
-    const T D0 = mkc(mt.D[0][0], mt.D[0][1]);
-    const T D1 = mkc(mt.D[1][0], mt.D[1][1]);
-    const T D2 = mkc(mt.D[2][0], mt.D[2][1]);
+    const T D0 = it.D[0], D1 = it.D[1], D2 = it.D[2];
     // Y_f = G (D_f X_f + U_f sum_s Vv_s X_s)
     for (int e = threadIdx.x; e < 2 * N * B; e += NT) {
         const int ep = e / (N * B), r = e % (N * B);
@@ -404,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_xmix(Geo g, cx<R>* __restrict__ W2,
         const int tx = ep * N + m;
         T* p0 = &buf[m * LD + ep * B + col];
         const T X0 = p0[0], X1 = p0[2 * B], X2 = p0[4 * B];
-        T Gv = mt.G[(long)fx[tx] * gs + fyoff + fz[col]];
+        T Gv = it.G[(long)fx[tx] * gs + fyoff + fz[col]];
         if (mt.gconj) Gv = cconj(Gv);
         const T sig = cadd(cadd(cmul(vx[tx], X0), cmul(vy, X1)), cmul(vz[col], X2));
         p0[0] = cmul(Gv, cadd(cmul(D0, X0), cmul(ux[tx], sig)));
@@ -438,8 +491,9 @@ __global__ __launch_bounds__(NT) void k_yinv(Geo g, const cx<R>* __restrict__ W2
     __shared__ T buf[N * LD];
     __shared__ T tw[N];
     __shared__ T tws[N];
+    done = bat_done(g.bt, done, bat_inst(g.bt, blockIdx.y / 3));
     if (done && *done) return;
-    const int s = blockIdx.y;
+    const int s = blockIdx.y;  // launch slot (component of one instance) of W1 and W2
     const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -483,8 +537,12 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     __shared__ T tw[N];
     __shared__ T tws[N];
     __shared__ double2 red[NT / 64];
+    // component f of instance inst (field entries of component fb)
+    const int f = blockIdx.y % 3, inst = bat_inst(g.bt, blockIdx.y / 3);
+    const long fb = 3L * inst + f;
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
-    const int f = blockIdx.y;
+    const cx<R>* dgd = dg.d + inst * dg.dstr;
     // z row length and the z plane of entry 0: without ZS the pencil is the
     // whole row (N = nz, entry n is plane n), fixed at compile time
     const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
@@ -502,7 +560,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         const int col = e / N, t = 2 * (e % N);
         const long row = rw[col];
         T v0 = czero<T>(), v1 = czero<T>();
-        if (r0 + col < nw) ld2(W1 + (f * rows + row) * (2 * N) + t, v0, v1);
+        if (r0 + col < nw) ld2(W1 + ((long)blockIdx.y * rows + row) * (2 * N) + t, v0, v1);
         const int ep = t >= N, m = t - ep * N;
         buf[m * LD + ep * B + col] = v0;
         const int t1 = t + 1, ep1 = t1 >= N, m1 = t1 - ep1 * N;
@@ -512,23 +570,27 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     fft_lds<T, N, COLS, LD, +1>(buf, tw);
     double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
     const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
-    auto term = [&](long idx, T out) {  // y = flag (out + d x), then its dot products
+    // ia: entry of the instance's field vectors; idx - ia is the same entry
+    // of the shared flags (and of a shared diagonal)
+    const long ioff = (fb - f) * g.V;
+    auto term = [&](long ia, T out) {  // y = flag (out + d x), then its dot products
+        const long idx = ia - ioff;
         const bool on = flags[idx] != 0;
         T yv = czero<T>();
         if (on) {  // d and x are read only where the face is active
-            const T dv = dg.conj ? cconj(dg.d[idx]) : dg.d[idx];
-            yv = cadd(out, cmul(dv, src[idx]));
+            const T dv = dg.conj ? cconj(dgd[idx]) : dgd[idx];
+            yv = cadd(out, cmul(dv, src[ia]));
         }
-        if (on || !dg.skip) y[idx] = yv;
+        if (on || !dg.skip) y[ia] = yv;
         const bool rd = on || !dots.masked;
-        dots_add(acc, dots, yv, need1 && rd ? dots.e1[idx] : czero<T>(), need2 && rd ? dots.e2[idx] : czero<T>());
+        dots_add(acc, dots, yv, need1 && rd ? dots.e1[ia] : czero<T>(), need2 && rd ? dots.e2[ia] : czero<T>());
     };
     if constexpr (N % 2 == 0) {
         for (int e = threadIdx.x; e < B * N / 2; e += NT) {
             const int col = e / (N / 2), k = 2 * (e % (N / 2));
             const long row = rw[col];
             if (r0 + col < nw && k + zs < nzr) {  // entry k: z plane k + zs (pairs in or out together)
-                const long idx = f * g.V + row * nzr + zs + k;
+                const long idx = fb * g.V + row * nzr + zs + k;
                 term(idx, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
                 term(idx + 1, cadd(buf[(k + 1) * LD + col], cmulc(buf[(k + 1) * LD + B + col], tws[k + 1])));
             }
@@ -538,17 +600,19 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
             const int col = e / N, k = e % N;
             const long row = rw[col];
             if (r0 + col < nw && k + zs < nzr)
-                term(f * g.V + row * nzr + zs + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
+                term(fb * g.V + row * nzr + zs + k, cadd(buf[k * LD + col], cmulc(buf[k * LD + B + col], tws[k])));
         }
     }
     if (dots.n > 0) {
-        const long nblk = (long)gridDim.x * gridDim.y;
-        const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        // per launch slot: 3 arrays of the slot's 3 gridDim.x partial sums
+        const long nblk = 3L * gridDim.x;
+        const long blk = (long)f * gridDim.x + blockIdx.x;
+        double2* part = dots.partial + (long)(blockIdx.y / 3) * 3 * nblk;
         #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (d >= dots.n) break;
             double2 s = block_sum(acc[d], red);
-            if (threadIdx.x == 0) dots.partial[d * nblk + blk] = s;
+            if (threadIdx.x == 0) part[d * nblk + blk] = s;
         }
     }
 }

@@ -23,6 +23,10 @@ struct KState {
     double pause;         // recursive residual below which the iterations stop for a residual replacement (0: never)
     int iter, maxit, done, converged, status;
     int fail_op;          // scalar program that stopped the solve, if any
+    // mixed-precision batches: the instance is finished, its residual
+    // replacements and its last true FP64 residual
+    int fin, nrep;
+    double terr;
 };
 
 enum { KS_OK = 0, KS_BREAKDOWN = 5, KS_NAN = 6 };

@@ -260,18 +260,21 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     __shared__ T_ tw[P];
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
     __shared__ T_ gsl[GSL ? (N + 1) * B : 1];    // G octant slice [fx][col]
+    const int inst = bat_inst(g.bt, blockIdx.y);  // launch slot blockIdx.y of W2
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
+    const InstTabs<R> it = inst_tabs(mt, inst);
     const int P1 = 2 * g.nyp, P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / nzt, z0 = (tile % nzt) * B;
     const int col = threadIdx.x % B, t = threadIdx.x / B;
     const bool colok = z0 + col < P2;
-    // element offsets fit 32 bits (the host checks 3 x 4V < 2^31 before
-    // launching), so the per-access address arithmetic stays 32-bit
+    // element offsets within a slot fit 32 bits (the host checks 3 x 4V <
+    // 2^31 before launching), so the per-access address arithmetic stays 32-bit
     const unsigned rowst = (unsigned)P1 * P2;  // stride between x rows
-    T_* base = W2 + ((long)g.xs * P1 + ty) * P2 + z0 + col;  // pencil entry 0: plane xs
     const unsigned cst = (unsigned)g.nx * rowst;  // stride between components
+    T_* base = W2 + (long)blockIdx.y * 3 * cst + ((long)g.xs * P1 + ty) * P2 + z0 + col;  // entry 0: plane xs
 
     // issue the pencil loads first, then the tables, so their latencies overlap
     T_ v[3][E];
@@ -294,21 +297,21 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
         for (int e = threadIdx.x; e < (N + 1) * B; e += NT) {
             const int fxe = e / B, ce = e % B;
             const int tze = z0 + ce < P2 ? z0 + ce : z0;
-            T_ gv = mt.G[(long)fxe * gs + gy + fold_split(tze, g.nzp)];
+            T_ gv = it.G[(long)fxe * gs + gy + fold_split(tze, g.nzp)];
             gsl[e] = mt.gconj ? cconj(gv) : gv;
         }
     }
     for (int e = threadIdx.x; e < P; e += NT) {
         tw[e] = twP[e];
         const int ts = (e & 1) * N + (e >> 1);  // split-order index of frequency e
-        uv[e] = mt.U[0][ts];
-        uv[P + e] = mt.Vv[0][ts];
+        uv[e] = it.U[0][ts];
+        uv[P + e] = it.Vv[0][ts];
     }
-    const T_ uy = mt.U[1][ty], vy = mt.Vv[1][ty];
+    const T_ uy = it.U[1][ty], vy = it.Vv[1][ty];
     const int tz = colok ? z0 + col : z0;
-    const T_ uz = mt.U[2][tz], vz = mt.Vv[2][tz];
+    const T_ uz = it.U[2][tz], vz = it.Vv[2][tz];
     // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
-    const T_* gcol = mt.G + (long)fold_split(ty, g.nyp) * (g.nzp + 1) + fold_split(tz, g.nzp);
+    const T_* gcol = it.G + (long)fold_split(ty, g.nyp) * (g.nzp + 1) + fold_split(tz, g.nzp);
     const unsigned gs = (unsigned)(g.nyp + 1) * (g.nzp + 1);
     auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
         const int kf = kx < P - kx ? kx : P - kx;
@@ -325,7 +328,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     x_forward<T_, C, PL, OPQ>(v, lds, tw, t, col);
 
     // D_f = j k0 Z0 delta_f / NP is imaginary: D_f X = (-Im D_f X.y, Im D_f X.x)
-    const R D0 = mt.D[0][1], D1 = mt.D[1][1], D2 = mt.D[2][1];
+    const R D0 = it.D[0].y, D1 = it.D[1].y, D2 = it.D[2].y;
     // last forward stage: register e = b rL + q holds frequency i + q P/rL, i = t + T b
 #pragma unroll
     for (int b = 0; b < E / rL; ++b) {
@@ -412,8 +415,11 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
+    // component s of instance inst (field entries of component sb)
+    const int s = blockIdx.y % 3, inst = bat_inst(g.bt, blockIdx.y / 3);
+    const long sb = 3L * inst + s;
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
-    const int s = blockIdx.y;
     // z row length and the z plane of entry 0: without ZS the pencil is the
     // whole row (N = nz, entry n is plane n), fixed at compile time
     const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
@@ -424,10 +430,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     const bool rowok = r < win_rows(g);
     const long row = rowok ? win_row(g, r) : 0;
     T_ c[3];
-    load_coef<R>(lin.coef, c);
+    load_coef<R>(lin.coef ? lin.coef + (long)inst * g.bt.cstr : nullptr, c);
     const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
     // entry n is z plane n + zs of the row
-    const long base = s * g.V + row * nzr + zs;
+    const long base = sb * g.V + row * nzr + zs;
     // z range outside which the inputs vanish (the whole row without lin.zr)
     int lo = 0, hi = nzr;
     if (lin.zr && rowok) {
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
     __syncthreads();
     r_forward<T_, C, PL>(v, lds, tw, t, col);
     if (rowok) {
-        T_* out = W1 + (s * rows + row) * (2 * N);
+        T_* out = W1 + ((long)blockIdx.y * rows + row) * (2 * N);
 #pragma unroll
         for (int b = 0; b < E / rL; ++b)
 #pragma unroll
@@ -493,8 +499,11 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
     __shared__ double2 red[NT / 64];
+    // component f of instance inst (field entries of component fb)
+    const int f = blockIdx.y % 3, inst = bat_inst(g.bt, blockIdx.y / 3);
+    const long fb = 3L * inst + f;
+    done = bat_done(g.bt, done, inst);
     if (done && *done) return;
-    const int f = blockIdx.y;
     // z row length and the z plane of entry 0: without ZS the pencil is the
     // whole row (N = nz, entry n is plane n), fixed at compile time
     const int nzr = ZS ? g.nz : N, zs = ZS ? g.zs : 0;
@@ -506,7 +515,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     const long row = rowok ? win_row(g, r) : 0;
     T_ v[E];
     {
-        const T_* in = W1 + (f * rows + (rowok ? row : 0)) * (2 * N);
+        const T_* in = W1 + ((long)blockIdx.y * rows + (rowok ? row : 0)) * (2 * N);
 #pragma unroll
         for (int b = 0; b < E / rL; ++b)
 #pragma unroll
@@ -519,8 +528,11 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
     uint8_t fl[E];
     const bool need1 = dots_use(dots, 1), need2 = dots_use(dots, 2);
     const bool e1src = dots.e1 == src;
-    // entry n is z plane n + zs of the row (entries past Nz are not on the grid)
+    // entry n is z plane n + zs of the row (entries past Nz are not on the grid);
+    // base indexes the shared flags (and a shared diagonal), bb the instance's vectors
     const long base = f * g.V + (rowok ? row : 0) * nzr + zs;
+    const long bb = base + (fb - f) * g.V;
+    const cx<R>* dgd = dg.d + inst * dg.dstr;
     if (rowok) {
         // flags first: x and d are read only on active faces (the output is
         // zero elsewhere), masked dot operands likewise
@@ -538,13 +550,13 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1), e = b * r1 + q;
                 if (n < N && n + zs < nzr) {
-                    const long idx = base + n;
+                    const long ia = bb + n;
                     sv[e] = dv[e] = e1v[e] = e2v[e] = czero<T_>();
-                    if (fl[e] || (!dots.masked && need1 && e1src)) sv[e] = src[idx];
-                    if (fl[e]) dv[e] = dg.d[idx];
+                    if (fl[e] || (!dots.masked && need1 && e1src)) sv[e] = src[ia];
+                    if (fl[e]) dv[e] = dgd[base + n];
                     if (fl[e] || !dots.masked) {
-                        if (need1 && !e1src) e1v[e] = dots.e1[idx];
-                        if (need2) e2v[e] = dots.e2[idx];
+                        if (need1 && !e1src) e1v[e] = dots.e1[ia];
+                        if (need2) e2v[e] = dots.e2[ia];
                     }
                 }
             }
@@ -562,10 +574,10 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1), e = b * r1 + q;
                 if (n < N && n + zs < nzr) {
-                    const long idx = base + n;
+                    const long ia = bb + n;
                     T_ yv = czero<T_>();
                     if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
-                    if (fl[e] || !dg.skip) y[idx] = yv;
+                    if (fl[e] || !dg.skip) y[ia] = yv;
                     dots_add(acc, dots, yv, e1src ? sv[e] : need1 ? e1v[e] : czero<T_>(),
                              need2 ? e2v[e] : czero<T_>());
                 }
@@ -573,13 +585,15 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
         }
     }
     if (dots.n > 0) {
-        const long nblk = (long)gridDim.x * gridDim.y;
-        const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        // per launch slot: 3 arrays of the slot's 3 gridDim.x partial sums
+        const long nblk = 3L * gridDim.x;
+        const long blk = (long)f * gridDim.x + blockIdx.x;
+        double2* part = dots.partial + (long)(blockIdx.y / 3) * 3 * nblk;
         #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (d >= dots.n) break;
             double2 sm = block_sum(acc[d], red);
-            if (threadIdx.x == 0) dots.partial[d * nblk + blk] = sm;
+            if (threadIdx.x == 0) part[d * nblk + blk] = sm;
         }
     }
 }
@@ -596,8 +610,9 @@ __global__ __launch_bounds__(NT) void k_yfwd_r(Geo g, const cx<R>* __restrict__ 
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
+    done = bat_done(g.bt, done, bat_inst(g.bt, blockIdx.y / 3));
     if (done && *done) return;
-    const int s = blockIdx.y;
+    const int s = blockIdx.y;  // launch slot (component of one instance) of W1 and W2
     const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -641,8 +656,9 @@ __global__ __launch_bounds__(NT) void k_yinv_r(Geo g, const cx<R>* __restrict__ 
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     __shared__ T_ lds[C::LDS];
     __shared__ T_ tw[P];
+    done = bat_done(g.bt, done, bat_inst(g.bt, blockIdx.y / 3));
     if (done && *done) return;
-    const int s = blockIdx.y;
+    const int s = blockIdx.y;  // launch slot (component of one instance) of W1 and W2
     const int P2 = 2 * g.nzp;
     const int nzt = (P2 + B - 1) / B;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);

@@ -10,11 +10,12 @@ permittivities and G~ are updated in place (``vie_set_k0``,
 right-hand side changes.  Permittivity tables are read in the format of the
 reference's ``data/*_epr.txt`` (``vie_fft/ReadFile.h:23-45``).
 
-    python -m vie_fft_amd.sweep [--count 64] [--n 96] [--fp64] [--out DIR]
+    python -m vie_fft_amd.sweep [--count 64] [--n 96] [--fp64] [--out DIR] [--sequential]
 
 runs the 96^3 Au-dimer sweep (Drude permittivity, 64 wavelengths from 400 to
-800 nm, two polarisations) and prints one JSON line per right-hand side and a
-summary line.
+800 nm, two polarisations) as one batched solve (``run_sweep_batched``; with
+--sequential one solve after the other, ``run_sweep``) and prints one JSON line
+per right-hand side and a summary line.
 """
 from __future__ import annotations
 
@@ -154,6 +155,71 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
     return results
 
 
+def run_sweep_batched(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=1, active=None, maxit=0,
+                      out_dir=None, log=None, chunk=None, timings=None):
+    """run_sweep with every point an instance of one batched solve.
+
+    One context holds the geometry; each (wavelength_nm, psi) point becomes an
+    instance with its own wavenumber and permittivity table
+    (``vie_batch_set_operators``: G~ for all wavelengths in one set of
+    launches) and its own masked incident field, and the instances are solved
+    together (``vie_batch_solve``) in batches of at most `chunk` (default: as
+    many as the device holds).  Same result records as run_sweep; solve_s and
+    setup_s are the batch's times shared evenly.  `timings` (a dict) receives
+    the set-up, solve and post-processing seconds.
+    """
+    scenes = [scene_at(lam, psi) for lam, psi in points]
+    sc0 = scenes[0]
+    results = []
+    wl_index = {}
+    t_setup = t_solve = t_post = 0.0
+    with VF.Solver(sc0.n, sc0.d, sc0.k0, VF.FP64) as s:
+        material, flags = sc0.material(), sc0.flags()
+        vol_active = active_face_volume(material, active, sc0.d) if active is not None else 0.0
+        t0 = time.perf_counter()
+        s.set_medium(material, sc0.eps_table, flags)
+        s.green_compute()
+        t_setup += time.perf_counter() - t0
+        plan = s.plan()
+        cap = chunk or max(1, min(len(points), s.batch_capacity()))
+        for b0 in range(0, len(points), cap):
+            part = scenes[b0:b0 + cap]
+            nb = len(part)
+            t0 = time.perf_counter()
+            s.batch_reserve(nb)
+            s.batch_set_operators(0, [sc.k0 for sc in part], np.stack([sc.eps_table for sc in part]))
+            for i, (sc, (lam, psi)) in enumerate(zip(part, points[b0:b0 + nb])):
+                s.batch_set_rhs_incident(i, sc.theta, sc.phi, psi)
+            t1 = time.perf_counter()
+            _, infos = s.batch_solve(nb, method=method, tol=tol, maxit=maxit, mixed=mixed)
+            t2 = time.perf_counter()
+            for i, (sc, (lam, psi)) in enumerate(zip(part, points[b0:b0 + nb])):
+                q_ext = s.batch_forward_scatter(i, sc.theta, sc.phi, psi) / sc.area
+                q_abs = s.batch_absorbed(i, metal) * 120.0 * np.pi / sc.area
+                act = 3.0 * s.batch_absorbed(i, active) / vol_active if active is not None and vol_active > 0 else 0.0
+                info = infos[i]
+                r = {"wavelength_nm": float(lam), "psi": float(psi), "q_ext": q_ext, "q_abs": q_abs,
+                     "active_density": act, "iterations": info["iterations"], "outer": info["outer"],
+                     "true_relres": info["true_relres"], "converged": bool(info["converged"]),
+                     "solve_s": (t2 - t1) / nb, "setup_s": (t1 - t0) / nb}
+                if b0 + i == 0:
+                    r["plan"] = plan
+                results.append(r)
+                if log:
+                    log(r)
+                if out_dir is not None and lam not in wl_index:
+                    wl_index[lam] = len(wl_index)
+                    write_point_files(out_dir, wl_index[lam], lam, q_ext, q_abs, act, sc.n, sc.d)
+            t_setup += t1 - t0
+            t_solve += t2 - t1
+            t_post += time.perf_counter() - t2
+    if out_dir is not None:
+        merge_point_files(out_dir, len(wl_index))
+    if timings is not None:
+        timings.update({"setup_s": t_setup, "solve_s": t_solve, "post_s": t_post, "chunk": cap})
+    return results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=64, help="wavelengths from 400 to 800 nm")
@@ -161,11 +227,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--fp64", action="store_true", help="plain FP64 solves instead of mixed precision")
     ap.add_argument("--out", default=None, help="directory for the reference-format result files")
+    ap.add_argument("--sequential", action="store_true", help="one solve after the other instead of one batch")
     args = ap.parse_args()
     points = G.dimer_sweep(args.count)
     t0 = time.perf_counter()
-    res = run_sweep(lambda lam, psi: G.dimer_case(lam, psi=psi, n=args.n), points, tol=args.tol,
-                    mixed=not args.fp64, out_dir=args.out, log=lambda r: print(json.dumps(r), flush=True))
+    run = run_sweep if args.sequential else run_sweep_batched
+    res = run(lambda lam, psi: G.dimer_case(lam, psi=psi, n=args.n), points, tol=args.tol,
+              mixed=not args.fp64, out_dir=args.out, log=lambda r: print(json.dumps(r), flush=True))
     wall = time.perf_counter() - t0
     its = np.array([r["iterations"] for r in res])
     print(json.dumps({"workload": f"{args.n}^3 Au dimer, {args.count} wavelengths x 2 polarisations",

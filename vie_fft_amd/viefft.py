@@ -33,6 +33,9 @@ EXPORTS = [
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
     "vie_model_bytes_matvec", "vie_plan", "vie_absorption", "vie_absorption_grad",
     "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
+    "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
+    "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
+    "vie_batch_absorbed",
 ]
 
 
@@ -108,6 +111,16 @@ def load():
     L.vie_solve_batched.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_int, dp, dp,
                                     ct.POINTER(Info)]
     L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
+    ip = ct.POINTER(ct.c_int)
+    L.vie_batch_reserve.argtypes = [vp, ct.c_int]
+    L.vie_batch_capacity.argtypes = [vp, ip]
+    L.vie_batch_set_operators.argtypes = [vp, ct.c_int, ct.c_int, dp, dp, ct.c_int]
+    L.vie_batch_set_rhs_incident.argtypes = [vp, ct.c_int, ct.c_double, ct.c_double, ct.c_double]
+    L.vie_batch_set_rhs.argtypes = [vp, ct.c_int, dp]
+    L.vie_batch_solve.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_int, dp, ip, ct.POINTER(Info)]
+    L.vie_batch_get_solution.argtypes = [vp, ct.c_int, dp]
+    L.vie_batch_forward_scatter.argtypes = [vp, ct.c_int, ct.c_double, ct.c_double, ct.c_double, dp]
+    L.vie_batch_absorbed.argtypes = [vp, ct.c_int, ct.c_int, dp]
     _lib = L
     return L
 
@@ -305,6 +318,60 @@ class Solver:
         self._check(self.lib.vie_absorption_grad(self.h, int(method), float(tol), int(maxit), ct.byref(v),
                                                  _dp(gre), _dp(gim), ct.byref(info)), allow=(OK, NOT_CONVERGED))
         return v.value, gre, gim, info.as_dict()
+
+    # -- batches of instances (one geometry, several wavelengths / polarisations)
+    def batch_reserve(self, nb):
+        """Device storage for nb instances solved together (one operator for all)."""
+        self._check(self.lib.vie_batch_reserve(self.h, int(nb)))
+
+    def batch_capacity(self):
+        v = ct.c_int()
+        self._check(self.lib.vie_batch_capacity(self.h, ct.byref(v)))
+        return v.value
+
+    def batch_set_operators(self, i0, k0, eps_tables=None):
+        """Own operators for instances i0.. : wavenumbers k0 (count,) and eps
+        tables (count, n_mat) complex, or None for the context's permittivity."""
+        k = np.ascontiguousarray(np.atleast_1d(k0), dtype=np.float64)
+        if eps_tables is None:
+            self._check(self.lib.vie_batch_set_operators(self.h, int(i0), k.size, _dp(k), None, 0))
+        else:
+            e = np.ascontiguousarray(np.asarray(eps_tables, dtype=np.complex128).reshape(k.size, -1))
+            self._check(self.lib.vie_batch_set_operators(self.h, int(i0), k.size, _dp(k), _dp(e), e.shape[1]))
+
+    def batch_set_rhs_incident(self, i, theta=0.0, phi=0.0, psi=0.0):
+        self._check(self.lib.vie_batch_set_rhs_incident(self.h, int(i), float(theta), float(phi), float(psi)))
+
+    def batch_set_rhs(self, i, b):
+        b = self._field(b)
+        self._check(self.lib.vie_batch_set_rhs(self.h, int(i), _dp(b)))
+
+    def batch_solve(self, nb, method=BICGSTAB, tol=1e-6, maxit=0, mixed=False, tols=None, maxits=None):
+        """Solve instances 0..nb-1 together; returns (status, infos)."""
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        infos = (Info * nb)()
+        tv = None if tols is None else np.ascontiguousarray(tols, dtype=np.float64)
+        mv = None if maxits is None else np.ascontiguousarray(maxits, dtype=np.int32)
+        st = self.lib.vie_batch_solve(self.h, int(nb), int(bool(mixed)), int(method), float(tol), int(maxit),
+                                      _dp(tv) if tv is not None else None,
+                                      mv.ctypes.data_as(ct.POINTER(ct.c_int)) if mv is not None else None, infos)
+        self._check(st, allow=(OK, NOT_CONVERGED, BREAKDOWN))
+        return st, [infos[k].as_dict() for k in range(nb)]
+
+    def batch_solution(self, i):
+        x = np.zeros((3,) + self.n, np.complex128)
+        self._check(self.lib.vie_batch_get_solution(self.h, int(i), _dp(x)))
+        return x
+
+    def batch_forward_scatter(self, i, theta=0.0, phi=0.0, psi=0.0):
+        v = ct.c_double()
+        self._check(self.lib.vie_batch_forward_scatter(self.h, int(i), theta, phi, psi, ct.byref(v)))
+        return v.value
+
+    def batch_absorbed(self, i, mat):
+        v = ct.c_double()
+        self._check(self.lib.vie_batch_absorbed(self.h, int(i), int(mat), ct.byref(v)))
+        return v.value
 
     # -- instrumentation -----------------------------------------------------
     def time_kernel(self, cls):
