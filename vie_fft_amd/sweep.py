@@ -231,9 +231,10 @@ def main():
     args = ap.parse_args()
     points = G.dimer_sweep(args.count)
     t0 = time.perf_counter()
+    kw = {} if args.sequential else {"timings": {}}
     run = run_sweep if args.sequential else run_sweep_batched
     res = run(lambda lam, psi: G.dimer_case(lam, psi=psi, n=args.n), points, tol=args.tol,
-              mixed=not args.fp64, out_dir=args.out, log=lambda r: print(json.dumps(r), flush=True))
+              mixed=not args.fp64, out_dir=args.out, log=lambda r: print(json.dumps(r), flush=True), **kw)
     wall = time.perf_counter() - t0
     its = np.array([r["iterations"] for r in res])
     print(json.dumps({"workload": f"{args.n}^3 Au dimer, {args.count} wavelengths x 2 polarisations",
@@ -241,7 +242,8 @@ def main():
                       "solve_s": sum(r["solve_s"] for r in res), "setup_s": sum(r["setup_s"] for r in res),
                       "iterations_min": int(its.min()), "iterations_median": float(np.median(its)),
                       "iterations_max": int(its.max()),
-                      "all_converged": all(r["converged"] for r in res), "plan": res[0].get("plan")}), flush=True)
+                      "all_converged": all(r["converged"] for r in res), "plan": res[0].get("plan"),
+                      "batched": not args.sequential, "batch_timings": kw.get("timings")}), flush=True)
 
 
 if __name__ == "__main__":
