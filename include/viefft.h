@@ -233,6 +233,16 @@ int vie_absorbed(vie_ctx* ctx, int mat, double* value);
    doubles).  One pass over J serves up to 1024 directions. */
 int vie_far_field(vie_ctx* ctx, int ndir, const double* theta_deg, const double* phi_deg, double* S);
 
+/* vie_far_field without any device allocation (no counterpart in the
+   reference): for callers that cannot allocate device memory -- code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+   Its per-block partials, sums and directions live in a fixed 6.3 MB static
+   device array (one per GPU, held until the process ends), so a call takes
+   at most 512 directions (more: VIE_ERR_ARG).  Calls on one GPU share that
+   array: calls that overlap on one GPU race.  Same S as vie_far_field (its
+   sums in another block order, so equal to roundoff). */
+int vie_far_field_noalloc(vie_ctx* ctx, int ndir, const double* theta_deg, const double* phi_deg, double* S);
+
 /* FarField2 (vie_fft/NearFar.h:302-456): RCS in dB relative to lambda^2 of the
    E plane (phi_s = phi, theta component) and the H plane (phi_s = phi + 90,
    phi component) for scattering angles 0..180 degrees (181 values each). */

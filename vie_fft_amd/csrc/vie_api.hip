@@ -24,6 +24,7 @@
 #include <complex>
 #include <cstring>
 #include <iterator>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1966,6 +1967,43 @@ void check_last_planes(const vie_ctx* c, const unsigned char* flags)
                 }
 }
 
+// ---- the far field without allocation (vie_far_field_noalloc) --------------
+// Its scratch lives in a file-scope device array: the per-block partials of
+// FF_NA_BLOCKS workgroups, the column sums and the direction vectors for up to
+// FF_NA_MAXDIR directions (6.3 MB).  Host code never uses the array's name as
+// an address: each device's copy is looked up once (hipGetSymbolAddress, at
+// vie_create with that device current) and that pointer is what kernels and
+// copies receive.  Every call on one device shares its copy and leaves it
+// holding the last call's partials; nothing in it needs to start at zero
+// (k_farfield writes every partial it later sums).  Calls that overlap on one
+// device race.
+constexpr int FF_NA_MAXDIR = 512;
+constexpr int FF_NA_BLOCKS = 256;
+constexpr long FF_NA_PART = (long)FF_NA_BLOCKS * 3 * FF_NA_MAXDIR;  // double2 entries
+constexpr long FF_NA_WORDS = FF_NA_PART + 3 * FF_NA_MAXDIR + (3 * FF_NA_MAXDIR + 1) / 2;
+static __device__ __align__(256) double2 scratch[FF_NA_WORDS];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch` (looked up once per device).
+static void* scratch_ptr()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Err(VIE_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Err(VIE_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    return g_scratch[dev];
+}
+
 // ================================================================ batches
 // Incident_Wave (vie_fft/GreenFunction.h:144-248) for wavenumber k0 and angles
 // in degrees, masked by the flags (vie_fft/CG_Iteration.h:559-562), into the
@@ -2411,6 +2449,7 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
         c->brhs.assign(1, 0);
         c->bop.assign(1, 0);
         c->bk0.assign(1, c->desc.k0);
+        scratch_ptr();  // this device's copy of the far field's static scratch
     } catch (const Err& e) {
         int code = e.code;
         vie_destroy(c);
@@ -3255,41 +3294,50 @@ int vie_forward_scatter(vie_ctx* c, double theta, double phi, double psi, double
 
 // Far-field amplitudes S_c(v) = dV exp(j k0 delta_c v_c / 2) sum_cells J_c
 // exp(j k0 r.v) of the resident solution (FarField1/FarField2 sums,
-// vie_fft/NearFar.h:59-96, 321-380) for ndir directions; S holds ndir x 3 complex.
-static void far_field_dev(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg,
-                          std::complex<double>* S)
+// vie_fft/NearFar.h:59-96, 321-380) for ndir directions; S holds ndir x 3
+// complex.  Device buffers: vdev (3 ndir directions), part (nblk x 3 ndir
+// per-block partials, nblk k_farfield workgroups), sums (3 ndir).
+static void far_field_run(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg,
+                          std::complex<double>* S, double* vdev, double2* part, double2* sums, unsigned nblk,
+                          double* vh, double2* h)
 {
     const int* n = c->desc.n;
     const double* d = c->desc.d;
     const double k0 = c->desc.k0;
-    std::vector<double> vh(3 * ndir);
     for (int q = 0; q < ndir; ++q) {
         const double th = kPi * theta_deg[q] / 180.0, ph = kPi * phi_deg[q] / 180.0;
         const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
         for (int a = 0; a < 3; ++a) vh[3 * q + a] = vc[c->perm[a]];  // internal axes
     }
-    const long npen = (long)n[0] * n[1];
+    copy_sync(vdev, vh, 3 * ndir * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipLaunchKernelGGL(k_farfield, dim3(nblk), dim3(NT), 3 * n[2] * sizeof(double2), c->stream, n[0], n[1], n[2],
+                       d[0], d[1], d[2], k0, ndir, vdev, c->x64, part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_colsum, dim3((3 * ndir + NT - 1) / NT), dim3(NT), 0, c->stream, (long)nblk, 3 * ndir, part,
+                       sums);
+    HIPCHK(hipGetLastError());
+    copy_sync(h, sums, 3 * ndir * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+    const double dV = d[0] * d[1] * d[2];
+    for (int q = 0; q < ndir; ++q)
+        for (int cpt = 0; cpt < 3; ++cpt)
+            S[3 * q + c->perm[cpt]] = std::polar(1.0, k0 * 0.5 * d[cpt] * vh[3 * q + cpt]) *
+                                      std::complex<double>(h[cpt * ndir + q].x, h[cpt * ndir + q].y) * dV;
+}
+
+static void far_field_dev(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg,
+                          std::complex<double>* S)
+{
+    const long npen = (long)c->desc.n[0] * c->desc.n[1];
     const unsigned nblk = (unsigned)std::min<long>(npen, 1024);
+    std::vector<double> vh(3 * ndir);
+    std::vector<double2> h(3 * ndir);
     double* vdev = dalloc<double>(3 * ndir);
     double2* part = nullptr;
     double2* sums = nullptr;
     try {
         part = dalloc<double2>((long)nblk * 3 * ndir);
         sums = dalloc<double2>(3 * ndir);
-        copy_sync(vdev, vh.data(), vh.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
-        hipLaunchKernelGGL(k_farfield, dim3(nblk), dim3(NT), 3 * n[2] * sizeof(double2), c->stream, n[0], n[1], n[2],
-                           d[0], d[1], d[2], k0, ndir, vdev, c->x64, part);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_colsum, dim3((3 * ndir + NT - 1) / NT), dim3(NT), 0, c->stream, (long)nblk, 3 * ndir,
-                           part, sums);
-        HIPCHK(hipGetLastError());
-        std::vector<double2> h(3 * ndir);
-        copy_sync(h.data(), sums, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
-        const double dV = d[0] * d[1] * d[2];
-        for (int q = 0; q < ndir; ++q)
-            for (int cpt = 0; cpt < 3; ++cpt)
-                S[3 * q + c->perm[cpt]] = std::polar(1.0, k0 * 0.5 * d[cpt] * vh[3 * q + cpt]) *
-                                 std::complex<double>(h[cpt * ndir + q].x, h[cpt * ndir + q].y) * dV;
+        far_field_run(c, ndir, theta_deg, phi_deg, S, vdev, part, sums, nblk, vh.data(), h.data());
     } catch (...) {
         dfree(vdev);
         dfree(part);
@@ -3311,6 +3359,32 @@ int vie_far_field(vie_ctx* c, int ndir, const double* theta_deg, const double* p
             const int m = std::min(ndir - q0, FF_MAXD * NT);
             far_field_dev(c, m, theta_deg + q0, phi_deg + q0, (std::complex<double>*)S + 3 * (long)q0);
         }
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_far_field_noalloc(vie_ctx* c, int ndir, const double* theta_deg, const double* phi_deg, double* S)
+{
+    if (!c || ndir < 0 || (ndir > 0 && (!theta_deg || !phi_deg || !S))) return VIE_ERR_ARG;
+    if (ndir > FF_NA_MAXDIR) {
+        c->err = "vie_far_field_noalloc: at most " + std::to_string(FF_NA_MAXDIR) + " directions per call";
+        return VIE_ERR_ARG;
+    }
+    if (ndir == 0) return VIE_OK;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, c->desc.n[2] <= 1024, "z extent too large for the far-field kernel");
+        double2* base = (double2*)scratch_ptr();
+        double2* part = base;
+        double2* sums = base + FF_NA_PART;
+        double* vdev = (double*)(base + FF_NA_PART + 3 * FF_NA_MAXDIR);
+        double vh[3 * FF_NA_MAXDIR];
+        double2 h[3 * FF_NA_MAXDIR];
+        const long npen = (long)c->desc.n[0] * c->desc.n[1];
+        const unsigned nblk = (unsigned)std::min<long>(npen, FF_NA_BLOCKS);
+        far_field_run(c, ndir, theta_deg, phi_deg, (std::complex<double>*)S, vdev, part, sums, nblk, vh, h);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

@@ -35,7 +35,7 @@ EXPORTS = [
     "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
     "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
-    "vie_batch_absorbed",
+    "vie_batch_absorbed", "vie_far_field_noalloc",
 ]
 
 
@@ -107,6 +107,7 @@ def load():
     L.vie_set_k0.argtypes = [vp, ct.c_double]
     L.vie_set_rhs_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
     L.vie_far_field.argtypes = [vp, ct.c_int, dp, dp, dp]
+    L.vie_far_field_noalloc.argtypes = [vp, ct.c_int, dp, dp, dp]
     L.vie_rcs.argtypes = [vp, ct.c_double, dp, dp]
     L.vie_solve_batched.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_int, dp, dp,
                                     ct.POINTER(Info)]
@@ -287,6 +288,17 @@ class Solver:
             raise ValueError("theta and phi must have the same length")
         S = np.zeros((th.size, 3), np.complex128)
         self._check(self.lib.vie_far_field(self.h, th.size, _dp(th), _dp(ph), _dp(S)))
+        return S
+
+    def far_field_noalloc(self, theta_deg, phi_deg):
+        """far_field through the entry point that allocates no device memory
+        (at most 512 directions per call)."""
+        th = np.ascontiguousarray(theta_deg, dtype=np.float64).ravel()
+        ph = np.ascontiguousarray(phi_deg, dtype=np.float64).ravel()
+        if th.shape != ph.shape:
+            raise ValueError("theta and phi must have the same length")
+        S = np.zeros((th.size, 3), np.complex128)
+        self._check(self.lib.vie_far_field_noalloc(self.h, th.size, _dp(th), _dp(ph), _dp(S)))
         return S
 
     def rcs(self, phi=0.0):
