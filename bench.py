@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 96^3 dimer sweep and 256^3 legs")
     ap.add_argument("--n", type=int, default=128, help="grid edge (128 = the metric workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a plain device copy of the x pass's working set (HBM counter calibration)")
@@ -146,16 +147,19 @@ def main():
         s.solve_resident(VF.BICGSTAB, tol=args.tol)
         s.solve_mixed(tol=args.tol)
 
-    # ---- timed: FP64 plain solves
-    s.time_kernel(3)
+    # ---- timed: FP64 plain solves (no instrumentation inside the timed region)
     sync()
     t0 = time.perf_counter()
     infos64 = [s.solve_resident(VF.BICGSTAB, tol=args.tol) for _ in range(args.steps)]
     sync()
     t64 = (time.perf_counter() - t0) / args.steps
-    n3, ms3_64 = s.kernel_time()
+    x64 = s.solution()
+    q64 = cross_sections(s, sc)
 
-    # ---- timed: mixed solves (the metric)
+    # ---- timed: mixed solves (the metric); the FP32 x pass (the roofline
+    # kernel) is timed live, HIP events around one launch in eight on the
+    # library's stream (an event pair adds a few us around the launch it
+    # brackets: about 0.5% of the solve)
     s.time_kernel(13)
     sync()
     t0 = time.perf_counter()
@@ -163,6 +167,15 @@ def main():
     sync()
     tm = (time.perf_counter() - t0) / args.steps
     n13, ms13 = s.kernel_time()
+    s.time_kernel(0)
+    xm = s.solution()
+    qm = cross_sections(s, sc)
+
+    # ---- the FP64 x pass: one more FP64 solve with the same events (outside
+    # the timed regions)
+    s.time_kernel(3)
+    s.solve_resident(VF.BICGSTAB, tol=args.tol)
+    n3, ms3_64 = s.kernel_time()
     s.time_kernel(0)
 
     it64 = infos64[-1]["iterations"]
@@ -201,6 +214,12 @@ def main():
                  "xpass_ms": ms3_64, "xpass_frac": xpass_bytes(plan, 16.0) / (ms3_64 * 1e-3) / HBM_PEAK
                  if ms3_64 > 0 else None},
         "speedup_mixed_vs_fp64": t64 / tm,
+        "mixed_vs_fp64": {"solution_rel_l2": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+                          "q_ext_rel": abs(qm[0] - q64[0]) / abs(q64[0]),
+                          "q_abs_rel": abs(qm[1] - q64[1]) / abs(q64[1]),
+                          "q_ext_fp64": q64[0], "q_abs_fp64": q64[1], "q_ext_mixed": qm[0], "q_abs_mixed": qm[1],
+                          "note": "both solves to the same FP64 true relative residual 1e-6; the difference is "
+                                  "bounded by the two solves' residuals, not by FP32 rounding"},
         "solve_roofline": solve_roofline(plan, tm, itm, outer, t64, it64),
         "plan": {"axis_order": list(plan["axis_order"]), "x_window": list(plan["x_window"]),
                  "y_window": list(plan["y_window"]), "y_pencil": plan["y_pencil"],
@@ -221,10 +240,75 @@ def main():
     _, _, copy_gbps = copy_rate(torch, n[0])
     out["roofline"]["copy_GBps"] = copy_gbps
     out["roofline"]["frac_of_copy"] = achieved / 1e9 / copy_gbps
+    s.close()
+    if not args.no_extra:
+        out["dimer_sweep_96cube"] = dimer_sweep_leg(args.tol)
+        out["spheres_256cube"] = spheres_256_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
     print(json.dumps(out))
-    s.close()
+
+
+def cross_sections(s, sc):
+    """(Q_ext, Q_abs) of the resident solution: FarField1's forward scattering
+    and NearField's Ene_Metal of the spheres, over the geometric area."""
+    th, ph, ps = sc.rhs_angles()
+    return (s.forward_scatter(th, ph, ps) / sc.area, s.absorbed(sc.absorb_mat) * 120.0 * np.pi / sc.area)
+
+
+def dimer_sweep_leg(tol):
+    """The 96^3 Au-dimer sweep (Drude eps, 64 wavelengths 400-800 nm x 2
+    polarisations = 128 right-hand sides, mixed precision to 1e-6) as one
+    batched solve: right-hand sides per second of a whole sweep (context,
+    per-instance operators and G~, right-hand sides, solve, Q_ext / Q_abs of
+    every instance), after one untimed sweep."""
+    from vie_fft_amd import geometry as G
+    from vie_fft_amd import sweep as SW
+
+    points = G.dimer_sweep(64)
+    scene = lambda lam, psi: G.dimer_case(lam, psi=psi)  # noqa: E731
+    SW.run_sweep_batched(scene, points, tol=tol, mixed=True)
+    tm = {}
+    t0 = time.perf_counter()
+    res = SW.run_sweep_batched(scene, points, tol=tol, mixed=True, timings=tm)
+    wall = time.perf_counter() - t0
+    its = np.array([r["iterations"] for r in res])
+    return {"rhs": len(res), "rhs_per_s": len(res) / wall, "wall_s": wall, "solve_s": tm["solve_s"],
+            "solve_rhs_per_s": len(res) / tm["solve_s"], "setup_s": tm["setup_s"], "post_s": tm["post_s"],
+            "instances_per_batch": tm["chunk"], "iterations_min": int(its.min()),
+            "iterations_median": float(np.median(its)), "iterations_max": int(its.max()),
+            "all_converged": all(r["converged"] for r in res),
+            "max_true_relres": max(r["true_relres"] for r in res)}
+
+
+def spheres_256_leg():
+    """256^3 grid, 60 seeded random spheres (cell 1.5 nm, 550 nm): every axis
+    window is the whole grid, so the passes run at their full lengths.  One
+    untimed mixed solve, then 2 timed mixed solves and 1 timed FP64 solve to
+    1e-6; per-iteration times and the dense-model fraction of 8 TB/s."""
+    from vie_fft_amd import geometry as G
+    from vie_fft_amd import viefft as VF
+
+    sc = G.random_spheres_case()
+    with VF.Solver(sc.n, sc.d, sc.k0, VF.FP64) as s:
+        s.set_medium(sc.material(), sc.eps_table, sc.flags())
+        s.green_compute()
+        s.set_rhs_incident(*sc.rhs_angles())
+        s.solve_mixed(tol=1e-6)
+        t0 = time.perf_counter()
+        im = [s.solve_mixed(tol=1e-6) for _ in range(2)]
+        tm = (time.perf_counter() - t0) / 2
+        t0 = time.perf_counter()
+        i64 = s.solve_resident(VF.BICGSTAB, tol=1e-6)
+        t64 = time.perf_counter() - t0
+        plan = s.plan()
+    dm, _ = solve_models(plan, 8.0, im[-1]["iterations"], im[-1]["outer"])
+    d6, _ = solve_models(plan, 16.0, i64["iterations"])
+    return {"mixed_s": tm, "mixed_iterations": im[-1]["iterations"], "mixed_replacements": im[-1]["outer"],
+            "mixed_us_per_iteration": tm * 1e6 / im[-1]["iterations"], "mixed_true_relres": im[-1]["true_relres"],
+            "fp64_s": t64, "fp64_iterations": i64["iterations"], "fp64_us_per_iteration": t64 * 1e6 / i64["iterations"],
+            "fp64_true_relres": i64["true_relres"], "mixed_dense_frac": dm / tm / HBM_PEAK,
+            "fp64_dense_frac": d6 / t64 / HBM_PEAK, "x_window": list(plan["x_window"]), "x_pencil": plan["x_pencil"]}
 
 
 def solve_roofline(plan, tm, itm, outer, t64, it64):
@@ -287,13 +371,21 @@ def _cpu_child():
 
 
 def _cpu_model():
+    """model name, family/model/stepping and MHz of the first CPU in /proc/cpuinfo."""
+    keys = {"model name": None, "cpu family": None, "model": None, "stepping": None, "cpu MHz": None}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" not in line:
+                if all(v is not None for v in keys.values()):
+                    break
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            if k in keys and keys[k] is None:
+                keys[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{keys['model name']} (family {keys['cpu family']}, model {keys['model']}, "
+            f"stepping {keys['stepping']}, {keys['cpu MHz']} MHz)")
 
 
 def cpu_baseline(sc, flags, iterations):
@@ -304,12 +396,15 @@ def cpu_baseline(sc, flags, iterations):
     products), a solve is `iterations` iterations.  Two legs: one core (the
     reference's MKL-sequential solve, README.md:37), and K cores each running
     an independent sample at once (its frequency-farm mode, vie_fft/main.cpp:27-36),
-    K = min(16, cpu_count) child processes."""
+    K = min(16, visible cores) child processes (16: one GPU's share of the host)."""
     import subprocess
 
     t_comp = _component_seconds(sc, flags)
     t_solve = 6.0 * t_comp * iterations
-    k = max(1, min(16, os.cpu_count() or 1))
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # the GPU host gives one GPU's job a share of 16 cores: the all-cores leg
+    # runs one process per core of that share (all of them on smaller hosts)
+    k = max(1, min(16, nproc))
     here = os.path.dirname(os.path.abspath(__file__))
     procs = [subprocess.Popen([sys.executable, "-c", "from bench import _cpu_child; _cpu_child()"], cwd=here,
                               stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for _ in range(k)]
@@ -321,9 +416,11 @@ def cpu_baseline(sc, flags, iterations):
     par = None
     if len(ts) == k:
         t_par = 6.0 * float(np.mean(ts)) * iterations
-        par = {"value": k / t_par, "unit": "solves/s", "cores": k,
+        par = {"value": k / t_par, "unit": "solves/s", "cores": k, "host_cores": nproc,
+               "cpu_count": os.cpu_count(),
                "sample": f"{k} concurrent processes, one component product each: mean {np.mean(ts):.2f} s "
-                         f"(max {max(ts):.2f} s); throughput = {k} solves per extrapolated solve time"}
+                         f"(max {max(ts):.2f} s); throughput = {k} solves per extrapolated solve time; "
+                         f"{k} = the 16-core CPU share of one GPU's job (of {nproc} cores visible)"}
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"one X_FFT-equivalent component product at {sc.n[0]}^3 (reference 9+9 FFT schedule, "
                       f"per-element trig) took {t_comp:.2f} s on 1 core; solve extrapolated as 6 x that x "

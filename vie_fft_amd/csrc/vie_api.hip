@@ -730,11 +730,27 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
         L.wV = c->V;
         L.wlo = o->wx0 * pl;
         L.whi = o->wx1 * pl;
+        // narrower y or z windows too (compact media): the box of the three
+        // windows only; its z range starts even and holds whole pairs
+        const int z1 = std::min(o->g.nz, (o->wz1 + 1) & ~1);
+        if ((o->wy0 > 0 || o->wy1 < o->g.ny || o->wz0 > 0 || z1 < o->g.nz) && o->g.nz % 2 == 0) {
+            L.bzp = (z1 - o->wz0) / 2;
+            L.bx0 = o->wx0;
+            L.bwx = o->wx1 - o->wx0;
+            L.by0 = o->wy0;
+            L.bwy = o->wy1 - o->wy0;
+            L.bz0 = o->wz0;
+            L.bny = o->g.ny;
+            L.bnz = o->g.nz;
+            L.blg = 0;
+            while ((1 << L.blg) < L.bzp && (1 << L.blg) < NT) ++L.blg;
+        }
     }
     // windowed: at most 1024 workgroups per component.  The FP32 BiCGSTAB
     // update over the 48-plane window at 128^3 takes 28.0 us per launch; 768
     // workgroups (two whole grid-stride rounds) 27.7, 1536 29.4, 2048 32.4 us.
-    const unsigned gsz = grid1(L.wV ? (L.whi - L.wlo) / 2 : (L.n + 1) / 2, NT, L.wV ? 1024 : 2048);
+    const long work = L.bzp ? (long)L.bwx * L.bwy * (1L << L.blg) : L.wV ? (L.whi - L.wlo) / 2 : (L.n + 1) / 2;
+    const unsigned gsz = grid1(work, NT, L.wV ? 1024 : 2048);
     const unsigned gy = L.wV ? 3 : 1;
     L.dots.partial = c->partial;
     L.bt = c->cur;
@@ -1698,7 +1714,10 @@ void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
         const unsigned gi = (unsigned)((nb + 63) / 64);
         hipLaunchKernelGGL(k_mix_select, dim3(gi), dim3(64), 0, c->stream, st, ax, nb);
         const Bat all = c->cur;
-        hipLaunchKernelGGL(k_fold64_b<float>, dim3(grid1(n, NT, 2048), nb), dim3(NT), 0, c->stream, n, all, sel, x,
+        // the x window's entries of each component (every active face lies in it)
+        const long pl = (long)op64->g.ny * op64->g.nz, wlo = op64->wx0 * pl, wn = (op64->wx1 - op64->wx0) * pl;
+        const dim3 gw(grid1(wn, NT, 512), 3, nb);
+        hipLaunchKernelGGL(k_fold64_b<float>, gw, dim3(NT), 0, c->stream, n, c->V, wlo, wn, op64->flags, all, sel, x,
                            c->x64);
         HIPCHK(hipGetLastError());
         op64->apply(lin_kw<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, sel, c->stream,
@@ -1710,13 +1729,14 @@ void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
         M.hc[0][0] = 1.0;
         M.hc[1][0] = -1.0;
         M.out = c->r64;
+        M.wskip = 1;  // only the active faces of r64 are read (below)
         M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
         long nbk = lincomb<double>(c, M, sel);
         scalar(c, ax, SC_RELRES, nbk, 1);
         hipLaunchKernelGGL(k_mix_decide, dim3(gi), dim3(64), 0, c->stream, st, ax, nb, delta);
         HIPCHK(hipGetLastError());
         // the resumed instances (ax.done still 0): load the replaced residual
-        hipLaunchKernelGGL(k_convert_b<float>, dim3(grid1(n, NT, 2048), nb), dim3(NT), 0, c->stream, n, all, sel,
+        hipLaunchKernelGGL(k_convert_b<float>, gw, dim3(NT), 0, c->stream, n, c->V, wlo, wn, op64->flags, all, sel,
                            c->r64, r);
         HIPCHK(hipGetLastError());
         LinComb<R> Rr{};  // b^H r, r^H r of the replaced residual

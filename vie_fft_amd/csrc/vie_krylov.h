@@ -37,6 +37,11 @@ template <class R> struct LinComb {
     // the three components (length wV, all even) hold active faces; the
     // kernel then runs over them alone (grid.y = 3).  wV = 0: every entry.
     long wV, wlo, whi;
+    // with a window, a box instead (bzp > 0): only the z-row pairs [bz0, bz0 +
+    // 2 bzp) of the rows (x, y), x in [bx0, bx0 + bwx), y in [by0, by0 + bwy),
+    // of each component (grid ny x nz = bny x bnz, rows per component bnx bny)
+    // hold active faces; 2^blg lanes serve a row (2^blg >= bzp, <= NT)
+    int bzp, bx0, bwx, by0, bwy, bz0, bny, bnz, blg;
     // batches (grid.z = launch slots): the vectors hold the instances n
     // entries apart; mask and window are shared
     Bat bt{};
@@ -113,8 +118,7 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
         if (sel == 2) r = e2;
         return r;
     };
-    for (long i2 = (long)blockIdx.x * NT + threadIdx.x; i2 < npair; i2 += stride) {
-        const long i = base + 2 * i2;
+    auto pair = [&](long i) {
         // every load of the pair first (more bytes in flight per thread)
         T a0[4], a1[4], e10 = czero<T>(), e11 = czero<T>(), e20 = czero<T>(), e21 = czero<T>();
         const bool act = !L.mask || *reinterpret_cast<const uint16_t*>(L.mask + i) != 0;
@@ -149,6 +153,18 @@ __global__ __launch_bounds__(NT) void k_lincomb(LinComb<R> L, const int* done)
             if (L.out) st2(L.out + i, v, w);
             if (L.out2) st2(L.out2 + i, v2, w2);
         }
+    };
+    if (L.wV && L.bzp) {  // the window box: rows of the x and y windows, the z window's pairs
+        const int lanes = 1 << L.blg, rpb = NT >> L.blg;
+        const int lane = threadIdx.x & (lanes - 1);
+        const long nrows = (long)L.bwx * L.bwy;
+        const long cbase = (long)blockIdx.y * L.wV + L.bz0;
+        for (long r = (long)blockIdx.x * rpb + (threadIdx.x >> L.blg); r < nrows; r += (long)gridDim.x * rpb) {
+            const long row = (long)(L.bx0 + (int)(r / L.bwy)) * L.bny + L.by0 + (int)(r % L.bwy);
+            for (int zp = lane; zp < L.bzp; zp += lanes) pair(cbase + row * L.bnz + 2 * zp);
+        }
+    } else {
+        for (long i2 = (long)blockIdx.x * NT + threadIdx.x; i2 < npair; i2 += stride) pair(base + 2 * i2);
     }
     if (!L.wV && blockIdx.x == 0 && threadIdx.x == 0 && (L.n & 1)) {
         const long i = L.n - 1;
@@ -352,33 +368,40 @@ __global__ void k_mix_decide(KState* st, KState* ax, int nb, double delta)
     }
 }
 
-// Batched element-wise updates over the instances of launch slots
-// (grid.y = slots, n entries per instance, done as in the pass kernels):
-// x64 += s e, e = 0 (the reliable update), and the conversions.
+// Batched element-wise updates of the residual replacements (grid.z = launch
+// slots, done as in the pass kernels; n entries per instance): over the
+// entries [wlo, wlo + wn) of each of the three components (grid.y = 3, V
+// entries per component) whose face is active -- every other entry of the
+// operands is zero and stays so.  x64 += e, e = 0 (the reliable update), and
+// out = (R) in.
 template <class R>
-__global__ void k_fold64_b(long n, Bat bt, const int* done, cx<R>* e, double2* x)
+__global__ void k_fold64_b(long n, long V, long wlo, long wn, const uint8_t* __restrict__ mask, Bat bt,
+                           const int* done, cx<R>* e, double2* x)
 {
-    const int inst = bat_inst(bt, blockIdx.y);
+    const int inst = bat_inst(bt, blockIdx.z);
     done = bat_done(bt, done, inst);
     if (done && *done) return;
-    e += (long)inst * n;
-    x += (long)inst * n;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        x[t].x += (double)e[t].x;
-        x[t].y += (double)e[t].y;
-        e[t] = czero<cx<R>>();
+    const long base = (long)inst * n + (long)blockIdx.y * V + wlo;
+    const uint8_t* m = mask + (long)blockIdx.y * V + wlo;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < wn; t += (long)gridDim.x * blockDim.x) {
+        if (!m[t]) continue;
+        const cx<R> v = e[base + t];
+        x[base + t].x += (double)v.x;
+        x[base + t].y += (double)v.y;
+        e[base + t] = czero<cx<R>>();
     }
 }
 template <class R>
-__global__ void k_convert_b(long n, Bat bt, const int* done, const double2* in, cx<R>* out)
+__global__ void k_convert_b(long n, long V, long wlo, long wn, const uint8_t* __restrict__ mask, Bat bt,
+                            const int* done, const double2* in, cx<R>* out)
 {
-    const int inst = bat_inst(bt, blockIdx.y);
+    const int inst = bat_inst(bt, blockIdx.z);
     done = bat_done(bt, done, inst);
     if (done && *done) return;
-    in += (long)inst * n;
-    out += (long)inst * n;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-        out[t] = mkc((R)in[t].x, (R)in[t].y);
+    const long base = (long)inst * n + (long)blockIdx.y * V + wlo;
+    const uint8_t* m = mask + (long)blockIdx.y * V + wlo;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < wn; t += (long)gridDim.x * blockDim.x)
+        if (m[t]) out[base + t] = mkc((R)in[base + t].x, (R)in[base + t].y);
 }
 
 // ------------------------------------------------------------ Green's function
