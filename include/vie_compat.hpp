@@ -6,6 +6,9 @@
  * its calls
  *     GetMaterial(); GetFlag(); Green_Function_Compute(); Incident_Wave();
  *     FFT_Program(); FarField1(); FarField2(); NearField();
+ * and code that applies the operator itself keeps X_FFT / Y_FFT / Z_FFT, the
+ * _Conjugate set and Current_Reset (vie_fft/CG_Iteration.h:86-173,
+ * vie_fft/fft.h:13-35) with the reference's signatures.
  * and replaces the three includes
  *     #include "GreenFunction.h"  #include "CG_Iteration.h"  (fft.h comes with it)
  * by
@@ -139,6 +142,97 @@ inline void Incident_Wave()
     std::vector<double> E(6 * vie_compat::volume());
     vie_compat::check(vie_incident(vie_compat::context(), Theta, Phi, Psi, E.data()), "vie_incident");
     vie_compat::scatter3(E, Ex_inc, Ey_inc, Ez_inc);
+}
+
+namespace vie_compat {
+
+// Field += component f of A S or A^H S (no output mask), S = (Sx, Sy, Sz):
+// the whole product runs on the GPU and component f is added on the host.
+inline void add_component(int f, int adjoint, MKL_Complex16*** Field, MKL_Complex16*** const Sx,
+                          MKL_Complex16*** const Sy, MKL_Complex16*** const Sz, const int* Dim)
+{
+    if (!context()) {
+        std::fprintf(stderr, "X_FFT/Y_FFT/Z_FFT before Green_Function_Compute\n");
+        std::exit(1);
+    }
+    if (Dim[0] != Nx || Dim[1] != Ny || Dim[2] != Nz) {
+        std::fprintf(stderr, "Dim (%d, %d, %d) is not the grid (%d, %d, %d)\n", Dim[0], Dim[1], Dim[2], Nx, Ny, Nz);
+        std::exit(1);
+    }
+    const long V = volume();
+    std::vector<double> x, y(6 * V);
+    gather3(Sx, Sy, Sz, x);
+    check(vie_apply_raw(context(), x.data(), y.data(), adjoint), "vie_apply_raw");
+    MKL_Complex16* p = Field[0][0];
+    for (long v = 0; v < V; ++v) {
+        p[v].real += y[2 * (f * V + v)];
+        p[v].imag += y[2 * (f * V + v) + 1];
+    }
+}
+
+}  // namespace vie_compat
+
+// The operator calls of vie_fft/CG_Iteration.h:86-173 with their signatures:
+// Field += component x, y or z of A S (X_FFT, Y_FFT, Z_FFT) or of A^H S (the
+// _Conjugate set), S = (Sourx, Soury, Sourz), without Current_Reset, as in the
+// reference.  temp and Green are not used: the padded transforms and G~ live
+// on the GPU (Green_Function_Compute set them up).
+inline void X_FFT(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                  MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green, const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(0, 0, Field, Sourx, Soury, Sourz, Dim);
+}
+inline void Y_FFT(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                  MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green, const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(1, 0, Field, Sourx, Soury, Sourz, Dim);
+}
+inline void Z_FFT(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                  MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green, const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(2, 0, Field, Sourx, Soury, Sourz, Dim);
+}
+inline void X_FFT_Conjugate(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                            MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green,
+                            const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(0, 1, Field, Sourx, Soury, Sourz, Dim);
+}
+inline void Y_FFT_Conjugate(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                            MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green,
+                            const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(1, 1, Field, Sourx, Soury, Sourz, Dim);
+}
+inline void Z_FFT_Conjugate(MKL_Complex16*** Field, MKL_Complex16*** const Sourx, MKL_Complex16*** const Soury,
+                            MKL_Complex16*** const Sourz, MKL_Complex16*** temp, MKL_Complex16*** const Green,
+                            const int* Dim)
+{
+    (void)temp;
+    (void)Green;
+    vie_compat::add_component(2, 1, Field, Sourx, Soury, Sourz, Dim);
+}
+
+// Current_Reset (vie_fft/fft.h:13-35): component Flag of a field times its
+// face flags (Flag_X, Flag_Y, Flag_Z).
+inline void Current_Reset(MKL_Complex16*** Current, const int Flag, const int* Dim)
+{
+    unsigned char*** fl = Flag == 0 ? Flag_X : Flag == 1 ? Flag_Y : Flag_Z;
+    const long V = (long)Dim[0] * Dim[1] * Dim[2];
+    MKL_Complex16* p = Current[0][0];
+    const unsigned char* f = fl[0][0];
+    for (long v = 0; v < V; ++v)
+        if (!f[v]) p[v].real = p[v].imag = 0.0;
 }
 
 // FFT_Program (vie_fft/CG_Iteration.h:1365-1388) with its solvers' output and

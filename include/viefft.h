@@ -11,6 +11,7 @@
  *   GetMaterial/GetFlag out  vie_fft/Model.h:305-398         -> vie_set_medium[_eps]
  *   X_FFT/Y_FFT/Z_FFT +      vie_fft/CG_Iteration.h:131-173, -> vie_apply
  *     Current_Reset            vie_fft/fft.h:13-35
+ *   X_FFT/Y_FFT/Z_FFT alone  vie_fft/CG_Iteration.h:131-173   -> vie_apply_raw
  *   *_FFT_Conjugate          vie_fft/CG_Iteration.h:86-128   -> vie_apply(adjoint=1)
  *   FFT_Program              vie_fft/CG_Iteration.h:1365-1388-> vie_solve
  *   FarField1                vie_fft/NearFar.h:16-109        -> vie_forward_scatter
@@ -45,7 +46,8 @@ typedef enum {
     VIE_NOT_CONVERGED = 4,    /* maxit reached before the stop rule    */
     VIE_BREAKDOWN = 5,        /* zero or non-finite Krylov denominator */
     VIE_ERR_NAN = 6,          /* non-finite input                      */
-    VIE_ERR_UNSUPPORTED = 7   /* grid size without a compiled kernel   */
+    VIE_ERR_UNSUPPORTED = 7,  /* grid size without a compiled kernel   */
+    VIE_DIVERGED = 8          /* divergence stop (vie_set_divergence_stop) */
 } vie_status;
 
 typedef enum { VIE_FP64 = 0, VIE_FP32 = 1 } vie_precision;
@@ -123,6 +125,11 @@ int vie_incident(vie_ctx* ctx, double theta, double phi, double psi, double* E);
 /* y = A x (adjoint=0) or A^H x (adjoint=1) including the output mask;
    host buffers, 3V complex each, FP64 interface whatever the precision. */
 int vie_apply(vie_ctx* ctx, const double* x, double* y, int adjoint);
+/* y = A x or A^H x without the output mask: the products of X_FFT/Y_FFT/
+   Z_FFT and of the _Conjugate set before Current_Reset
+   (vie_fft/CG_Iteration.h:86-173; the reference's solvers mask after each,
+   e.g. vie_fft/CG_Iteration.h:643-649) */
+int vie_apply_raw(vie_ctx* ctx, const double* x, double* y, int adjoint);
 /* same on device buffers of the context's precision (complex<double> for
    VIE_FP64, complex<float> for VIE_FP32), enqueued on `stream` (a hipStream_t,
    NULL = the context's stream) without a host synchronisation; both buffers
@@ -201,6 +208,22 @@ int vie_batch_absorbed(vie_ctx* ctx, int i, int mat, double* value);
    masked by the flags (vie_fft/CG_Iteration.h:559-562); no host transfer. */
 int vie_set_rhs_incident(vie_ctx* ctx, double theta, double phi, double psi);
 int vie_solve_resident(vie_ctx* ctx, int method, double tol, int maxit, vie_info* info);
+/* Divergence stop (off by default, as in the reference, whose solvers run
+   to Nx*Ny*Nz iterations whatever the residual does, e.g.
+   vie_fft/CG_Iteration.h:591,734): with factor >= 1 every later solve of the
+   context ends with VIE_DIVERGED once the method's stop quantity exceeds
+   factor times the smallest value it reached; 0 turns it off. */
+int vie_set_divergence_stop(vie_ctx* ctx, double factor);
+/* Warm start (the reference always starts from x = 0, vie_fft/Model.h:189-191):
+   the next vie_solve_resident / vie_solve_mixed_resident starts from x0
+   (host, 3 V complex, masked by the flags like the solution), or from the
+   resident solution when x0 is NULL (a neighbouring wavelength's, in a
+   sweep).  The stop rule still measures ||b - A x|| / ||b||; a guess that
+   already meets it returns at once (0 iterations).  Not for VIE_BICG, whose
+   stop quantity is not a residual norm. */
+int vie_set_guess(vie_ctx* ctx, const double* x0);
+/* vie_solve with x as the initial guess on entry and the solution on exit */
+int vie_solve_x0(vie_ctx* ctx, int method, double tol, int maxit, const double* b, double* x, vie_info* info);
 int vie_get_solution(vie_ctx* ctx, double* x);
 /* Mixed precision: the Krylov recurrences, FFTs and spectral product run in
    FP32 complex while the solution accumulates in FP64 and the residual that
@@ -268,6 +291,11 @@ int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double*
                         double* grad_im, vie_info* info);
 
 /* ---- instrumentation ----------------------------------------------------- */
+/* The last single solve's stop quantity after each iteration (the values the
+   reference prints as "CG Error", e.g. vie_fft/CG_Iteration.h:737-747): the
+   first min(max, count) into err, *count = iterations recorded (at most
+   65536).  For a mixed solve these are the FP32 recursive residuals. */
+int vie_history(vie_ctx* ctx, int max, double* err, int* count);
 /* Per-launch HIP-event timing of one kernel class while solving:
    0 off, 1 z-forward, 2 y-forward, 3 x spectral pass, 4 y-inverse,
    5 z-inverse of the FP64 operator, 6 FP64 vector update (linear

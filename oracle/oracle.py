@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libvie_oracle.so")
+# VIE_ORACLE_LIB selects another build of the same source (the sanitizer build)
+_LIB_PATH = os.environ.get("VIE_ORACLE_LIB") or os.path.join(_HERE, "libvie_oracle.so")
 _lib = None
 
 

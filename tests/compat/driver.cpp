@@ -6,9 +6,13 @@
 // Current_X/Y/Z.
 //
 //   driver <case.bin> <out.bin>
+//   driver op <case.bin> <x.bin> <out.bin>
 // case.bin: int32 n[3]; float64 d[3], k0, theta, phi, psi; int32 nmat;
 //           float64 eps[2 nmat]; uint8 material[V]; uint8 flags[3V]
 // out.bin:  float64 J[6V] (Current_X | Current_Y | Current_Z, interleaved)
+// The op mode reads a field S (x.bin: 3V complex) after Green_Function_Compute
+// and writes, each 3V complex: X_FFT/Y_FFT/Z_FFT of S into zeroed fields,
+// the same after Current_Reset, and the _Conjugate set of S.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -69,12 +73,20 @@ static void rd(void* p, size_t n, FILE* f)
     }
 }
 
+static void write3(FILE* o, MKL_Complex16*** a, MKL_Complex16*** b, MKL_Complex16*** c, long V)
+{
+    MKL_Complex16*** J[3] = {a, b, c};
+    for (int s = 0; s < 3; ++s) std::fwrite(J[s][0][0], sizeof(MKL_Complex16), V, o);
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: driver case.bin out.bin\n");
+    const bool op = argc == 5 && std::strcmp(argv[1], "op") == 0;
+    if (argc != 3 && !op) {
+        std::fprintf(stderr, "usage: driver case.bin out.bin | driver op case.bin x.bin out.bin\n");
         return 2;
     }
+    if (op) ++argv;
     FILE* f = std::fopen(argv[1], "rb");
     if (!f) return 2;
     int n[3], nmat;
@@ -118,13 +130,48 @@ int main(int argc, char** argv)
     std::fclose(f);
 
     Green_Function_Compute();
+    if (op) {
+        // the operator calls of vie_fft/CG_Iteration.h:86-173 on a given field
+        MKL_Complex16*** S[3] = {nullptr, nullptr, nullptr};
+        MKL_Complex16*** F[3] = {nullptr, nullptr, nullptr};
+        FILE* xf = std::fopen(argv[2], "rb");
+        if (!xf) return 2;
+        for (int s = 0; s < 3; ++s) {
+            S[s] = Memory3D(S[s], Nx, Ny, Nz);
+            F[s] = Memory3D(F[s], Nx, Ny, Nz);
+            rd(S[s][0][0], sizeof(MKL_Complex16) * V, xf);
+        }
+        std::fclose(xf);
+        const int Dim[3] = {Nx, Ny, Nz};
+        MKL_Complex16*** temp = Memory3D(temp, 2 * Nx, 2 * Ny, 2 * Nz);
+        X_FFT(F[0], S[0], S[1], S[2], temp, Green_Function, Dim);
+        Y_FFT(F[1], S[0], S[1], S[2], temp, Green_Function, Dim);
+        Z_FFT(F[2], S[0], S[1], S[2], temp, Green_Function, Dim);
+        FILE* o = std::fopen(argv[3], "wb");
+        if (!o) return 2;
+        write3(o, F[0], F[1], F[2], V);
+        for (int s = 0; s < 3; ++s) Current_Reset(F[s], s, Dim);
+        write3(o, F[0], F[1], F[2], V);
+        for (int s = 0; s < 3; ++s) std::memset(F[s][0][0], 0, sizeof(MKL_Complex16) * V);
+        X_FFT_Conjugate(F[0], S[0], S[1], S[2], temp, Green_Function, Dim);
+        Y_FFT_Conjugate(F[1], S[0], S[1], S[2], temp, Green_Function, Dim);
+        Z_FFT_Conjugate(F[2], S[0], S[1], S[2], temp, Green_Function, Dim);
+        write3(o, F[0], F[1], F[2], V);
+        std::fclose(o);
+        for (int s = 0; s < 3; ++s) {
+            RMemory3D(S[s]);
+            RMemory3D(F[s]);
+        }
+        RMemory3D(temp);
+        vie_destroy(vie_compat::context());
+        return 0;
+    }
     Incident_Wave();
     FFT_Program();
 
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 2;
-    MKL_Complex16*** J[3] = {Current_X, Current_Y, Current_Z};
-    for (int s = 0; s < 3; ++s) std::fwrite(J[s][0][0], sizeof(MKL_Complex16), V, o);
+    write3(o, Current_X, Current_Y, Current_Z, V);
     std::fclose(o);
     RMemory3D(Current_X);
     RMemory3D(Current_Y);

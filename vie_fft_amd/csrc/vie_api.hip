@@ -492,7 +492,7 @@ template <class R> struct Op {
     // written here unless the caller keeps them zero (wskip), in which case
     // the z rows and y entries outside the y window are skipped too.
     long apply(const Lin<R>& lin, const T* src, T* y, Dots<R> dots, bool adj, const int* done,
-               hipStream_t st, Timer* tm)
+               hipStream_t st, Timer* tm, bool mask = true)
     {
         long nblk = 0;
         Geo gw = g;
@@ -560,6 +560,7 @@ template <class R> struct Op {
         dd.dstr = multi ? 3 * g.V : 0;
         dd.conj = adj ? 1 : 0;
         dd.skip = lin.wskip;
+        dd.nomask = mask ? 0 : 1;
         t0(5);
         HIPCHK(run_zinv<R>(gw, W1, src, y, dd, flags, dots, tw[az], twist[az], twP[az], done, st, &nblk));
         t0(5);
@@ -670,6 +671,9 @@ struct vie_ctx {
     std::vector<double> btab;      // their eps tables (bnmat complex each)
     int bnmat = 0;
     double2* beps = nullptr;       // per-instance permittivity (V complex, post-processing scratch)
+    bool guess = false;            // the next solve starts from the guess in the VW vector
+    bool x_held = false;           // x64 holds a solution in the current layout (a guess for the next solve)
+    double dvg = 0.0;              // divergence stop factor of the solvers (0: off)
 };
 
 // Krylov state records of a batch are this many ints / complex values apart.
@@ -781,11 +785,14 @@ void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd, const int* mask = 
 // tolerance and limit.
 void state_init(vie_ctx* c, KState* s, double tol, int maxit, double pause = 0.0, bool per_inst = true)
 {
+    // the divergence stop applies to the solvers' own records (per_inst), not
+    // to the residual bookkeeping records
+    const double dvg = per_inst ? c->dvg : 0.0;
     if (c->in_batch) {
         hipLaunchKernelGGL(k_state_init_n, dim3((c->bn + 63) / 64), dim3(64), 0, c->stream, s, c->bn,
-                           per_inst ? c->btol : nullptr, per_inst ? c->bmaxit : nullptr, tol, maxit, pause);
+                           per_inst ? c->btol : nullptr, per_inst ? c->bmaxit : nullptr, tol, maxit, pause, dvg);
     } else {
-        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit, pause);
+        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, c->stream, s, tol, maxit, pause, dvg);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1073,7 +1080,8 @@ void require(vie_ctx* c, bool ok, const char* what)
 // ---------------------------------------------------------------- solvers
 // Vectors: index constants into the per-precision work-vector pool.
 // VT/VT2: staging of bare products; VG/VNU: adjoint right-hand side and solution.
-enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, NVEC };
+// VW/VWB: a warm start's guess and its correction's right-hand side (FP64).
+enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, VW, VWB, NVEC };
 
 // Drives iterations until the device flags done; `iter` enqueues one.  Two
 // batches of c->ahead iterations are in flight: the snapshot taken after one
@@ -1514,6 +1522,17 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         long nb = lincomb<double>(c, L, nullptr);
         scalar(c, ax, SC_BNRM, nb, 1);
     }
+    {   // a zero or non-finite right-hand side ends the solve here (x = 0)
+        const KState a0 = read_state(c, ax);
+        if (a0.status != KS_OK) {
+            KState h{};
+            h.status = a0.status;
+            h.done = 1;
+            *true_err = std::nan("");
+            *replacements = 0;
+            return h;
+        }
+    }
     state_init(c, st, tol, maxit, delta);  // err_ref = 1 until the first replacement
     // p, q and their products hold zero on inactive faces from here on: the
     // iterations leave those entries unwritten (wskip)
@@ -1825,6 +1844,11 @@ int status_of(vie_ctx* c, const KState& h)
         c->err = "non-finite residual at iteration " + std::to_string(h.iter);
         return VIE_ERR_NAN;
     }
+    if (h.status == KS_DIVERGED) {
+        c->err = "diverging: stop quantity " + std::to_string(h.err) + " at iteration " + std::to_string(h.iter) +
+                 " exceeds " + std::to_string(h.dvg) + " x its minimum " + std::to_string(h.emin);
+        return VIE_DIVERGED;
+    }
     if (!h.converged) c->err = "maxit reached after " + std::to_string(h.iter) + " iterations";
     return h.converged ? VIE_OK : VIE_NOT_CONVERGED;
 }
@@ -1862,12 +1886,13 @@ void timing_end(vie_ctx* c)
 }  // namespace
 
 namespace {
-template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjoint, hipStream_t st)
+template <class R> void apply_dev(vie_ctx* c, const cx<R>* x, cx<R>* y, int adjoint, hipStream_t st,
+                                  bool mask = true)
 {
     Op<R>* op = make_op<R>(c);
     const bool timed = c->timer.cls != 0;  // vie_time_kernel also covers bare products
     if (timed) timing_begin(c);
-    op->apply(lin1<R>(x), x, y, nodots<R>(), adjoint != 0, nullptr, st, timed ? &c->timer : nullptr);
+    op->apply(lin1<R>(x), x, y, nodots<R>(), adjoint != 0, nullptr, st, timed ? &c->timer : nullptr, mask);
     if (timed) {
         HIPCHK(hipStreamSynchronize(st));
         timing_end(c);
@@ -2024,6 +2049,67 @@ static void* scratch_ptr()
     return g_scratch[dev];
 }
 
+// ============================================================== warm start
+// A solve from a guess w (vie_set_guess) runs as the solve of the correction
+// A e = r0 = b - A w from e = 0, x = w + e; the correction's tolerance is tol
+// ||b|| / ||r0|| (squared ratios for CG's squared stop rule), so the stop rule
+// still reads ||b - A x|| / ||b||.  warm_begin computes r0 into VWB, swaps it
+// in as the resident right-hand side and returns ||r0|| / ||b|| (x64 = w and
+// no solve is needed when that already meets tol); warm_end swaps b back and
+// adds w to the correction.
+double warm_begin(vie_ctx* c)
+{
+    Op<double>* op = make_op<double>(c);
+    const long n = 3 * c->V;
+    double2* w = vec<double>(c, VW);
+    double2* rb = vec<double>(c, VWB);
+    KState* s = c->aux;
+    state_init(c, s, 0.0, 1, 0.0, false);
+    LinComb<double> L{};
+    L.n = n;
+    L.x[0] = c->b64;
+    L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
+    long nb = lincomb<double>(c, L, nullptr);
+    scalar(c, s, SC_BNRM, nb, 1);
+    op->apply(lin_k<double>(op->zr, w), w, c->t64, nodots<double>(), false, nullptr, c->stream, nullptr);
+    LinComb<double> M{};
+    M.n = n;
+    M.x[0] = c->b64;
+    M.x[1] = c->t64;
+    M.hc[0][0] = 1.0;
+    M.hc[1][0] = -1.0;
+    M.out = rb;
+    M.dots = dots_with<double>(c, 1, {0, 0}, nullptr, nullptr);
+    nb = lincomb<double>(c, M, nullptr);
+    scalar(c, s, SC_RELRES, nb, 1);
+    const KState h = read_state(c, s);
+    if (h.status != KS_OK) throw Err(h.status == KS_NAN ? VIE_ERR_NAN : VIE_BREAKDOWN, "zero or non-finite rhs");
+    std::swap(c->b64, *reinterpret_cast<double2**>(&vecs_of<double>(c)[VWB]));
+    return h.err;
+}
+void warm_end(vie_ctx* c)
+{
+    std::swap(c->b64, *reinterpret_cast<double2**>(&vecs_of<double>(c)[VWB]));
+    const long n = 3 * c->V;
+    LinComb<double> L{};  // x = e + w
+    L.n = n;
+    L.x[0] = c->x64;
+    L.x[1] = vec<double>(c, VW);
+    L.hc[0][0] = 1.0;
+    L.hc[1][0] = 1.0;
+    L.out = c->x64;
+    lincomb<double>(c, L, nullptr);
+    c->guess = false;
+}
+// The guess itself (it already meets the tolerance): x64 = w.
+void warm_take_guess(vie_ctx* c)
+{
+    std::swap(c->b64, *reinterpret_cast<double2**>(&vecs_of<double>(c)[VWB]));
+    HIPCHK(hipMemcpyAsync(c->x64, vec<double>(c, VW), 3 * c->V * sizeof(double2), hipMemcpyDeviceToDevice,
+                          c->stream));
+    c->guess = false;
+}
+
 // ================================================================ batches
 // Incident_Wave (vie_fft/GreenFunction.h:144-248) for wavenumber k0 and angles
 // in degrees, masked by the flags (vie_fft/CG_Iteration.h:559-562), into the
@@ -2166,7 +2252,7 @@ void batch_reserve(vie_ctx* c, int nb)
         c->snap = nullptr;
     }
     c->nbcap = nb;
-    c->rhs_ready = c->sol_ready = false;
+    c->rhs_ready = c->sol_ready = c->x_held = c->guess = false;
     c->state = dalloc<KState>(nb);
     c->aux = dalloc<KState>(nb);
     c->b64 = dalloc<double2>(n * nb);
@@ -2566,7 +2652,7 @@ static void set_frame(vie_ctx* c, const int* perm)
         ident = ident && perm[a] == a;
     }
     c->remap = c->emb || !ident;
-    c->green_ready = c->medium_ready = c->rhs_ready = c->sol_ready = false;
+    c->green_ready = c->medium_ready = c->rhs_ready = c->sol_ready = c->x_held = c->guess = false;
     const int* ni = c->desc.n;
     ensure_partial(c, 3 * std::max<long>(std::max(zinv_blocks<double>(Geo{ni[0], ni[1], ni[2], c->V}),
                                                   zinv_blocks<float>(Geo{ni[0], ni[1], ni[2], c->V})),
@@ -2752,7 +2838,7 @@ int vie_apply_async(vie_ctx* c, const void* x, void* y, int adjoint, void* strea
     }
 }
 
-int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
+static int apply_host(vie_ctx* c, const double* x, double* y, int adjoint, bool mask)
 {
     if (!c || !x || !y) return VIE_ERR_ARG;
     try {
@@ -2767,12 +2853,12 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
             copy_sync(c->t64, x, n * sizeof(double2), hipMemcpyHostToDevice, c->stream);
         }
         if (c->desc.precision == VIE_FP64) {
-            apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream);
+            apply_dev<double>(c, c->t64, c->r64, adjoint, c->stream, mask);
         } else {
             float2* xs = vec<float>(c, VT);
             float2* ys = vec<float>(c, VT2);
             hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->t64, xs);
-            apply_dev<float>(c, xs, ys, adjoint, c->stream);
+            apply_dev<float>(c, xs, ys, adjoint, c->stream, mask);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, ys, c->r64);
         }
         if (c->remap) {
@@ -2787,6 +2873,10 @@ int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint)
         return fail(c, e);
     }
 }
+
+int vie_apply(vie_ctx* c, const double* x, double* y, int adjoint) { return apply_host(c, x, y, adjoint, true); }
+
+int vie_apply_raw(vie_ctx* c, const double* x, double* y, int adjoint) { return apply_host(c, x, y, adjoint, false); }
 
 int vie_set_rhs_incident(vie_ctx* c, double theta, double phi, double psi)
 {
@@ -2832,6 +2922,20 @@ int vie_solve_resident(vie_ctx* c, int method, double tol, int maxit, vie_info* 
         timing_begin(c);
         auto t0 = std::chrono::steady_clock::now();
         KState h;
+        const bool warm = c->guess;
+        double e0 = 1.0;  // ||b - A x0|| / ||b|| of a guess x0
+        if (warm) {
+            require(c, method != VIE_BICG, "a guess needs a residual-norm stop rule (not BiCG's r^T r)");
+            e0 = warm_begin(c);
+            if ((method == VIE_CG ? e0 * e0 : e0) < tol) {  // the guess meets the tolerance: no iteration
+                warm_take_guess(c);
+                h = KState{};
+                h.converged = 1;
+                h.err = method == VIE_CG ? e0 * e0 : e0;
+                goto solved;
+            }
+            tol = method == VIE_CG ? tol / (e0 * e0) : tol / e0;
+        }
         if (c->desc.precision == VIE_FP64) {
             h = solve_method<double>(c, method, c->b64, c->x64, tol, maxit);
         } else {
@@ -2841,10 +2945,16 @@ int vie_solve_resident(vie_ctx* c, int method, double tol, int maxit, vie_info* 
             h = solve_method<float>(c, method, b32, x32, tol, maxit);
             hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, x32, c->x64);
         }
+        if (warm) {
+            warm_end(c);
+            h.err *= method == VIE_CG ? e0 * e0 : e0;  // the stop quantity relative to b
+        }
+    solved:
         HIPCHK(hipStreamSynchronize(c->stream));
         auto t1 = std::chrono::steady_clock::now();
         timing_end(c);
         c->sol_ready = true;
+        c->x_held = true;
         if (info) {
             info->iterations = h.iter;
             info->converged = h.converged;
@@ -2876,11 +2986,25 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
             auto t0 = std::chrono::steady_clock::now();
             double terr = 1.0;
             int nrep = 0;
-            KState h = solve_bicgstab_reliable(c, tol, maxit, delta, &terr, &nrep);
+            KState h{};
+            const bool warm = c->guess;
+            const double e0 = warm ? warm_begin(c) : 1.0;
+            if (warm && e0 < tol) {  // the guess meets the tolerance
+                warm_take_guess(c);
+                h.converged = 1;
+                terr = e0;
+            } else {
+                h = solve_bicgstab_reliable(c, tol / e0, maxit, delta, &terr, &nrep);
+                if (warm) {
+                    warm_end(c);
+                    terr *= e0;  // relative to b
+                }
+            }
             HIPCHK(hipStreamSynchronize(c->stream));
             auto t1 = std::chrono::steady_clock::now();
             timing_end(c);
             c->sol_ready = true;
+            c->x_held = true;
             if (info) {
                 info->iterations = h.iter;
                 info->converged = h.converged;
@@ -2913,12 +3037,18 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
         KState* s = c->aux;
         timing_begin(c);
         auto t0 = std::chrono::steady_clock::now();
-        // bnrm2 and x = 0
+        // bnrm2 and x = 0, or x = the guess (refinement starts from its residual)
+        const bool fresh = !c->guess;
+        if (!fresh) {
+            HIPCHK(hipMemcpyAsync(c->x64, vec<double>(c, VW), n * sizeof(double2), hipMemcpyDeviceToDevice,
+                                  c->stream));
+            c->guess = false;
+        }
         state_init(c, s, tol, 1);
         LinComb<double> L{};
         L.n = n;
         L.x[0] = c->b64;
-        L.out2 = c->x64;  // x = 0 * b
+        L.out2 = fresh ? c->x64 : nullptr;  // x = 0 * b
         L.dots = dots_with<double>(c, 1, {1, 1}, c->b64, nullptr);
         long nb = lincomb<double>(c, L, nullptr);
         scalar(c, s, SC_BNRM, nb, 1);
@@ -2931,7 +3061,7 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
         const int inner_cap = std::max(200, (int)std::min<long>(maxit, 4000));
         for (;;) {
             // r64 = b - A64 x64 (FP64 residual), ||r64||
-            if (outer == 0) {
+            if (outer == 0 && fresh) {
                 HIPCHK(hipMemcpyAsync(c->r64, c->b64, n * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
                 KState h = read_state(c, s);
                 if (h.status) throw Err(h.status == KS_NAN ? VIE_ERR_NAN : VIE_BREAKDOWN, "zero or non-finite rhs");
@@ -2993,6 +3123,7 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
         auto t1 = std::chrono::steady_clock::now();
         timing_end(c);
         c->sol_ready = true;
+        c->x_held = true;
         if (info) {
             info->iterations = total;
             info->converged = status == VIE_OK;
@@ -3005,6 +3136,40 @@ int vie_solve_mixed_resident(vie_ctx* c, int strategy, int inner_method, double 
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int vie_set_guess(vie_ctx* c, const double* x0)
+{
+    if (!c) return VIE_ERR_ARG;
+    try {
+        require(c, c->medium_ready, "vie_set_medium has not been called");
+        double2* w = vec<double>(c, VW);
+        const long n = 3 * c->V;
+        if (x0) {  // masked like the solution (it vanishes on inactive faces)
+            rhs_host(c, x0, w);
+        } else {
+            require(c, c->x_held, "no solution on the device to start from");
+            HIPCHK(hipMemcpyAsync(w, c->x64, n * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->guess = true;
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_solve_x0(vie_ctx* c, int method, double tol, int maxit, const double* b, double* x, vie_info* info)
+{
+    if (!c || !b || !x) return VIE_ERR_ARG;
+    int rc = vie_set_rhs(c, b);
+    if (rc != VIE_OK) return rc;
+    rc = vie_set_guess(c, x);
+    if (rc != VIE_OK) return rc;
+    rc = vie_solve_resident(c, method, tol, maxit, info);
+    if (rc != VIE_OK && rc != VIE_NOT_CONVERGED && rc != VIE_BREAKDOWN) return rc;
+    int rc2 = vie_get_solution(c, x);
+    return rc2 != VIE_OK ? rc2 : rc;
 }
 
 int vie_get_solution(vie_ctx* c, double* x)
@@ -3160,6 +3325,7 @@ int vie_batch_solve(vie_ctx* c, int nb, int strategy, int method, double tol, in
         const std::vector<KState> hs = c->bstate;
         if (strategy != VIE_BATCH_MIXED) true_relres_batch(c, tres);
         c->sol_ready = true;
+        c->x_held = true;
         int worst = VIE_OK;
         std::string first_err;
         for (int i = 0; i < nb; ++i) {
@@ -3543,6 +3709,32 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
             info->true_relres = -1.0;
         }
         return status_of(c, h);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_divergence_stop(vie_ctx* c, double factor)
+{
+    if (!c) return VIE_ERR_ARG;
+    if (!(factor == 0.0 || (factor >= 1.0 && std::isfinite(factor)))) {
+        c->err = "divergence factor must be 0 (off) or a finite value >= 1";
+        return VIE_ERR_ARG;
+    }
+    c->dvg = factor;
+    return VIE_OK;
+}
+
+int vie_history(vie_ctx* c, int max, double* err, int* count)
+{
+    if (!c || max < 0 || (max > 0 && !err) || !count) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solve has run");
+        KState h = read_state(c, c->state);
+        const int n = std::min(std::min(h.iter, MAX_HIST), max);
+        if (n > 0) copy_sync(err, c->hist, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        *count = std::min(h.iter, MAX_HIST);
+        return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);
     }

@@ -285,20 +285,21 @@ __global__ void k_state_resume(KState* s, double pause)
 }
 
 // KState initialisation from the host-side tolerance, limit and pause level.
-__global__ void k_state_init(KState* s, double tol, int maxit, double pause)
+__global__ void k_state_init(KState* s, double tol, int maxit, double pause, double dvg)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         KState z = {};
         z.tol = tol;
         z.maxit = maxit;
         z.pause = pause;
+        z.dvg = dvg;
         *s = z;
     }
 }
 // The same for nb instances: tolerance and limit tolv[i] and maxitv[i], or
 // tol and maxit for every instance when those are nullptr.
 __global__ void k_state_init_n(KState* s, int nb, const double* tolv, const int* maxitv, double tol, int maxit,
-                               double pause)
+                               double pause, double dvg)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nb) {
@@ -306,6 +307,7 @@ __global__ void k_state_init_n(KState* s, int nb, const double* tolv, const int*
         z.tol = tolv ? tolv[i] : tol;
         z.maxit = maxitv ? maxitv[i] : maxit;
         z.pause = pause;
+        z.dvg = dvg;
         s[i] = z;
     }
 }

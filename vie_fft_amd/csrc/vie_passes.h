@@ -114,6 +114,7 @@ template <class R> struct Diag {
     int conj;             // adjoint: use conj(d_f)
     int skip;             // y already holds zero on inactive faces: leave them unwritten
     long dstr = 0;        // batches: instance i reads d + i dstr (0: shared)
+    int nomask = 0;       // every face counts as active: no output mask (Current_Reset left out)
 };
 
 // Block-wide sum of a double2; sh holds NT/64 entries.  Result valid in thread 0.
@@ -575,7 +576,7 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
     const long ioff = (fb - f) * g.V;
     auto term = [&](long ia, T out) {  // y = flag (out + d x), then its dot products
         const long idx = ia - ioff;
-        const bool on = flags[idx] != 0;
+        const bool on = dg.nomask || flags[idx] != 0;
         T yv = czero<T>();
         if (on) {  // d and x are read only where the face is active
             const T dv = dg.conj ? cconj(dgd[idx]) : dgd[idx];

@@ -27,9 +27,12 @@ struct KState {
     // replacements and its last true FP64 residual
     int fin, nrep;
     double terr;
+    // divergence stop (0: off): the solve ends once its stop quantity exceeds
+    // dvg times the smallest value it had (emin)
+    double dvg, emin;
 };
 
-enum { KS_OK = 0, KS_BREAKDOWN = 5, KS_NAN = 6 };
+enum { KS_OK = 0, KS_BREAKDOWN = 5, KS_NAN = 6, KS_DIVERGED = 8 };
 
 // Coefficient offsets per solver (see the host drivers).
 enum {
@@ -161,6 +164,14 @@ __device__ __forceinline__ void ks_finish(KState* s, double err, double* hist)
     s->iter += 1;
     if (!isfinite(err)) { ks_fail(s, KS_NAN); return; }
     if (err < s->tol) { s->converged = 1; s->done = 1; return; }
+    if (s->dvg > 0.0) {
+        if (s->iter == 1 || err < s->emin) {
+            s->emin = err;
+        } else if (err > s->dvg * s->emin) {
+            ks_fail(s, KS_DIVERGED);
+            return;
+        }
+    }
     if (err < s->pause || s->iter >= s->maxit) s->done = 1;
 }
 

@@ -541,7 +541,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = t + T * b + q * (P / r1);
-                if (n < N && n + zs < nzr) fl[b * r1 + q] = flags[base + n];
+                if (n < N && n + zs < nzr) fl[b * r1 + q] = dg.nomask ? 1 : flags[base + n];
             }
 #pragma unroll
         for (int b = 0; b < E / r1; ++b) {

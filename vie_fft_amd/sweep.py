@@ -93,7 +93,7 @@ def merge_point_files(out_dir, count):
 
 
 def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=1, active=None, maxit=0,
-              out_dir=None, log=None):
+              out_dir=None, log=None, warm=False):
     """Solve every (wavelength_nm, psi) of `points` on one resident context.
 
     scene_at(wavelength_nm, psi) -> geometry.Scene (same grid and cells for
@@ -103,12 +103,15 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
     point of each wavelength also holds the context's plan (Solver.plan).  With
     out_dir the per-wavelength files are written (first polarisation of each
     wavelength, as the reference writes one file per process) and merged.
+    With warm=True every solve starts from the previous wavelength's solution
+    of the same polarisation (``vie_set_guess``) instead of x = 0.
     """
     results = []
     s = None
     cur = None
     material = flags = None
     wl_index = {}
+    prev = {}  # psi -> the last solution of that polarisation (warm starts)
     try:
         for lam, psi in points:
             sc = scene_at(lam, psi)
@@ -130,8 +133,12 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
                 plan = None
             t0 = time.perf_counter()
             s.set_rhs_incident(sc.theta, sc.phi, psi)
+            if warm and psi in prev:
+                s.set_guess(prev[psi])
             info = s.solve_mixed(tol=tol, maxit=maxit) if mixed else s.solve_resident(method, tol=tol, maxit=maxit)
             solve_s = time.perf_counter() - t0
+            if warm:
+                prev[psi] = s.solution()
             q_ext = s.forward_scatter(sc.theta, sc.phi, psi) / sc.area
             q_abs = s.absorbed(metal) * 120.0 * np.pi / sc.area
             act = 3.0 * s.absorbed(active) / vol_active if active is not None and vol_active > 0 else 0.0
@@ -228,10 +235,12 @@ def main():
     ap.add_argument("--fp64", action="store_true", help="plain FP64 solves instead of mixed precision")
     ap.add_argument("--out", default=None, help="directory for the reference-format result files")
     ap.add_argument("--sequential", action="store_true", help="one solve after the other instead of one batch")
+    ap.add_argument("--warm", action="store_true",
+                    help="with --sequential: start each solve from the previous wavelength's solution")
     args = ap.parse_args()
     points = G.dimer_sweep(args.count)
     t0 = time.perf_counter()
-    kw = {} if args.sequential else {"timings": {}}
+    kw = {"warm": args.warm} if args.sequential else {"timings": {}}
     run = run_sweep if args.sequential else run_sweep_batched
     res = run(lambda lam, psi: G.dimer_case(lam, psi=psi, n=args.n), points, tol=args.tol,
               mixed=not args.fp64, out_dir=args.out, log=lambda r: print(json.dumps(r), flush=True), **kw)

@@ -21,9 +21,9 @@ FP64, FP32 = 0, 1
 MIXED_REFINE, MIXED_RELIABLE = 0, 1
 CG, BICG, BICGSTAB, CORS, BICORSTAB = 1, 2, 3, 4, 5
 METHODS = {"cg": CG, "bicg": BICG, "bicgstab": BICGSTAB, "cors": CORS, "bicorstab": BICORSTAB}
-OK, ERR_ARG, ERR_DEVICE, ERR_OOM, NOT_CONVERGED, BREAKDOWN, ERR_NAN, ERR_UNSUPPORTED = range(8)
+OK, ERR_ARG, ERR_DEVICE, ERR_OOM, NOT_CONVERGED, BREAKDOWN, ERR_NAN, ERR_UNSUPPORTED, DIVERGED = range(9)
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_OOM", 4: "NOT_CONVERGED", 5: "BREAKDOWN",
-          6: "ERR_NAN", 7: "ERR_UNSUPPORTED"}
+          6: "ERR_NAN", 7: "ERR_UNSUPPORTED", 8: "DIVERGED"}
 
 # every symbol include/viefft.h declares
 EXPORTS = [
@@ -35,7 +35,8 @@ EXPORTS = [
     "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
     "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
-    "vie_batch_absorbed", "vie_far_field_noalloc",
+    "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
+    "vie_history", "vie_set_divergence_stop",
 ]
 
 
@@ -89,11 +90,16 @@ def load():
     L.vie_green_compute.argtypes = [vp]
     L.vie_incident.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double, dp]
     L.vie_apply.argtypes = [vp, dp, dp, ct.c_int]
+    L.vie_apply_raw.argtypes = [vp, dp, dp, ct.c_int]
     L.vie_apply_async.argtypes = [vp, vp, vp, ct.c_int, vp]
     L.vie_solve.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, ct.POINTER(Info)]
     L.vie_set_rhs.argtypes = [vp, dp]
     L.vie_solve_resident.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(Info)]
     L.vie_get_solution.argtypes = [vp, dp]
+    L.vie_set_guess.argtypes = [vp, dp]
+    L.vie_history.argtypes = [vp, ct.c_int, dp, ct.POINTER(ct.c_int)]
+    L.vie_set_divergence_stop.argtypes = [vp, ct.c_double]
+    L.vie_solve_x0.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, ct.POINTER(Info)]
     L.vie_solve_mixed_resident.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                                            ct.POINTER(Info)]
     L.vie_forward_scatter.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double, dp]
@@ -215,6 +221,13 @@ class Solver:
         self._check(self.lib.vie_apply(self.h, _dp(x), _dp(y), int(adjoint)))
         return y
 
+    def apply_raw(self, x, adjoint=False):
+        """A x or A^H x without the output mask (X_FFT/Y_FFT/Z_FFT before Current_Reset)."""
+        x = self._field(x)
+        y = np.zeros_like(x)
+        self._check(self.lib.vie_apply_raw(self.h, _dp(x), _dp(y), int(adjoint)))
+        return y
+
     def apply_async(self, x_ptr, y_ptr, adjoint=False, stream=None):
         """Device-pointer product (e.g. torch tensors' data_ptr()), enqueued on stream."""
         self._check(self.lib.vie_apply_async(self.h, ct.c_void_p(x_ptr), ct.c_void_p(y_ptr), int(adjoint),
@@ -230,6 +243,24 @@ class Solver:
         self._check(self.lib.vie_solve(self.h, int(method), float(tol), int(maxit), _dp(b), _dp(x),
                                        ct.byref(info)), allow=(OK, NOT_CONVERGED))
         return x, info.as_dict()
+
+    def solve_x0(self, b, x0, method=BICGSTAB, tol=1e-3, maxit=0):
+        """vie_solve from the initial guess x0: returns (x, info)."""
+        method = METHODS.get(method, method) if isinstance(method, str) else method
+        b = self._field(b)
+        x = np.array(self._field(x0), copy=True)
+        info = Info()
+        self._check(self.lib.vie_solve_x0(self.h, int(method), float(tol), int(maxit), _dp(b), _dp(x),
+                                          ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        return x, info.as_dict()
+
+    def set_guess(self, x0=None):
+        """Initial guess of the next resident solve: x0, or the resident solution (None)."""
+        if x0 is None:
+            self._check(self.lib.vie_set_guess(self.h, None))
+        else:
+            x = self._field(x0)
+            self._check(self.lib.vie_set_guess(self.h, _dp(x)))
 
     def solve_batched(self, B, method=BICGSTAB, tol=1e-3, maxit=0, mixed=False):
         """Solve for several right-hand sides B (nrhs, 3, Nx, Ny, Nz): returns (X, infos)."""
@@ -386,6 +417,17 @@ class Solver:
         return v.value
 
     # -- instrumentation -----------------------------------------------------
+    def set_divergence_stop(self, factor):
+        """Later solves end with DIVERGED once the stop quantity exceeds factor x its minimum (0: off)."""
+        self._check(self.lib.vie_set_divergence_stop(self.h, float(factor)))
+
+    def history(self, max_n=65536):
+        """Stop quantity after each iteration of the last single solve."""
+        buf = np.zeros(max_n)
+        cnt = ct.c_int()
+        self._check(self.lib.vie_history(self.h, int(max_n), _dp(buf), ct.byref(cnt)))
+        return buf[: min(cnt.value, max_n)].copy()
+
     def time_kernel(self, cls):
         self._check(self.lib.vie_time_kernel(self.h, int(cls)))
 
