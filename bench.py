@@ -177,6 +177,7 @@ def main():
     s.solve_resident(VF.BICGSTAB, tol=args.tol)
     n3, ms3_64 = s.kernel_time()
     s.time_kernel(0)
+    tuning = s.tuning()
 
     it64 = infos64[-1]["iterations"]
     itm = infosm[-1]["iterations"]
@@ -225,7 +226,8 @@ def main():
                  "y_window": list(plan["y_window"]), "y_pencil": plan["y_pencil"],
                  "z_window": list(plan["z_window"]), "z_pencil": plan["z_pencil"],
                  "x_pencil": nxp, "grid": list(plan["grid"]),
-                 "product_bytes_f32": product_bytes(plan, 8.0), "product_bytes_f64": product_bytes(plan, 16.0)},
+                 "product_bytes_f32": product_bytes(plan, 8.0), "product_bytes_f64": product_bytes(plan, 16.0),
+                 "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": XPASS_F32_128_TRAFFIC if plan == XPASS_F32_128_PLAN else None,
@@ -302,13 +304,15 @@ def spheres_256_leg():
         i64 = s.solve_resident(VF.BICGSTAB, tol=1e-6)
         t64 = time.perf_counter() - t0
         plan = s.plan()
+        tuning = s.tuning()
     dm, _ = solve_models(plan, 8.0, im[-1]["iterations"], im[-1]["outer"])
     d6, _ = solve_models(plan, 16.0, i64["iterations"])
     return {"mixed_s": tm, "mixed_iterations": im[-1]["iterations"], "mixed_replacements": im[-1]["outer"],
             "mixed_us_per_iteration": tm * 1e6 / im[-1]["iterations"], "mixed_true_relres": im[-1]["true_relres"],
             "fp64_s": t64, "fp64_iterations": i64["iterations"], "fp64_us_per_iteration": t64 * 1e6 / i64["iterations"],
             "fp64_true_relres": i64["true_relres"], "mixed_dense_frac": dm / tm / HBM_PEAK,
-            "fp64_dense_frac": d6 / t64 / HBM_PEAK, "x_window": list(plan["x_window"]), "x_pencil": plan["x_pencil"]}
+            "fp64_dense_frac": d6 / t64 / HBM_PEAK, "x_window": list(plan["x_window"]), "x_pencil": plan["x_pencil"],
+            "tuning": tuning}
 
 
 def solve_roofline(plan, tm, itm, outer, t64, it64):

@@ -320,6 +320,17 @@ double vie_model_bytes_matvec(const vie_ctx* ctx);
    pencil length (the short z period, or the internal Nz).  out holds 15
    ints. */
 int vie_plan(const vie_ctx* ctx, int* out);
+/* The autotuner's plan of the FP64 (out[0..4], us[0]) and FP32 (out[5..9],
+   us[1]) operators (no counterpart in the reference, whose MKL plans are
+   built per call, vie_fft/fft.h:65-73): tuned (1; 0 fixed defaults, -1 no
+   such operator), pass variants (bit 0 register z forward, bit 1 LDS-staged
+   y/z passes, bit 2 LDS-staged x pass; -1 defaults), the x, y and z pencil
+   lengths of the windowed products; us (may be NULL, 4 values) the measured
+   microseconds per product of the FP64 and FP32 plans, then of the fixed
+   defaults (the tuner's starting point).  vie_set_medium picks the plan by timing the
+   candidates on the device once per geometry and process (VIE_TUNE=0: the
+   fixed defaults; VIE_TUNE_CACHE=file keeps plans across processes). */
+int vie_tuning(const vie_ctx* ctx, int* out, double* us);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,11 @@ oracle's, which works on the caller's grid with every plane:
   periods);
 * an embedded grid (lengths without compiled kernels) with a slab;
 * the same solves with VIE_WINDOW=0 (caller's axis order, every plane).
+
+These tests pin the fixed short-period table and the pass paths it selects
+(some through the VIE_*SHORT overrides), so the autotuner is off here
+(VIE_TUNE=0); tuned plans are checked in test_gpu_tuning.py and run in every
+other module.
 """
 import os
 
@@ -26,6 +31,11 @@ from tests.gpu_util import random_field, rel
 from vie_fft_amd import viefft as VF
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _fixed_plans(monkeypatch):
+    monkeypatch.setenv("VIE_TUNE", "0")
 
 N = (16, 12, 20)
 D = (0.02, 0.025, 0.03)

@@ -23,7 +23,9 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <cstdio>
 #include <iterator>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -175,6 +177,14 @@ template <class R> struct Op {
     // lengths bl[3] along x, y, z): G~ octants (gob values each), forward Vv
     // and adjoint U tables (2 lb values per axis each), D_f forward and
     // adjoint (3 each) and d_f (3V each), for nbt instances.
+    // tuned plan (autotuner): pass variants of every product (Geo::var) and
+    // the short periods of the windowed products; var -1 and tuned false:
+    // the fixed defaults
+    int var = -1;
+    bool tuned = false;
+    int tns = 0, tnys = 0, tnzs = 0;
+    double tus = 0.0;      // microseconds per windowed product of the plan
+    double tus_def = 0.0;  // and of the fixed defaults (measured when tuning)
     const Bat* bt = nullptr;
     bool bmulti = false;
     int nbt = 0, lb = 0, bl[3] = {0, 0, 0};
@@ -496,6 +506,7 @@ template <class R> struct Op {
     {
         long nblk = 0;
         Geo gw = g;
+        gw.var = var;
         if (lin.zr) {
             gw.x0 = wx0;
             gw.x1 = wx1;
@@ -696,6 +707,11 @@ template <> Op<float>* op_of<float>(vie_ctx* c) { return c->op32; }
 template <class R> std::vector<void*>& vecs_of(vie_ctx* c);
 template <> std::vector<void*>& vecs_of<double>(vie_ctx* c) { return c->vec64; }
 template <> std::vector<void*>& vecs_of<float>(vie_ctx* c) { return c->vec32; }
+
+// Vectors: index constants into the per-precision work-vector pool.
+// VT/VT2: staging of bare products; VG/VNU: adjoint right-hand side and
+// solution; VW/VWB: a warm start's guess and its correction's right-hand side.
+enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, VW, VWB, NVEC };
 
 // k-th solver work vector (3V entries) of precision R, allocated on demand
 template <class R> cx<R>* vec(vie_ctx* c, int k)
@@ -999,31 +1015,271 @@ void short_octant(vie_ctx* c, int px, int py, int pz, double2* g64, float2* g32)
     HIPCHK(hipStreamSynchronize(c->stream));
 }
 
-// Short periods of every operator for the current windows, with their G~
-// octants once G~ is computed.
-void short_sync(vie_ctx* c)
+// Short periods of an operator for the current windows (its tuned plan's,
+// else the shortest lengths with register passes), with their G~ octants once
+// G~ is computed.
+template <class R> void short_set(vie_ctx* c, Op<R>* op, int ns, int nsy, int nsz)
 {
-    Op<double>* o64 = c->op64;
-    Op<float>* o32 = c->op32;
-    if (!o64 && !o32) return;
-    const int wx = o64 ? o64->wx1 - o64->wx0 : o32->wx1 - o32->wx0;
-    const int wy = o64 ? o64->wy1 - o64->wy0 : o32->wy1 - o32->wy0;
-    const int wz = o64 ? o64->wz1 - o64->wz0 : o32->wz1 - o32->wz0;
-    const int ns = c->medium_ready ? short_len(c, wx, 0) : 0;
-    const int nsy = c->medium_ready ? short_len(c, wy, 1) : 0;
-    const int nsz = c->medium_ready ? short_len(c, wz, 2) : 0;
     const int* n = c->desc.n;
     if (nsz) {  // partial sums of the z inverse at the short z length
         Geo gz{n[0], n[1], nsz, c->V};
-        ensure_partial(c, 3 * std::max(zinv_blocks<double>(gz), zinv_blocks<float>(gz)));
+        ensure_partial(c, 3L * c->nbcap * std::max<long>(std::max(zinv_blocks<double>(gz), zinv_blocks<float>(gz)), 4096));
     }
-    if (o64) o64->set_short(ns, nsy, nsz, c->stream);
-    if (o32) o32->set_short(ns, nsy, nsz, c->stream);
+    op->set_short(ns, nsy, nsz, c->stream);
     if (!c->green_ready) return;
-    if (ns) short_octant(c, ns, n[1], n[2], o64 ? o64->Gs : nullptr, o32 ? o32->Gs : nullptr);
-    if (nsy || nsz)
-        short_octant(c, ns ? ns : n[0], nsy ? nsy : n[1], nsz ? nsz : n[2], o64 ? o64->Gw : nullptr,
-                     o32 ? o32->Gw : nullptr);
+    double2* g64 = nullptr;
+    float2* g32 = nullptr;
+    if constexpr (sizeof(R) == 8) g64 = op->Gs; else g32 = op->Gs;
+    if (ns) short_octant(c, ns, n[1], n[2], g64, g32);
+    if (nsy || nsz) {
+        g64 = nullptr;
+        g32 = nullptr;
+        if constexpr (sizeof(R) == 8) g64 = op->Gw; else g32 = op->Gw;
+        short_octant(c, ns ? ns : n[0], nsy ? nsy : n[1], nsz ? nsz : n[2], g64, g32);
+    }
+}
+
+template <class R> void short_plan(vie_ctx* c, Op<R>* op, int* ns, int* nsy, int* nsz)
+{
+    if (op->tuned) {
+        *ns = op->tns;
+        *nsy = op->tnys;
+        *nsz = op->tnzs;
+        return;
+    }
+    const bool m = c->medium_ready;
+    *ns = m ? short_len(c, op->wx1 - op->wx0, 0) : 0;
+    *nsy = m ? short_len(c, op->wy1 - op->wy0, 1) : 0;
+    *nsz = m ? short_len(c, op->wz1 - op->wz0, 2) : 0;
+}
+
+void short_sync(vie_ctx* c)
+{
+    int a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    if (c->op64) short_plan<double>(c, c->op64, &a[0], &a[1], &a[2]);
+    if (c->op32) short_plan<float>(c, c->op32, &b[0], &b[1], &b[2]);
+    // one FP64 octant computation serves both operators when their lengths agree
+    if (c->op64 && c->op32 && a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && c->green_ready) {
+        const int* n = c->desc.n;
+        short_set<double>(c, c->op64, a[0], a[1], a[2]);
+        c->op32->set_short(b[0], b[1], b[2], c->stream);
+        if (b[0]) hipLaunchKernelGGL(k_convert<float>, dim3(grid1((long)(b[0] + 1) * (n[1] + 1) * (n[2] + 1), 256, 1L << 30)),
+                                     dim3(256), 0, c->stream, (long)(b[0] + 1) * (n[1] + 1) * (n[2] + 1), c->op64->Gs,
+                                     c->op32->Gs);
+        if (b[1] || b[2]) {
+            const long t = (long)((b[0] ? b[0] : n[0]) + 1) * ((b[1] ? b[1] : n[1]) + 1) * ((b[2] ? b[2] : n[2]) + 1);
+            hipLaunchKernelGGL(k_convert<float>, dim3(grid1(t, 256, 1L << 30)), dim3(256), 0, c->stream, t,
+                               c->op64->Gw, c->op32->Gw);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return;
+    }
+    if (c->op64) short_set<double>(c, c->op64, a[0], a[1], a[2]);
+    if (c->op32) short_set<float>(c, c->op32, b[0], b[1], b[2]);
+}
+
+// ================================================================ autotuner
+// Each operator's plan -- the variant of every pass (register-resident or
+// LDS-staged x pass, y/z passes and z forward, Geo::var) and the short
+// periods of its windowed products along x, y and z -- is chosen by timing
+// the solvers' windowed product on the device for the candidates: the eight
+// variant combinations, then, axis by axis, the full length and the compiled
+// lengths from the window width up, then the variants again.  Plans depend
+// on the grid, the precision and the three windows only (not on eps or k0),
+// so they are kept per process for every geometry met (and in the file
+// VIE_TUNE_CACHE names, if set).  VIE_TUNE=0 keeps the fixed defaults.
+struct TunePlan {
+    int var, ns, nys, nzs;
+    double us, us_def;
+};
+static std::mutex g_tune_mu;
+static std::map<std::string, TunePlan> g_tune;
+static bool g_tune_loaded = false;
+
+static bool tuning_on()
+{
+    const char* e = std::getenv("VIE_TUNE");
+    return !(e && e[0] == '0');
+}
+
+template <class R> static std::string tune_key(const vie_ctx* c, const Op<R>* op)
+{
+    const int* n = c->desc.n;
+    char b[160];
+    std::snprintf(b, sizeof b, "%s %d %d %d %d %d %d %d %d %d", sizeof(R) == 8 ? "f64" : "f32", n[0], n[1], n[2],
+                  op->wx0, op->wx1, op->wy0, op->wy1, op->wz0, op->wz1);
+    return b;
+}
+
+static void tune_cache_load()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char* path = std::getenv("VIE_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+        char pr[8];
+        int v[10];
+        TunePlan p;
+        if (std::sscanf(line, "%7s %d %d %d %d %d %d %d %d %d | %d %d %d %d %lf %lf", pr, &v[0], &v[1], &v[2], &v[3],
+                        &v[4], &v[5], &v[6], &v[7], &v[8], &p.var, &p.ns, &p.nys, &p.nzs, &p.us, &p.us_def) == 16) {
+            char key[160];
+            std::snprintf(key, sizeof key, "%s %d %d %d %d %d %d %d %d %d", pr, v[0], v[1], v[2], v[3], v[4], v[5],
+                          v[6], v[7], v[8]);
+            g_tune[key] = p;
+        }
+    }
+    std::fclose(f);
+}
+
+static void tune_cache_store(const std::string& key, const TunePlan& p)
+{
+    const char* path = std::getenv("VIE_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s | %d %d %d %d %.3f %.3f\n", key.c_str(), p.var, p.ns, p.nys, p.nzs, p.us, p.us_def);
+    std::fclose(f);
+}
+
+// microseconds per windowed product (the solvers' kind) with the operator's
+// current plan: the best of two rounds of `reps` products
+template <class R> static double time_products(vie_ctx* c, Op<R>* op, int reps)
+{
+    const long n = 3 * c->V;
+    cx<R>* x = vec<R>(c, VT);
+    cx<R>* y = vec<R>(c, VT2);
+    fill0<R>(c, x, n);
+    fill0<R>(c, y, n);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e30;
+    try {
+        op->apply(lin_kw<R>(op->zr, x), x, y, nodots<R>(), false, nullptr, c->stream, nullptr);
+        for (int round = 0; round < 2; ++round) {
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int k = 0; k < reps; ++k)
+                op->apply(lin_kw<R>(op->zr, x), x, y, nodots<R>(), false, nullptr, c->stream, nullptr);
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, 1e3 * ms / reps);
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+// compiled lengths a window of w planes along an axis of N may use: the full
+// length (0) and the supported lengths from w up to (but not including) N,
+// the smallest three; z short periods need an even N
+static std::vector<int> short_candidates(int w, int N, int axis)
+{
+    std::vector<int> v{0};
+    if (w >= N || (axis == 2 && N % 2)) return v;
+    for (int m = w; m < N && v.size() < 4; ++m)
+        if (size_supported(m)) v.push_back(m);
+    return v;
+}
+
+template <class R> void tune_op(vie_ctx* c, Op<R>* op)
+{
+    op->tuned = false;
+    op->var = -1;
+    if (!tuning_on() || !c->medium_ready) return;
+    const std::string key = tune_key(c, op);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_cache_load();
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            op->var = it->second.var;
+            op->tns = it->second.ns;
+            op->tnys = it->second.nys;
+            op->tnzs = it->second.nzs;
+            op->tus = it->second.us;
+            op->tus_def = it->second.us_def;
+            op->tuned = true;
+            return;
+        }
+    }
+    const bool green = c->green_ready;
+    c->green_ready = false;  // timing does not need G~ values: skip the octants while trying
+    try {
+        // start from the defaults
+        int ns = 0, nys = 0, nzs = 0;
+        short_plan<R>(c, op, &ns, &nys, &nzs);
+        op->set_short(ns, nys, nzs, c->stream);
+        // reps: about 2 ms of products per round
+        op->var = -1;
+        const double t0 = time_products<R>(c, op, 2);
+        const int reps = std::max(2, std::min(20, (int)(2000.0 / std::max(t0, 1.0))));
+        op->tus_def = time_products<R>(c, op, reps);  // the fixed defaults, for comparison
+        auto best_var = [&](double* tbest) {
+            int bv = 0;
+            double bt = 1e30;
+            for (int v = 0; v < 8; ++v) {
+                op->var = v;
+                const double t = time_products<R>(c, op, reps);
+                if (t < bt) {
+                    bt = t;
+                    bv = v;
+                }
+            }
+            op->var = bv;
+            *tbest = bt;
+        };
+        double tb = 0;
+        best_var(&tb);
+        const int w[3] = {op->wx1 - op->wx0, op->wy1 - op->wy0, op->wz1 - op->wz0};
+        int len[3] = {ns, nys, nzs};
+        for (int a = 0; a < 3; ++a) {
+            int bl = len[a];
+            for (int m : short_candidates(w[a], c->desc.n[a], a)) {
+                if (m == len[a]) continue;
+                int l2[3] = {len[0], len[1], len[2]};
+                l2[a] = m;
+                if (l2[2]) {
+                    Geo gz{c->desc.n[0], c->desc.n[1], l2[2], c->V};
+                    ensure_partial(c, 3L * c->nbcap *
+                                          std::max<long>(std::max(zinv_blocks<double>(gz), zinv_blocks<float>(gz)), 4096));
+                }
+                op->set_short(l2[0], l2[1], l2[2], c->stream);
+                const double t = time_products<R>(c, op, reps);
+                if (t < tb) {
+                    tb = t;
+                    bl = m;
+                }
+            }
+            len[a] = bl;
+            op->set_short(len[0], len[1], len[2], c->stream);
+        }
+        best_var(&tb);
+        op->tns = len[0];
+        op->tnys = len[1];
+        op->tnzs = len[2];
+        op->tus = tb;
+        op->tuned = true;
+    } catch (...) {
+        c->green_ready = green;
+        throw;
+    }
+    c->green_ready = green;
+    TunePlan p{op->var, op->tns, op->tnys, op->tnzs, op->tus, op->tus_def};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = p;
+    tune_cache_store(key, p);
 }
 
 template <> Op<double>* make_op<double>(vie_ctx* c)
@@ -1042,6 +1298,7 @@ template <> Op<double>* make_op<double>(vie_ctx* c)
                       hipMemcpyDeviceToDevice, c->stream);
         if (c->medium_ready) {
             upload_medium<double>(c, c->op64);
+            tune_op<double>(c, c->op64);
             short_sync(c);
         }
     }
@@ -1065,6 +1322,7 @@ template <> Op<float>* make_op<float>(vie_ctx* c)
         }
         if (c->medium_ready) {
             upload_medium<float>(c, c->op32);
+            tune_op<float>(c, c->op32);
             short_sync(c);
         }
     }
@@ -1078,11 +1336,6 @@ void require(vie_ctx* c, bool ok, const char* what)
 }
 
 // ---------------------------------------------------------------- solvers
-// Vectors: index constants into the per-precision work-vector pool.
-// VT/VT2: staging of bare products; VG/VNU: adjoint right-hand side and solution.
-// VW/VWB: a warm start's guess and its correction's right-hand side (FP64).
-enum { VB = 0, VX, VR, VP, VQ, VAP, VAQ, VRC, VQC, VPC, VT, VT2, VG, VNU, VW, VWB, NVEC };
-
 // Drives iterations until the device flags done; `iter` enqueues one.  Two
 // batches of c->ahead iterations are in flight: the snapshot taken after one
 // is read while the next runs, so the device never waits for the host (the
@@ -2629,6 +2882,8 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
     c->brhs.assign(c->nbcap, 0);
     if (c->op64) upload_medium<double>(c, c->op64);
     if (c->op32) upload_medium<float>(c, c->op32);
+    if (c->op64) tune_op<double>(c, c->op64);
+    if (c->op32) tune_op<float>(c, c->op32);
     short_sync(c);
     return VIE_OK;
 }
@@ -3712,6 +3967,23 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int vie_tuning(const vie_ctx* c, int* out, double* us)
+{
+    if (!c || !out) return VIE_ERR_ARG;
+    auto put = [&](const auto* op, int* o, double* u, double* ud) {
+        o[0] = op ? (op->tuned ? 1 : 0) : -1;
+        o[1] = op ? op->var : -1;
+        o[2] = op ? (op->nxs ? op->nxs : c->desc.n[0]) : 0;
+        o[3] = op ? (op->nys ? op->nys : c->desc.n[1]) : 0;
+        o[4] = op ? (op->nzs ? op->nzs : c->desc.n[2]) : 0;
+        if (u) *u = op ? op->tus : 0.0;
+        if (ud) *ud = op ? op->tus_def : 0.0;
+    };
+    put(c->op64, out, us, us ? us + 2 : nullptr);
+    put(c->op32, out + 5, us ? us + 1 : nullptr, us ? us + 3 : nullptr);
+    return VIE_OK;
 }
 
 int vie_set_divergence_stop(vie_ctx* c, double factor)

@@ -64,13 +64,20 @@ template <class R> static bool zfwd_reg()
     return v < 0 ? sizeof(R) == 8 : v == 1;
 }
 
+// The variant a launch uses: the Geo's tuned plan, else the fixed defaults
+// (and their VIE_* overrides).
+template <class R> static bool use_zfwd_reg(const Geo& g) { return g.var < 0 ? zfwd_reg<R>() : (g.var & 1) != 0; }
+static bool use_yz_lds(const Geo& g) { return g.var < 0 ? yz_lds_forced() : (g.var & 2) != 0; }
+static bool xpass_lds_forced();
+static bool use_x_lds(const Geo& g) { return g.var < 0 ? xpass_lds_forced() : (g.var & 4) != 0; }
+
 template <class R, int n, bool ZS>
 static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const cx<R>* tw, const cx<R>* twist,
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
     const long rows = win_rows(g);  // the rows of the x and y windows
     if constexpr (YZReg<R, n>::ok) {
-        if (zfwd_reg<R>() && !yz_lds_forced()) {
+        if (use_zfwd_reg<R>(g) && !use_yz_lds(g)) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
@@ -89,7 +96,7 @@ static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
     if constexpr (YZReg<R, n>::ok) {
-        if (!yz_lds_forced()) {
+        if (!use_yz_lds(g)) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
@@ -110,7 +117,7 @@ static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx
                               const cx<R>* twP, const int* done, hipStream_t st)
 {
     if constexpr (YZReg<R, n>::ok) {
-        if (!yz_lds_forced()) {
+        if (!use_yz_lds(g)) {
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
@@ -133,7 +140,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
 {
     const long rows = win_rows(g);  // the rows of the x and y windows
     if constexpr (ZInvReg<R, n>::ok) {
-        if (!yz_lds_forced() && dg.d) {
+        if (!use_yz_lds(g) && dg.d) {
             using X = ZInvReg<R, n>;
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
@@ -252,7 +259,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
                            const MixTabs<R>& mt, const int* done, hipStream_t st)
 {
     if constexpr (XReg<R, n>::ok) {
-        if (!xpass_lds_forced()) {
+        if (!use_x_lds(g)) {
             using X = XReg<R, n>;
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;

@@ -53,6 +53,9 @@ struct Geo {
     int nyp = 0, ys = 0;
     int nzp = 0, zs = 0;
     Bat bt{};
+    // pass variants (a tuned plan): bit 0 register z forward, bit 1 LDS-staged
+    // y and z passes, bit 2 LDS-staged x pass; -1: the fixed defaults
+    int var = -1;
 };
 
 // z row of window row r, r < (x1 - x0)(y1 - y0): x0 + r / wy, y0 + r % wy

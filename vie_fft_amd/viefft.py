@@ -36,7 +36,7 @@ EXPORTS = [
     "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
     "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
-    "vie_history", "vie_set_divergence_stop",
+    "vie_history", "vie_set_divergence_stop", "vie_tuning",
 ]
 
 
@@ -99,6 +99,7 @@ def load():
     L.vie_set_guess.argtypes = [vp, dp]
     L.vie_history.argtypes = [vp, ct.c_int, dp, ct.POINTER(ct.c_int)]
     L.vie_set_divergence_stop.argtypes = [vp, ct.c_double]
+    L.vie_tuning.argtypes = [vp, ct.POINTER(ct.c_int), dp]
     L.vie_solve_x0.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, ct.POINTER(Info)]
     L.vie_solve_mixed_resident.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                                            ct.POINTER(Info)]
@@ -417,6 +418,19 @@ class Solver:
         return v.value
 
     # -- instrumentation -----------------------------------------------------
+    def tuning(self):
+        """The autotuner's plan per precision: tuned, variant bits, x/y/z pencil lengths, us per product."""
+        a = (ct.c_int * 10)()
+        u = (ct.c_double * 4)()
+        self._check(self.lib.vie_tuning(self.h, a, u))
+        out = {}
+        for k, name in enumerate(("fp64", "fp32")):
+            o = a[5 * k: 5 * k + 5]
+            if o[0] >= 0:
+                out[name] = {"tuned": bool(o[0]), "variant": o[1], "pencils": tuple(o[2:5]),
+                             "us_per_product": u[k], "us_per_product_defaults": u[k + 2]}
+        return out
+
     def set_divergence_stop(self, factor):
         """Later solves end with DIVERGED once the stop quantity exceeds factor x its minimum (0: off)."""
         self._check(self.lib.vie_set_divergence_stop(self.h, float(factor)))
