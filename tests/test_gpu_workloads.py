@@ -82,3 +82,30 @@ def test_c3_mixed_matches_fp64():
     assert abs(qm - q64) <= 1e-7 * abs(q64)
     assert abs(am - a64) <= 1e-7 * abs(a64)
     assert rel(xm, x64) <= 1e-6
+
+
+def test_c2_cors_history_matches_oracle(c2):
+    """CORS on the 128x128x64 workload is erratic (its residual spikes to 245
+    at iteration 46 and grows past 1e8 by iteration 250): the GPU's residual
+    history over the first 60 iterations against the oracle's
+    (tests/golden/cors_c2_history.json, tests/golden/make_cors_history.py)
+    shows the behaviour is the method's, not the kernels'."""
+    import json
+    import os
+
+    sc, op, s, b, flags = c2
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cors_c2_history.json")))
+    ref = np.array(fx["history"])
+    s.set_rhs(b)
+    st = s.lib.vie_solve_resident(s.h, VF.CORS, 1e-300, len(ref), None)
+    assert st == VF.NOT_CONVERGED
+    h = s.history()
+    assert len(h) == len(ref)
+    d = np.abs(h - ref) / ref
+    print("CORS history rel. diff, max over iterations 1-20, 21-40, 41-60:",
+          d[:20].max(), d[20:40].max(), d[40:].max(), flush=True)
+    # the near-breakdown at the spike amplifies roundoff (8.6e-4 measured
+    # after it); before it the histories agree to 1e-8
+    assert d[:40].max() <= 1e-6
+    assert d.max() <= 1e-2
+    assert ref.max() > 100 and h.max() > 100  # the spike is in both
