@@ -104,6 +104,32 @@ int vie_set_medium(vie_ctx* ctx, const unsigned char* material, const double* ep
                    int n_mat, const unsigned char* flags);
 /* per-voxel permittivity, V complex */
 int vie_set_medium_eps(vie_ctx* ctx, const double* eps_voxel, const unsigned char* flags);
+/* A call with the material ids (or, for vie_set_medium_eps, the flags) of the
+   resident medium -- a wavelength sweep with dispersive tables -- only
+   rebuilds the permittivity and the diagonal term on the device. */
+
+/* Analytic shapes voxelised on the device with the reference's rules
+   (GetMaterial / GetFlag, vie_fft/Model.h:305-398): material ids at the
+   cell centres (i+1/2, j+1/2, k+1/2) in cell units, the first shape holding
+   a point gives its id (0 = background), a sphere holds a point when
+   |p - c|^2 <= r^2 (Model.h:229), a z band when z0 <= z < z1, and points
+   outside the box [0, n] are background; a face is active when its centre
+   and the centres of the two cells it joins are all non-background
+   (Model.h:345-394).  material (V bytes, may be NULL) and flags (3 V bytes)
+   are written in the caller's layout. */
+enum { VIE_SPHERE = 0, VIE_ZBAND = 1 };
+typedef struct {
+    int kind;        /* VIE_SPHERE or VIE_ZBAND            */
+    int mat;         /* material id 0..255                  */
+    double c[3];     /* sphere centre (cells)               */
+    double r;        /* sphere radius (cells)               */
+    double z0, z1;   /* band [z0, z1) (cells)               */
+} vie_shape;
+int vie_voxelize(vie_ctx* ctx, int nshape, const vie_shape* shapes, unsigned char* material,
+                 unsigned char* flags);
+/* vie_voxelize then vie_set_medium with the eps table */
+int vie_set_medium_shapes(vie_ctx* ctx, int nshape, const vie_shape* shapes, const double* eps_table,
+                          int n_mat);
 
 /* ---- set-up ------------------------------------------------------------- */
 /* Green_Function_Compute (vie_fft/GreenFunction.h:8-141): computed on the

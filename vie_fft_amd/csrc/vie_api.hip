@@ -642,6 +642,10 @@ struct vie_ctx {
     bool medium_ready = false;
     std::vector<double> eps_h;     // per-voxel eps, 2V doubles
     std::vector<uint8_t> flags_h;  // 3V
+    // the last medium's caller-layout material ids (empty for per-voxel eps)
+    // and flags: a medium with the same ones only needs its eps and d_f rebuilt
+    std::vector<uint8_t> mat_in, flags_in;
+    int mat_max = 0;
     double2* eps_dev = nullptr;
     uint8_t* mat_dev = nullptr;
     // solver storage
@@ -2854,9 +2858,10 @@ int vie_destroy(vie_ctx* c)
 
 const char* vie_last_error(const vie_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-static int set_medium_common(vie_ctx* c, const unsigned char* flags)
+// eps (c->eps_h) must be finite, and 1/(eps - 1) finite on every face the
+// operator uses (flags: c->flags_h)
+static void check_eps(vie_ctx* c)
 {
-    c->flags_h.assign(flags, flags + 3 * c->V);
     for (long i = 0; i < 2 * c->V; ++i)
         if (!std::isfinite(c->eps_h[i])) throw Err(VIE_ERR_NAN, "non-finite permittivity");
     // 1/(eps - 1) must be finite on every face the operator uses
@@ -2873,6 +2878,35 @@ static int set_medium_common(vie_ctx* c, const unsigned char* flags)
                     throw Err(VIE_ERR_ARG, "eps = 1 on an active face (1/(eps-1) is infinite)");
             }
         }
+}
+
+// The diagonal term d_f of every operator from eps_dev (k_diag, the part of
+// upload_medium that depends on eps and k0).
+static void rebuild_diag(vie_ctx* c, bool new_medium = true)
+{
+    const int* n = c->desc.n;
+    const double* d = c->desc.d;
+    double cim[3];
+    for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / c->desc.k0;
+    if (c->op64)
+        hipLaunchKernelGGL(k_diag<double>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                           cim[2], c->eps_dev, c->op64->dg);
+    if (c->op32)
+        hipLaunchKernelGGL(k_diag<float>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                           cim[2], c->eps_dev, c->op32->dg);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!new_medium) return;
+    // per-instance operators and right-hand sides follow a new medium
+    c->bop.assign(c->nbcap, 0);
+    c->bmulti = false;
+    c->brhs.assign(c->nbcap, 0);
+}
+
+static int set_medium_common(vie_ctx* c, const unsigned char* flags)
+{
+    c->flags_h.assign(flags, flags + 3 * c->V);
+    check_eps(c);
     if (!c->eps_dev) c->eps_dev = dalloc<double2>(c->V);
     copy_sync(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     c->medium_ready = true;
@@ -2947,41 +2981,180 @@ static int set_medium_int(vie_ctx* c, const unsigned char* material, const doubl
     return set_medium_common(c, flags);
 }
 
+// whether the caller's flags (and material ids) are those of the resident medium
+static bool same_geometry(const vie_ctx* c, const unsigned char* material, const unsigned char* flags)
+{
+    if (!c->medium_ready || c->flags_in.size() != (size_t)(3 * c->Vu)) return false;
+    if (material ? c->mat_in.size() != (size_t)c->Vu : !c->mat_in.empty()) return false;
+    if (std::memcmp(c->flags_in.data(), flags, 3 * c->Vu) != 0) return false;
+    return !material || std::memcmp(c->mat_in.data(), material, c->Vu) == 0;
+}
+
+static void keep_geometry(vie_ctx* c, const unsigned char* material, const unsigned char* flags)
+{
+    c->flags_in.assign(flags, flags + 3 * c->Vu);
+    if (material) {
+        c->mat_in.assign(material, material + c->Vu);
+        c->mat_max = *std::max_element(material, material + c->Vu);
+    } else {
+        c->mat_in.clear();
+    }
+}
+
+// A new eps table for the resident material ids (the frame, flags, windows,
+// row ranges, plans and G~ octants stay): eps and d_f are rebuilt on the
+// device.  Returns false when the full path must run (a table entry of
+// exactly 1 needs the face check over every cell).
+static bool set_eps_table_fast(vie_ctx* c, const double* eps_table, int n_mat)
+{
+    require(c, n_mat <= 256, "at most 256 materials");
+    if (c->mat_max >= n_mat) throw Err(VIE_ERR_ARG, "material id outside the eps table");
+    std::vector<double2> t(256, make_double2(kEpsBackground, 0.0));
+    for (int m = 0; m < n_mat; ++m) {
+        if (!std::isfinite(eps_table[2 * m]) || !std::isfinite(eps_table[2 * m + 1]))
+            throw Err(VIE_ERR_NAN, "non-finite permittivity");
+        if (eps_table[2 * m] == 1.0 && eps_table[2 * m + 1] == 0.0) return false;
+        t[m] = make_double2(eps_table[2 * m], eps_table[2 * m + 1]);
+    }
+    double2* tab = dalloc<double2>(256);
+    try {
+        copy_sync(tab, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev, tab, c->eps_dev);
+        HIPCHK(hipGetLastError());
+        rebuild_diag(c);
+    } catch (...) {
+        dfree(tab);
+        throw;
+    }
+    dfree(tab);
+    return true;
+}
+
 int vie_set_medium(vie_ctx* c, const unsigned char* material, const double* eps_table, int n_mat,
                    const unsigned char* flags)
 {
     if (!c || !material || !eps_table || !flags || n_mat < 1) return VIE_ERR_ARG;
     try {
+        if (same_geometry(c, material, flags) && set_eps_table_fast(c, eps_table, n_mat)) return VIE_OK;
+        c->flags_in.clear();  // a failed set-up leaves no resident geometry
         prepare_frame(c, flags);
+        int rc;
         if (c->remap) {  // background (id 0) and inactive faces on the embedding cells
             std::vector<unsigned char> m(c->V), f(3 * c->V);
             embed_h<unsigned char>(c, material, m.data(), 1, 0);
             embed_h<unsigned char>(c, flags, f.data(), 3, 0);
-            return set_medium_int(c, m.data(), eps_table, n_mat, f.data());
+            rc = set_medium_int(c, m.data(), eps_table, n_mat, f.data());
+        } else {
+            rc = set_medium_int(c, material, eps_table, n_mat, flags);
         }
-        return set_medium_int(c, material, eps_table, n_mat, flags);
+        if (rc == VIE_OK) keep_geometry(c, material, flags);
+        return rc;
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+static int set_medium_eps_full(vie_ctx* c, const double* eps, const unsigned char* flags)
+{
+    prepare_frame(c, flags);
+    if (c->remap) {  // dummy background permittivity (vie_fft/Model.h:82) and inactive faces outside
+        std::vector<cd> e(c->V);
+        std::vector<unsigned char> f(3 * c->V);
+        embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
+        embed_h<unsigned char>(c, flags, f.data(), 3, 0);
+        c->eps_h.assign((const double*)e.data(), (const double*)e.data() + 2 * c->V);
+        dfree(c->mat_dev);
+        return set_medium_common(c, f.data());
+    }
+    c->eps_h.assign(eps, eps + 2 * c->V);
+    dfree(c->mat_dev);
+    return set_medium_common(c, flags);
 }
 
 int vie_set_medium_eps(vie_ctx* c, const double* eps, const unsigned char* flags)
 {
     if (!c || !eps || !flags) return VIE_ERR_ARG;
     try {
-        prepare_frame(c, flags);
-        if (c->remap) {  // dummy background permittivity (vie_fft/Model.h:82) and inactive faces outside
-            std::vector<cd> e(c->V);
-            std::vector<unsigned char> f(3 * c->V);
-            embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
-            embed_h<unsigned char>(c, flags, f.data(), 3, 0);
-            c->eps_h.assign((const double*)e.data(), (const double*)e.data() + 2 * c->V);
-            dfree(c->mat_dev);
-            return set_medium_common(c, f.data());
+        if (same_geometry(c, nullptr, flags)) {  // only eps changed: eps and d_f
+            if (c->remap) {
+                std::vector<cd> e(c->V);
+                embed_h<cd>(c, (const cd*)eps, e.data(), 1, cd(kEpsBackground, 0.0));
+                c->eps_h.assign((const double*)e.data(), (const double*)e.data() + 2 * c->V);
+            } else {
+                c->eps_h.assign(eps, eps + 2 * c->V);
+            }
+            check_eps(c);
+            copy_sync(c->eps_dev, c->eps_h.data(), c->V * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+            rebuild_diag(c);
+            return VIE_OK;
         }
-        c->eps_h.assign(eps, eps + 2 * c->V);
-        dfree(c->mat_dev);
-        return set_medium_common(c, flags);
+        c->flags_in.clear();
+        const int rc = set_medium_eps_full(c, eps, flags);
+        if (rc == VIE_OK) keep_geometry(c, nullptr, flags);
+        return rc;
+    } catch (const Err& e) {
+        c->flags_in.clear();
+        return fail(c, e);
+    }
+}
+
+// Material ids and face flags of analytic shapes, on the device, in the
+// caller's layout (GetMaterial / GetFlag, vie_fft/Model.h:305-398).
+static void voxelize(vie_ctx* c, int nshape, const vie_shape* shapes, unsigned char* material, unsigned char* flags)
+{
+    static_assert(sizeof(vie_shape) == sizeof(VoxShape), "vie_shape layout");
+    require(c, nshape >= 0 && nshape <= 4096, "0..4096 shapes");
+    for (int s = 0; s < nshape; ++s) {
+        const vie_shape& q = shapes[s];
+        require(c, q.kind == VIE_SPHERE || q.kind == VIE_ZBAND, "unknown shape kind");
+        require(c, q.mat >= 0 && q.mat <= 255, "material id must be 0..255");
+        for (double v : {q.c[0], q.c[1], q.c[2], q.r, q.z0, q.z1})
+            if (std::isnan(v)) throw Err(VIE_ERR_NAN, "NaN in a shape");
+    }
+    const long V = c->Vu;
+    const int* u = c->nu;
+    VoxShape* sd = nullptr;
+    uint8_t* md = nullptr;
+    uint8_t* fd = nullptr;
+    try {
+        sd = dalloc<VoxShape>(std::max(nshape, 1));
+        md = dalloc<uint8_t>(V);
+        fd = dalloc<uint8_t>(3 * V);
+        if (nshape) copy_sync(sd, shapes, nshape * sizeof(VoxShape), hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_voxelize, dim3(grid1(V, NT, 65536)), dim3(NT), 0, c->stream, u[0], u[1], u[2], nshape, sd,
+                           md, fd);
+        HIPCHK(hipGetLastError());
+        if (material) copy_sync(material, md, V, hipMemcpyDeviceToHost, c->stream);
+        copy_sync(flags, fd, 3 * V, hipMemcpyDeviceToHost, c->stream);
+    } catch (...) {
+        dfree(sd);
+        dfree(md);
+        dfree(fd);
+        throw;
+    }
+    dfree(sd);
+    dfree(md);
+    dfree(fd);
+}
+
+int vie_voxelize(vie_ctx* c, int nshape, const vie_shape* shapes, unsigned char* material, unsigned char* flags)
+{
+    if (!c || (nshape > 0 && !shapes) || !flags) return VIE_ERR_ARG;
+    try {
+        voxelize(c, nshape, shapes, material, flags);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_set_medium_shapes(vie_ctx* c, int nshape, const vie_shape* shapes, const double* eps_table, int n_mat)
+{
+    if (!c || (nshape > 0 && !shapes) || !eps_table || n_mat < 1) return VIE_ERR_ARG;
+    try {
+        std::vector<unsigned char> m(c->Vu), f(3 * c->Vu);
+        voxelize(c, nshape, shapes, m.data(), f.data());
+        return vie_set_medium(c, m.data(), eps_table, n_mat, f.data());
     } catch (const Err& e) {
         return fail(c, e);
     }
@@ -2999,10 +3172,7 @@ int vie_set_k0(vie_ctx* c, double k0)
         c->rhs_ready = false;
         c->sol_ready = false;
         // the diagonal term's factor (Z0 / k0) changed: rebuild it
-        if (c->medium_ready) {
-            if (c->op64) upload_medium<double>(c, c->op64);
-            if (c->op32) upload_medium<float>(c, c->op32);
-        }
+        if (c->medium_ready) rebuild_diag(c, false);
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

@@ -699,6 +699,57 @@ __global__ __launch_bounds__(NT) void k_crop(Remap m, int ncomp, const T* __rest
 }
 
 // ------------------------------------------------------------ set-up kernels
+// Analytic shapes (GetMaterial / GetFlag, vie_fft/Model.h:305-398): a sphere
+// holds a point when |p - c|^2 <= r^2 (Model.h:229), a band of z planes when
+// z0 <= z < z1; points are in cell units, the first shape that holds a point
+// gives its material, and points outside the box [0, n] are background (0).
+struct VoxShape {
+    int kind;  // 0 sphere, 1 z band
+    int mat;
+    double cx, cy, cz, r, z0, z1;
+};
+__device__ inline int vox_material(const VoxShape* sh, int ns, int n0, int n1, int n2, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    if (!(x >= 0.0 && x <= n0 && y >= 0.0 && y <= n1 && z >= 0.0 && z <= n2)) return 0;
+    for (int s = 0; s < ns; ++s) {
+        const VoxShape& q = sh[s];
+        if (q.kind == 0) {
+            // rounded like ((x - cx)^2 + (y - cy)^2) + (z - cz)^2 <= r r, every
+            // product and sum rounded on its own (no fused multiply-add)
+            const double dx = x - q.cx, dy = y - q.cy, dz = z - q.cz;
+            const double d2 = (dx * dx + dy * dy) + dz * dz;
+            if (d2 <= q.r * q.r) return q.mat;
+        } else if (z >= q.z0 && z < q.z1) {
+            return q.mat;
+        }
+    }
+    return 0;
+}
+// Material at the cell centres ((i, j, k) + 1/2) and the 3-point face flags:
+// face f of cell v is active when the face centre and the centres of the two
+// cells it joins are all non-background (GetFlag, vie_fft/Model.h:345-394).
+__global__ __launch_bounds__(NT) void k_voxelize(int n0, int n1, int n2, int ns, const VoxShape* __restrict__ sh,
+                                                 uint8_t* __restrict__ mat, uint8_t* __restrict__ flags)
+{
+    const long V = (long)n0 * n1 * n2;
+    for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+        const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), k = (int)(v % n2);
+        const double c[3] = {i + 0.5, j + 0.5, k + 0.5};
+        const int m = vox_material(sh, ns, n0, n1, n2, c[0], c[1], c[2]);
+        if (mat) mat[v] = (uint8_t)m;
+        for (int f = 0; f < 3; ++f) {
+            double p[3] = {c[0], c[1], c[2]};
+            p[f] = c[f] + 0.5;
+            const bool face = vox_material(sh, ns, n0, n1, n2, p[0], p[1], p[2]) != 0;
+            p[f] = c[f] + 1.0;
+            const bool up = vox_material(sh, ns, n0, n1, n2, p[0], p[1], p[2]) != 0;
+            flags[f * V + v] = (uint8_t)(m != 0 && face && up);
+        }
+    }
+}
+
+
 // Per-voxel permittivity from material ids and a table (GetMaterial's
 // Material with Real_EPR/Imag_EPR, vie_fft/Model.h:17,82-83).
 __global__ __launch_bounds__(NT) void k_eps_table(long V, const uint8_t* __restrict__ mat,

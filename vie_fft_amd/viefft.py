@@ -36,7 +36,7 @@ EXPORTS = [
     "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
     "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
-    "vie_history", "vie_set_divergence_stop", "vie_tuning",
+    "vie_history", "vie_set_divergence_stop", "vie_tuning", "vie_voxelize", "vie_set_medium_shapes",
 ]
 
 
@@ -50,6 +50,32 @@ class Info(ct.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SPHERE, ZBAND = 0, 1
+
+
+class Shape(ct.Structure):
+    """vie_shape: a sphere (centre, radius) or a z band [z0, z1), in cells."""
+    _fields_ = [("kind", ct.c_int), ("mat", ct.c_int), ("c", ct.c_double * 3), ("r", ct.c_double),
+                ("z0", ct.c_double), ("z1", ct.c_double)]
+
+
+def shapes_of(scene):
+    """The vie_shape array of a geometry.Scene's shapes (spheres and z bands)."""
+    from . import geometry as G
+
+    arr = (Shape * max(len(scene.shapes), 1))()
+    for i, sh in enumerate(scene.shapes):
+        if isinstance(sh, G.Sphere):
+            arr[i].kind, arr[i].mat, arr[i].r = SPHERE, int(sh.mat), float(sh.radius)
+            for a in range(3):
+                arr[i].c[a] = float(sh.center[a])
+        elif isinstance(sh, G.Band):
+            arr[i].kind, arr[i].mat, arr[i].z0, arr[i].z1 = ZBAND, int(sh.mat), float(sh.z0), float(sh.z1)
+        else:
+            raise TypeError(sh)
+    return arr, len(scene.shapes)
 
 
 class VieError(RuntimeError):
@@ -100,6 +126,8 @@ def load():
     L.vie_history.argtypes = [vp, ct.c_int, dp, ct.POINTER(ct.c_int)]
     L.vie_set_divergence_stop.argtypes = [vp, ct.c_double]
     L.vie_tuning.argtypes = [vp, ct.POINTER(ct.c_int), dp]
+    L.vie_voxelize.argtypes = [vp, ct.c_int, ct.POINTER(Shape), ub, ub]
+    L.vie_set_medium_shapes.argtypes = [vp, ct.c_int, ct.POINTER(Shape), dp, ct.c_int]
     L.vie_solve_x0.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, ct.POINTER(Info)]
     L.vie_solve_mixed_resident.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                                            ct.POINTER(Info)]
@@ -198,6 +226,20 @@ class Solver:
         e = np.ascontiguousarray(eps, dtype=np.complex128).reshape(self.n)
         f = np.ascontiguousarray(flags, dtype=np.uint8).reshape((3,) + self.n)
         self._check(self.lib.vie_set_medium_eps(self.h, _dp(e), _ub(f)))
+
+    def voxelize(self, scene):
+        """(material, flags) of the scene's shapes, computed on the device."""
+        arr, n = shapes_of(scene)
+        m = np.zeros(self.n, np.uint8)
+        f = np.zeros((3,) + self.n, np.uint8)
+        self._check(self.lib.vie_voxelize(self.h, n, arr, _ub(m), _ub(f)))
+        return m, f
+
+    def set_medium_shapes(self, scene, eps_table=None):
+        """Medium from the scene's shapes (voxelised on the device) and its eps table."""
+        arr, n = shapes_of(scene)
+        e = np.ascontiguousarray(scene.eps_table if eps_table is None else eps_table, dtype=np.complex128).ravel()
+        self._check(self.lib.vie_set_medium_shapes(self.h, n, arr, _dp(e), e.size))
 
     def set_k0(self, k0):
         """Move the context to another wavenumber (then green_compute, set_rhs)."""
