@@ -1346,14 +1346,68 @@ void require(vie_ctx* c, bool ok, const char* what)
 // iterations enqueued after convergence exit at once).
 template <class F> KState drive_batch(vie_ctx* c, KState* st, F iter);
 
+// A round of c->ahead iterations, enqueued as one HIP graph: captured at the
+// first round and replayed after (the iterations' kernel arguments -- buffers,
+// the state's coefficient and done addresses, launch slots -- stay the same;
+// every scalar the kernels use lives in device memory).  A new capture is
+// made when the launch slots change (batches).  Off while kernels are being
+// timed (host events) and with VIE_GRAPH=0.
+struct IterGraph {
+    hipGraphExec_t exec = nullptr;
+    Bat slots{};
+    ~IterGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+static bool graphs_on()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("VIE_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+template <class F> void round_of(vie_ctx* c, IterGraph& g, F& iter)
+{
+    if (!graphs_on() || c->timer.cls) {
+        for (int k = 0; k < c->ahead; ++k) iter();
+        return;
+    }
+    if (g.exec && (g.slots.nb != c->cur.nb || g.slots.ilist != c->cur.ilist)) {
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    if (!g.exec) {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            for (int k = 0; k < c->ahead; ++k) iter();
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+        g.slots = c->cur;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, c->stream));
+}
+
 template <class F> KState drive(vie_ctx* c, KState* st, F iter)
 {
     if (c->in_batch) return drive_batch(c, st, iter);
+    IterGraph g;
     int slot = 0;
-    for (int k = 0; k < c->ahead; ++k) iter();
+    round_of(c, g, iter);
     snap_state(c, st, slot);
     for (;;) {
-        for (int k = 0; k < c->ahead; ++k) iter();
+        round_of(c, g, iter);
         snap_state(c, st, slot ^ 1);
         const KState h = wait_snap(c, slot);
         slot ^= 1;
@@ -1389,13 +1443,14 @@ template <class F> KState drive_batch(vie_ctx* c, KState* st, F iter)
 {
     const int nb = c->bn;
     int nact = nb, slot = 0;
+    IterGraph g;
     batch_all(c);
-    for (int k = 0; k < c->ahead; ++k) iter();
+    round_of(c, g, iter);
     snap_state(c, st, slot);
     int live = nb;
     for (;;) {
         batch_slots(c, st, live, &nact);
-        for (int k = 0; k < c->ahead; ++k) iter();
+        round_of(c, g, iter);
         snap_state(c, st, slot ^ 1);
         wait_snap(c, slot);
         const KState* h = snap_all(c, slot);
@@ -1843,13 +1898,14 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     // previous one is checked; a replacement waits for the batch in flight.
     int slot = 0;
     bool primed = false;
+    IterGraph g;
     for (;;) {
         if (!primed) {
-            for (int k = 0; k < c->ahead; ++k) iter();
+            round_of(c, g, iter);
             snap_state(c, st, slot);
             primed = true;
         }
-        for (int k = 0; k < c->ahead; ++k) iter();
+        round_of(c, g, iter);
         snap_state(c, st, slot ^ 1);
         h = wait_snap(c, slot);
         slot ^= 1;
@@ -2026,11 +2082,12 @@ void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
     };
     int nact = nb, slot = 0, may_run = nb;
     bool need = false;
-    for (int k = 0; k < c->ahead; ++k) iter();
+    IterGraph g;
+    round_of(c, g, iter);
     snap_state(c, st, slot);
     for (;;) {
         batch_slots(c, st, may_run, &nact);
-        for (int k = 0; k < c->ahead; ++k) iter();
+        round_of(c, g, iter);
         if (need) replace();
         snap_state(c, st, slot ^ 1);
         wait_snap(c, slot);
