@@ -323,6 +323,46 @@ def test_short_z_period_matches_oracle(n, lo, hi, order, zw, zpen):
     assert rel(x32, op.solve(b, 3, tol=1e-300, maxit=8)["x"]) <= 1e-4
 
 
+# Windows that end near the far edge of their axis: the window start plus the
+# short period passes the grid's extent, so pencil entries at and beyond the
+# last plane exist and the passes' load and store guards must drop them.
+# xshort forces the x period (48: the FP32 x pass on 96-point register
+# pencils, whose 24- and 36-entry variants serve windows up to 24 and 36
+# planes; FP64 then takes the LDS-staged pass at 48).
+EDGES = [
+    # n, box lo, box hi, axis checked, xshort
+    ((128, 6, 8), (84, 0, 0), (126, 6, 8), 0, None),
+    ((128, 6, 8), (84, 0, 0), (126, 6, 8), 0, "48"),
+    ((64, 8, 10), (40, 0, 0), (62, 8, 10), 0, "48"),
+    ((64, 8, 10), (30, 0, 0), (62, 8, 10), 0, "48"),
+    ((96, 8, 6), (50, 0, 0), (96, 8, 6), 0, "48"),
+    ((12, 128, 10), (3, 70, 0), (7, 126, 10), 1, None),
+    ((12, 16, 128), (3, 2, 60), (7, 10, 126), 2, None),
+]
+
+
+@pytest.mark.parametrize("n,lo,hi,axis,xshort", EDGES)
+def test_short_period_past_the_far_edge_matches_oracle(monkeypatch, n, lo, hi, axis, xshort):
+    if xshort:
+        monkeypatch.setenv("VIE_XSHORT", xshort)
+    mat, eps, flags = box_scene(lo, hi, 23, n)
+    b = O.incident(n, D, K0, 25.0, 35.0, 45.0) * flags
+    ref = oracle(n, eps, flags).solve(b, 3, tol=1e-300, maxit=8)
+    key = ("x", "y", "z")[axis]
+    for prec, bound in ((VF.FP64, 1e-9), (VF.FP32, 1e-4)):
+        with solver(n, mat, flags, prec) as s:
+            p = s.plan()
+            assert p["axis_order"] == (0, 1, 2), p
+            w0, w1 = p[key + "_window"]
+            pen = p[key + "_pencil"]
+            assert pen < n[axis] and w0 + pen > n[axis], p
+            if xshort:
+                assert pen == int(xshort)
+            x, info = s.solve(b, VF.BICGSTAB, tol=1e-300, maxit=8)
+        assert info["iterations"] == 8
+        assert rel(x, ref["x"]) <= bound, (prec, rel(x, ref["x"]))
+
+
 def test_embedded_slab_matches_oracle():
     n = (18, 14, 11)  # compiled as 20 x 16 x 12
     assert not VF.supported_n(18) and not VF.supported_n(14) and not VF.supported_n(11)

@@ -152,6 +152,26 @@ template <int D, class T> struct dft_impl<5, D, T> {
     }
 };
 
+// radix 6 = 2 x 3, prime-factor (Good-Thomas) order, so no twiddles between
+// the two: n = (3 n1 + 2 n2) mod 6, k = (3 k1 + 4 k2) mod 6 and
+// w6^{n k} = w2^{n1 k1} w3^{n2 k2}.  Length-2 transforms over n1 for each n2,
+// then length-3 transforms over n2 for each k1.
+template <int D, class T> struct dft_impl<6, D, T> {
+    static __device__ __forceinline__ void run(T* v)
+    {
+        T e[3] = {cadd(v[0], v[3]), cadd(v[2], v[5]), cadd(v[4], v[1])};
+        T o[3] = {csub(v[0], v[3]), csub(v[2], v[5]), csub(v[4], v[1])};
+        dft_impl<3, D, T>::run(e);
+        dft_impl<3, D, T>::run(o);
+        v[0] = e[0];
+        v[4] = e[1];
+        v[2] = e[2];
+        v[3] = o[0];
+        v[1] = o[1];
+        v[5] = o[2];
+    }
+};
+
 // radix 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2,
 // X[k] = sum_n2 w4^{n2 k2} (w16^{n2 k1} sum_n1 w4^{n1 k1} x[4 n1 + n2])
 template <int D, class T> struct dft_impl<16, D, T> {

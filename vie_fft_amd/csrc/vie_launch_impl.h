@@ -225,23 +225,24 @@ hipError_t run_yfwd<R_>(const Geo& g, const cx<R_>* W1, cx<R_>* W2, const cx<R_>
 // with OP and G~ read in the product reaches three waves without spilling
 // (147 registers) but issues 23% more instructions: 117.7 against 105.0 us.
 template <class R, int N> struct XReg { static constexpr bool ok = false; };
-#define VIE_XREG(R, N, T_, GM_, OP_, ...)                                                                    \
+#define VIE_XREG(R, N, T_, W_, GM_, OP_, ...)                                                                \
     template <> struct XReg<R, N> {                                                                       \
         static constexpr bool ok = true;                                                                  \
-        static constexpr int T = T_, W = 0, GM = GM_;                                                     \
+        static constexpr int T = T_, W = W_, GM = GM_;                                                    \
         static constexpr bool OP = OP_;                                                                   \
         using PL = XPlan<__VA_ARGS__>;                                                                    \
     };
-VIE_XREG(float, 32, 8, 2, false, 8, 8)
-VIE_XREG(float, 64, 16, 0, false, 8, 4, 4)
-VIE_XREG(float, 96, 16, 2, true, 4, 4, 4, 3)
-VIE_XREG(float, 128, 16, 2, true, 16, 16)
-VIE_XREG(float, 256, 32, 0, true, 16, 16, 2)
-VIE_XREG(double, 32, 8, 2, false, 8, 8)
-VIE_XREG(double, 64, 16, 2, false, 8, 4, 4)
-VIE_XREG(double, 96, 16, 2, false, 4, 4, 4, 3)
-VIE_XREG(double, 128, 32, 2, false, 8, 8, 4)
-VIE_XREG(double, 256, 64, 2, false, 8, 8, 8)
+VIE_XREG(float, 32, 8, 0, 2, false, 8, 8)
+VIE_XREG(float, 48, 8, 3, 0, false, 4, 4, 6)
+VIE_XREG(float, 64, 16, 0, 0, false, 8, 4, 4)
+VIE_XREG(float, 96, 16, 0, 2, true, 4, 4, 4, 3)
+VIE_XREG(float, 128, 16, 0, 2, true, 16, 16)
+VIE_XREG(float, 256, 32, 0, 0, true, 16, 16, 2)
+VIE_XREG(double, 32, 8, 0, 2, false, 8, 8)
+VIE_XREG(double, 64, 16, 0, 2, false, 8, 4, 4)
+VIE_XREG(double, 96, 16, 0, 2, false, 4, 4, 4, 3)
+VIE_XREG(double, 128, 32, 0, 2, false, 8, 8, 4)
+VIE_XREG(double, 256, 64, 0, 2, false, 8, 8, 8)
 #undef VIE_XREG
 
 // VIE_XPASS_LDS=1 selects the LDS-staged x pass for every size (A/B timing).
