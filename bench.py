@@ -41,7 +41,7 @@ XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil":
 # VALU instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, median launch):
 # profiles/k_xreg_f32_128cube_valu.md.  The pass is VALU-bound; MI355X issues at most
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles = 614.4e9 wave64 VALU instructions per second.
-XPASS_F32_128_VALU = 25059328
+XPASS_F32_128_VALU = 25092096
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # register-resident x-pass kernels by pencil length (FP32), without the last
@@ -112,8 +112,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-extra", action="store_true", help="skip the 96^3 dimer sweep and 256^3 legs")
-    ap.add_argument("--xpass-timing", choices=["live", "separate"], default="live",
-                    help="time the FP32 x pass during the timed mixed solves (live) or in one more solve")
+    ap.add_argument("--xpass-timing", choices=["live", "separate"], default="separate",
+                    help="time the FP32 x pass in one more mixed solve after the timed ones (separate), or "
+                         "during the timed solves (live: HIP events on one launch in eight, no graph replay)")
     ap.add_argument("--n", type=int, default=128, help="grid edge (128 = the metric workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a plain device copy of the x pass's working set (HBM counter calibration)")
@@ -158,10 +159,11 @@ def main():
     x64 = s.solution()
     q64 = cross_sections(s, sc)
 
-    # ---- timed: mixed solves (the metric); the FP32 x pass (the roofline
-    # kernel) is timed live, HIP events around one launch in eight on the
-    # library's stream (an event pair adds a few us around the launch it
-    # brackets: about 0.5% of the solve)
+    # ---- timed: mixed solves (the metric).  The FP32 x pass (the roofline
+    # kernel) is timed with HIP events around one launch in eight on the
+    # library's stream, by default in one more solve after the timed ones so
+    # that the timed solves run uninstrumented (with events on, iterations are
+    # enqueued one by one instead of replayed as graphs)
     live = args.xpass_timing == "live"
     if live:
         s.time_kernel(13)
