@@ -104,16 +104,27 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
     out_dir the per-wavelength files are written (first polarisation of each
     wavelength, as the reference writes one file per process) and merged.
     With warm=True every solve starts from the previous wavelength's solution
-    of the same polarisation (``vie_set_guess``) instead of x = 0.
+    of the same polarisation instead of x = 0: the points are then visited
+    polarisation by polarisation (wavelengths in their given order), so that
+    guess is the solution still resident on the device (``vie_set_guess`` with
+    no host copy); results come back in the order of `points`.
     """
-    results = []
+    results = [None] * len(points)
     s = None
     cur = None
     material = flags = None
     wl_index = {}
-    prev = {}  # psi -> the last solution of that polarisation (warm starts)
+    order = list(range(len(points)))
+    if warm:
+        first = {}
+        for i, (_, psi) in enumerate(points):
+            first.setdefault(psi, i)
+        order.sort(key=lambda i: (first[points[i][1]], i))
+    last_psi = None  # polarisation of the resident solution (warm starts)
+    wl_plan = {}
     try:
-        for lam, psi in points:
+        for idx in order:
+            lam, psi = points[idx]
             sc = scene_at(lam, psi)
             if s is None:
                 s = VF.Solver(sc.n, sc.d, sc.k0, VF.FP64)
@@ -133,12 +144,11 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
                 plan = None
             t0 = time.perf_counter()
             s.set_rhs_incident(sc.theta, sc.phi, psi)
-            if warm and psi in prev:
-                s.set_guess(prev[psi])
+            if warm and last_psi == psi:
+                s.set_guess(None)
             info = s.solve_mixed(tol=tol, maxit=maxit) if mixed else s.solve_resident(method, tol=tol, maxit=maxit)
             solve_s = time.perf_counter() - t0
-            if warm:
-                prev[psi] = s.solution()
+            last_psi = psi
             q_ext = s.forward_scatter(sc.theta, sc.phi, psi) / sc.area
             q_abs = s.absorbed(metal) * 120.0 * np.pi / sc.area
             act = 3.0 * s.absorbed(active) / vol_active if active is not None and vol_active > 0 else 0.0
@@ -146,9 +156,10 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
                  "active_density": act, "iterations": info["iterations"], "outer": info["outer"],
                  "true_relres": info["true_relres"], "converged": bool(info["converged"]),
                  "solve_s": solve_s, "setup_s": setup}
-            if plan is not None:
+            if plan is not None and "plan" not in wl_plan.get(lam, {}):
                 r["plan"] = plan
-            results.append(r)
+                wl_plan[lam] = r
+            results[idx] = r
             if log:
                 log(r)
             if out_dir is not None and lam not in wl_index:
