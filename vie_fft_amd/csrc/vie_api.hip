@@ -2835,8 +2835,10 @@ int vie_create(vie_ctx** out, const vie_desc* desc)
     }
     if (!(desc->k0 > 0) || !std::isfinite(desc->k0)) return VIE_ERR_ARG;
     if (desc->precision != VIE_FP64 && desc->precision != VIE_FP32) return VIE_ERR_ARG;
-    // the pass kernels index the 3 x 4V spectral array with 32-bit offsets
-    if (12L * nint[0] * nint[1] * nint[2] >= (1L << 31)) return VIE_ERR_UNSUPPORTED;
+    // the pass kernels index the 3 x 4V spectral array with 32-bit element
+    // offsets, and the x pass with 32-bit byte offsets (16-byte FP64 entries)
+    if (12L * nint[0] * nint[1] * nint[2] >= (1L << 31) || 12L * 16 * nint[0] * nint[1] * nint[2] >= (1L << 32))
+        return VIE_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VIE_ERR_DEVICE;
     vie_ctx* c = new vie_ctx();

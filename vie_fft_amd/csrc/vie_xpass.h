@@ -274,7 +274,17 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     // 2^31 before launching), so the per-access address arithmetic stays 32-bit
     const unsigned rowst = (unsigned)P1 * P2;  // stride between x rows
     const unsigned cst = (unsigned)g.nx * rowst;  // stride between components
-    T_* base = W2 + (long)blockIdx.y * 3 * cst + ((long)g.xs * P1 + ty) * P2 + z0 + col;  // entry 0: plane xs
+    // row n = i + m of a pencil sits at t rowst + (n - t) rowst: the lane's part
+    // once (a 24-bit multiply: t < 64 and rowst <= 512^2, exact below 2^32),
+    // the rest workgroup-uniform (scalar multiplies); a 32-bit vector multiply
+    // issues at a quarter of the rate
+    const unsigned trow = __umul24((unsigned)t, rowst);
+    // entry 0 (plane xs) of the workgroup's first column: a workgroup-uniform
+    // base, so the pencil accesses take a 32-bit per-lane byte offset from it
+    // (a slot's 3 x Nx x 2Ny x 2Nz entries stay below 2^32 bytes up to 256^3)
+    char* base = reinterpret_cast<char*>(W2 + (long)blockIdx.y * 3 * cst + ((long)g.xs * P1 + ty) * P2 + z0);
+    const unsigned lane0 = trow + (unsigned)col;
+    auto at = [base](unsigned e) -> T_& { return *reinterpret_cast<T_*>(base + e * (unsigned)sizeof(T_)); };
 
     // issue the pencil loads first, then the tables, so their latencies overlap
     T_ v[3][E];
@@ -286,8 +296,9 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
             for (int q = 0; q < r1; ++q) {
                 const int n = i + q * (P / r1);
+                const unsigned off = (unsigned)((P / E) * b + q * (P / r1)) * rowst;
                 v[c][b * r1 + q] = (n < SUP && colok && n + g.xs >= g.x0 && n + g.xs < g.x1)
-                                       ? base[c * cst + (unsigned)n * rowst]
+                                       ? at(c * cst + lane0 + off)
                                        : czero<T_>();
             }
         }
@@ -311,8 +322,14 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     const int tz = colok ? z0 + col : z0;
     const T_ uz = it.U[2][tz], vz = it.Vv[2][tz];
     // G octant column of this thread: G[fx][fold(ky)][fold(kz)]
-    const T_* gcol = it.G + (long)fold_split(ty, g.nyp) * (g.nzp + 1) + fold_split(tz, g.nzp);
-    const unsigned gs = (unsigned)(g.nyp + 1) * (g.nzp + 1);
+    // (a uniform row base and a 32-bit per-lane byte offset, as for the pencils:
+    // the octant stays below 2^32 bytes up to 256^3)
+    const char* grow = reinterpret_cast<const char*>(it.G + (long)fold_split(ty, g.nyp) * (g.nzp + 1));
+    const unsigned gl = (unsigned)fold_split(tz, g.nzp);
+    auto gat = [grow, gl](unsigned fxgs) -> const T_& {
+        return *reinterpret_cast<const T_*>(grow + (gl + fxgs) * (unsigned)sizeof(T_));
+    };
+    const unsigned gs = (unsigned)(g.nyp + 1) * (g.nzp + 1);  // < 2^24; fx gs < 2^32 (24-bit multiplies)
     auto fxof = [](int kx) {  // parity-order index of min(kx, P - kx)
         const int kf = kx < P - kx ? kx : P - kx;
         return (kf & 1) ? (N / 2 + 1) + (kf >> 1) : (kf >> 1);
@@ -322,7 +339,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
         for (int b = 0; b < E / rL; ++b)
 #pragma unroll
-            for (int q = 0; q < rL; ++q) gpre[b * rL + q] = gcol[(unsigned)fxof(t + (P / E) * b + q * (P / rL)) * gs];
+            for (int q = 0; q < rL; ++q)
+                gpre[b * rL + q] = gat(__umul24((unsigned)fxof(t + (P / E) * b + q * (P / rL)), gs));
     }
     __syncthreads();
     x_forward<T_, C, PL, OPQ>(v, lds, tw, t, col);
@@ -340,7 +358,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             if constexpr (GM == 1) {
                 Gv = gsl[fx * B + col];
             } else {
-                Gv = GM == 2 ? gpre[b * rL + q] : gcol[(unsigned)fx * gs];
+                Gv = GM == 2 ? gpre[b * rL + q] : gat(__umul24((unsigned)fx, gs));
                 if (mt.gconj) Gv = cconj(Gv);
             }
             const int e = b * rL + q;
@@ -362,8 +380,8 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
 #pragma unroll
                 for (int q = 0; q < r1; ++q) {
                     const int n = i + q * (P / r1);
-                    if (n < SUP && n + g.xs >= g.x0 && n + g.xs < g.x1)
-                        base[c * cst + (unsigned)n * rowst] = v[c][b * r1 + q];
+                    const unsigned off = (unsigned)((P / E) * b + q * (P / r1)) * rowst;
+                    if (n < SUP && n + g.xs >= g.x0 && n + g.xs < g.x1) at(c * cst + lane0 + off) = v[c][b * r1 + q];
                 }
             }
     }
