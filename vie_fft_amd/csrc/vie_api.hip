@@ -1197,6 +1197,22 @@ static std::vector<int> short_candidates(int w, int N, int axis)
     return v;
 }
 
+// A plan read from the cache file must be one the tuner could have picked
+// for this operator: a known variant, and per axis the full length (0) or a
+// compiled length that covers the window (a shorter period would change the
+// operator) and is shorter than the axis.
+template <class R> static bool plan_valid(const vie_ctx* c, const Op<R>* op, const TunePlan& p)
+{
+    if (p.var < -1 || p.var > 7) return false;
+    const int w[3] = {op->wx1 - op->wx0, op->wy1 - op->wy0, op->wz1 - op->wz0};
+    const int len[3] = {p.ns, p.nys, p.nzs};
+    for (int a = 0; a < 3; ++a) {
+        const int m = len[a], N = c->desc.n[a];
+        if (m != 0 && !(m > 0 && size_supported(m) && m >= w[a] && m < N && (a != 2 || N % 2 == 0))) return false;
+    }
+    return true;
+}
+
 template <class R> void tune_op(vie_ctx* c, Op<R>* op)
 {
     op->tuned = false;
@@ -1207,6 +1223,10 @@ template <class R> void tune_op(vie_ctx* c, Op<R>* op)
         std::lock_guard<std::mutex> lk(g_tune_mu);
         tune_cache_load();
         auto it = g_tune.find(key);
+        if (it != g_tune.end() && !plan_valid(c, op, it->second)) {
+            g_tune.erase(it);  // a stale or damaged cache line: tune again
+            it = g_tune.end();
+        }
         if (it != g_tune.end()) {
             op->var = it->second.var;
             op->tns = it->second.ns;
