@@ -628,6 +628,7 @@ struct vie_ctx {
     double dr[3] = {0, 0, 0};  // cell sizes in the caller's axis order
     bool remap = false;        // caller layout != internal layout (embedding or axis order)
     bool windows = true;       // x/y windows and the axis order that favours them (VIE_WINDOW=0: off)
+    bool no_graph = false;     // a round could not be captured as a graph: direct launches
     vie_desc desc{};
     hipStream_t stream = nullptr;
     std::string err;
@@ -1390,30 +1391,40 @@ static bool graphs_on()
     return v;
 }
 
+// If a capture cannot be made (the runtime refuses a call the round makes
+// while capturing, or the graph does not instantiate), the context runs its
+// rounds as direct launches from then on; nothing captured has executed, so
+// the round is simply enqueued again.
 template <class F> void round_of(vie_ctx* c, IterGraph& g, F& iter)
 {
-    if (!graphs_on() || c->timer.cls) {
+    auto direct = [&] {
         for (int k = 0; k < c->ahead; ++k) iter();
-        return;
-    }
+    };
+    if (!graphs_on() || c->timer.cls || c->no_graph) return direct();
     if (g.exec && (g.slots.nb != c->cur.nb || g.slots.ilist != c->cur.ilist)) {
         (void)hipGraphExecDestroy(g.exec);
         g.exec = nullptr;
     }
     if (!g.exec) {
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipGraph_t graph = nullptr;
-        try {
-            for (int k = 0; k < c->ahead; ++k) iter();
-        } catch (...) {
-            (void)hipStreamEndCapture(c->stream, &graph);
+        bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            hipGraph_t graph = nullptr;
+            try {
+                direct();
+            } catch (...) {
+                ok = false;
+            }
+            ok = hipStreamEndCapture(c->stream, &graph) == hipSuccess && ok && graph;
+            ok = ok && hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
             if (graph) (void)hipGraphDestroy(graph);
-            throw;
         }
-        HIPCHK(hipStreamEndCapture(c->stream, &graph));
-        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPCHK(e);
+        if (!ok) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            (void)hipGetLastError();  // the refused call's error, not the solve's
+            c->no_graph = true;
+            return direct();
+        }
         g.slots = c->cur;
     }
     HIPCHK(hipGraphLaunch(g.exec, c->stream));
