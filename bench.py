@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 96^3 dimer sweep and 256^3 legs")
     ap.add_argument("--xpass-timing", choices=["live", "separate"], default="separate",
                     help="time the FP32 x pass in one more mixed solve after the timed ones (separate), or "
-                         "during the timed solves (live: HIP events on one launch in eight, no graph replay)")
+                         "during the timed solves (live: dispatch events on one launch in eight, no graph replay)")
     ap.add_argument("--n", type=int, default=128, help="grid edge (128 = the metric workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a plain device copy of the x pass's working set (HBM counter calibration)")
@@ -160,10 +160,11 @@ def main():
     q64 = cross_sections(s, sc)
 
     # ---- timed: mixed solves (the metric).  The FP32 x pass (the roofline
-    # kernel) is timed with HIP events around one launch in eight on the
-    # library's stream, by default in one more solve after the timed ones so
-    # that the timed solves run uninstrumented (with events on, iterations are
-    # enqueued one by one instead of replayed as graphs)
+    # kernel) is timed per launch with HIP events recorded by the dispatch
+    # itself (one launch in eight, on the library's stream), by default in one
+    # more solve after the timed ones so that the timed solves run
+    # uninstrumented (with timing on, iterations are enqueued one by one
+    # instead of replayed as graphs)
     live = args.xpass_timing == "live"
     if live:
         s.time_kernel(13)

@@ -33,6 +33,9 @@
 
 using namespace vie;
 
+// the event pair armed for the next timed launch (see Timer)
+thread_local vie::LaunchEvents vie::g_launch_ev;
+
 namespace {
 
 const double kPi = 3.1415926535897932384626433832795;
@@ -92,11 +95,12 @@ template <class R> cx<R> hc(std::complex<double> z)
     return v;
 }
 
-// Kernel-class timing: HIP events around every launch of one pass.
 // Per-launch HIP-event timing of one kernel class.  mark() is called before
-// and after every launch of the class; one launch in kEvery is bracketed by
-// events (an event pair costs a few microseconds of stream time around the
-// launch, so timing every launch would slow the solve it measures).
+// and after every launch of the class; for one launch in kEvery the first
+// mark arms a start/stop event pair that the launch records with its own
+// dispatch (VIE_LAUNCH, hipExtLaunchKernel), so the time is the kernel's
+// alone.  A launch that leaves the pair armed (it did not go through
+// VIE_LAUNCH) is not sampled.
 struct Timer {
     static constexpr long kEvery = 8;
     int cls = 0;
@@ -104,25 +108,32 @@ struct Timer {
     size_t used = 0;
     long seen = 0;     // launches of the class since reset
     bool open = false; // between the two marks of a launch
-    bool take = false; // this launch is sampled
     void reset()
     {
         used = 0;
         seen = 0;
         open = false;
+        vie::g_launch_ev = LaunchEvents{};
     }
-    void mark(hipStream_t st)
+    void mark(hipStream_t)
     {
         if (!cls) return;
         open = !open;
-        if (open) take = seen++ % kEvery == 0;
-        if (!take) return;
-        if (used >= ev.size()) {
+        if (!open) {  // after the launch
+            if (vie::g_launch_ev.start) {
+                vie::g_launch_ev = LaunchEvents{};
+                used -= 2;
+            }
+            return;
+        }
+        if (seen++ % kEvery) return;
+        while (used + 2 > ev.size()) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
             ev.push_back(e);
         }
-        HIPCHK(hipEventRecord(ev[used++], st));
+        vie::g_launch_ev = LaunchEvents{ev[used], ev[used + 1]};
+        used += 2;
     }
     ~Timer()
     {
@@ -781,7 +792,7 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
     L.bt = c->cur;
     const int cls = sizeof(R) == 8 ? 6 : 16;
     if (c->timer.cls == cls) c->timer.mark(c->stream);
-    hipLaunchKernelGGL(k_lincomb<R>, dim3(gsz, gy, c->cur.nb), dim3(NT), 0, c->stream, L, done);
+    VIE_LAUNCH(k_lincomb<R>, dim3(gsz, gy, c->cur.nb), dim3(NT), c->stream, L, done);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == cls) c->timer.mark(c->stream);
     return (long)gsz * gy;
@@ -794,8 +805,8 @@ void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd, const int* mask = 
 {
     if (c->timer.cls == 7) c->timer.mark(c->stream);
     const bool single = c->cur.nb == 1 && !c->cur.ilist;
-    hipLaunchKernelGGL(k_scalar, dim3(c->cur.nb), dim3(NTS), 0, c->stream, op, s, c->partial, nblk, nd,
-                       single ? c->hist : nullptr, c->cur, mask);
+    VIE_LAUNCH(k_scalar, dim3(c->cur.nb), dim3(NTS), c->stream, op, s, c->partial, nblk, nd,
+               single ? c->hist : nullptr, c->cur, mask);
     HIPCHK(hipGetLastError());
     if (c->timer.cls == 7) c->timer.mark(c->stream);
 }

@@ -2,9 +2,36 @@
 // instantiated for double and float in vie_kernels_f64.hip / _f32.hip.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "vie_passes.h"
 
 namespace vie {
+
+// Kernel timing (vie_time_kernel): a sampled launch of the timed class takes
+// its start and stop events from here, armed just before it, and records
+// them with the dispatch itself (hipExtLaunchKernel), so the measured time is
+// the kernel's own, without the stream time an event record between two
+// launches adds.  Launches that find nothing armed go out as usual.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchEvents g_launch_ev;
+
+template <class F, class... Args>
+void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
+{
+    const LaunchEvents e = g_launch_ev;
+    g_launch_ev = LaunchEvents{};
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e.start, e.stop, 0, args...);
+}
+#define VIE_LAUNCH(K, G, B, S, ...)                               \
+    do {                                                          \
+        if (::vie::g_launch_ev.start)                             \
+            ::vie::launch_timed(K, G, B, S, __VA_ARGS__);         \
+        else                                                      \
+            hipLaunchKernelGGL(K, G, B, 0, S, __VA_ARGS__);       \
+    } while (0)
 
 template <class R> struct LinComb;
 

@@ -81,13 +81,13 @@ static hipError_t launch_zfwd(const Geo& g, const Lin<R>& lin, cx<R>* W1, const 
             using X = YZReg<R, n>;
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
-            hipLaunchKernelGGL((k_zfwd_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, lin, W1, twP, done);
+            VIE_LAUNCH((k_zfwd_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), st, g, lin, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_z<R>(n);
     dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
-    hipLaunchKernelGGL((k_zfwd<R, n, B, ZS>), grid, dim3(NT), 0, st, g, lin, W1, tw, twist, done);
+    VIE_LAUNCH((k_zfwd<R, n, B, ZS>), grid, dim3(NT), st, g, lin, W1, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -101,14 +101,14 @@ static hipError_t launch_yfwd(const Geo& g, const cx<R>* W1, cx<R>* W2, const cx
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
             dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
-            hipLaunchKernelGGL((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W1, W2, twP, done);
+            VIE_LAUNCH((k_yfwd_r<R, n, X::T, typename X::PL>), grid, dim3(NT), st, g, W1, W2, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
-    hipLaunchKernelGGL((k_yfwd<R, n, B>), grid, dim3(NT), 0, st, g, W1, W2, tw, twist, done);
+    VIE_LAUNCH((k_yfwd<R, n, B>), grid, dim3(NT), st, g, W1, W2, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -122,14 +122,14 @@ static hipError_t launch_yinv(const Geo& g, const cx<R>* W2, cx<R>* W1, const cx
             constexpr int B = NT / X::T;
             const long nzt = (2L * g.nzp + B - 1) / B;
             dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
-            hipLaunchKernelGGL((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), 0, st, g, W2, W1, twP, done);
+            VIE_LAUNCH((k_yinv_r<R, n, X::T, typename X::PL>), grid, dim3(NT), st, g, W2, W1, twP, done);
             return hipGetLastError();
         }
     }
     constexpr int B = b_y<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)((g.x1 - g.x0) * nzt), 3 * g.bt.nb);
-    hipLaunchKernelGGL((k_yinv<R, n, B>), grid, dim3(NT), 0, st, g, W2, W1, tw, twist, done);
+    VIE_LAUNCH((k_yinv<R, n, B>), grid, dim3(NT), st, g, W2, W1, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -145,7 +145,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
             constexpr int B = NT / X::T;
             dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
             if (nblk) *nblk = 3L * grid.x;  // per launch slot
-            hipLaunchKernelGGL((k_zinv_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg,
+            VIE_LAUNCH((k_zinv_r<R, n, X::T, typename X::PL, ZS>), grid, dim3(NT), st, g, W1, src, y, dg,
                                flags, dots, twP, done);
             return hipGetLastError();
         }
@@ -153,7 +153,7 @@ static hipError_t launch_zinv(const Geo& g, const cx<R>* W1, const cx<R>* src, c
     constexpr int B = b_z<R>(n);
     dim3 grid((unsigned)((rows + B - 1) / B), 3 * g.bt.nb);
     if (nblk) *nblk = 3L * grid.x;  // per launch slot
-    hipLaunchKernelGGL((k_zinv<R, n, B, ZS>), grid, dim3(NT), 0, st, g, W1, src, y, dg, flags, dots, tw, twist, done);
+    VIE_LAUNCH((k_zinv<R, n, B, ZS>), grid, dim3(NT), st, g, W1, src, y, dg, flags, dots, tw, twist, done);
     return hipGetLastError();
 }
 
@@ -272,17 +272,15 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
             const int need = g.x1 - g.xs;
             if constexpr (sizeof(R) == 4 && n % 4 == 0) {
                 if (need <= n / 2) {
-                    hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP, n / 2>), grid, dim3(NT), 0,
-                                       st, g, W2, twP, mt, done);
+                    VIE_LAUNCH((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP, n / 2>), grid, dim3(NT), st, g, W2, twP, mt, done);
                     return hipGetLastError();
                 }
                 if (need <= 3 * n / 4) {
-                    hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP, 3 * n / 4>), grid,
-                                       dim3(NT), 0, st, g, W2, twP, mt, done);
+                    VIE_LAUNCH((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP, 3 * n / 4>), grid, dim3(NT), st, g, W2, twP, mt, done);
                     return hipGetLastError();
                 }
             }
-            hipLaunchKernelGGL((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP>), grid, dim3(NT), 0, st, g, W2, twP, mt,
+            VIE_LAUNCH((k_xreg<R, n, X::T, typename X::PL, X::W, X::GM, X::OP>), grid, dim3(NT), st, g, W2, twP, mt,
                                done);
             return hipGetLastError();
         }
@@ -290,7 +288,7 @@ static hipError_t launch_x(const Geo& g, cx<R>* W2, const cx<R>* tw, const cx<R>
     constexpr int B = b_x<R>(n);
     const long nzt = (2L * g.nzp + B - 1) / B;
     dim3 grid((unsigned)(2L * g.nyp * nzt), g.bt.nb);
-    hipLaunchKernelGGL((k_xmix<R, n, B>), grid, dim3(NT), 0, st, g, W2, tw, twist, mt, done);
+    VIE_LAUNCH((k_xmix<R, n, B>), grid, dim3(NT), st, g, W2, tw, twist, mt, done);
     return hipGetLastError();
 }
 
