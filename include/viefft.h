@@ -228,6 +228,12 @@ int vie_batch_get_solution(vie_ctx* ctx, int i, double* x);
    wavenumber and permittivity */
 int vie_batch_forward_scatter(vie_ctx* ctx, int i, double theta, double phi, double psi, double* value);
 int vie_batch_absorbed(vie_ctx* ctx, int i, int mat, double* value);
+/* Both of the above for instances 0..nb-1 with one read-back (no counterpart
+   in the reference, whose processes each print their own): theta, phi, psi
+   (nb each, degrees) per instance; forward[i] = Forward_RCS; absorbed (may be
+   NULL, else nb) = Ene_Metal of material mat. */
+int vie_batch_outputs(vie_ctx* ctx, int nb, const double* theta, const double* phi, const double* psi, int mat,
+                      double* forward, double* absorbed);
 
 /* Resident right-hand side computed on the device: Incident_Wave
    (vie_fft/GreenFunction.h:144-248) for angles theta, phi, psi (degrees)

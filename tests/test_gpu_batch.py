@@ -54,6 +54,11 @@ def batch_of(points, mixed, method=VF.BICGSTAB, tol=1e-8, maxits=None, precision
         qs = [s.batch_forward_scatter(i, sc.theta, sc.phi, psi) for i, (sc, (lam, psi)) in
               enumerate(zip(scenes, points))]
         ab = [s.batch_absorbed(i, 1) for i in range(len(points))]
+        # all instances' outputs with one read-back: the same values
+        fw, ab2 = s.batch_outputs([sc.theta for sc in scenes], [sc.phi for sc in scenes], [p for _, p in points], 1)
+        assert list(fw) == qs and list(ab2) == ab
+        fw1, none = s.batch_outputs([sc.theta for sc in scenes], [sc.phi for sc in scenes], [p for _, p in points])
+        assert list(fw1) == qs and none is None
     return st, infos, xs, qs, ab, scenes
 
 

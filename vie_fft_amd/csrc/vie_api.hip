@@ -454,22 +454,26 @@ template <class R> struct Op {
         mab.Dd = Dab;
     }
     // instance i's k0 tables (host part: Vv, adjoint U, D)
-    void batch_k0(int i, double kk, hipStream_t st)
+    // the k0 tables of instances [i0, i0 + cnt): built on the host for all of
+    // them, one upload per table
+    void batch_k0(int i0, int cnt, const double* ks, hipStream_t st)
     {
-        std::vector<T> v(2 * lb), ua(2 * lb);
+        std::vector<T> v(2L * lb * cnt), ua(2L * lb * cnt), df(3L * cnt), da(3L * cnt);
         for (int a = 0; a < 3; ++a) {
-            k0_tables(bl[a], d[a], kk, v.data(), ua.data());
-            copy_sync(Vb[a] + 2L * lb * i, v.data(), 2 * bl[a] * sizeof(T), hipMemcpyHostToDevice, st);
-            copy_sync(Uab[a] + 2L * lb * i, ua.data(), 2 * bl[a] * sizeof(T), hipMemcpyHostToDevice, st);
+            for (int i = 0; i < cnt; ++i) k0_tables(bl[a], d[a], ks[i], &v[2L * lb * i], &ua[2L * lb * i]);
+            HIPCHK(hipMemcpyAsync(Vb[a] + 2L * lb * i0, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(Uab[a] + 2L * lb * i0, ua.data(), ua.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));  // v and ua are refilled for the next axis
         }
-        T df[3], da[3];
-        for (int a = 0; a < 3; ++a) {
-            df[a] = d_of(a, kk);
-            da[a] = df[a];
-            da[a].y = -da[a].y;
-        }
-        copy_sync(Db + 3L * i, df, sizeof(df), hipMemcpyHostToDevice, st);
-        copy_sync(Dab + 3L * i, da, sizeof(da), hipMemcpyHostToDevice, st);
+        for (int i = 0; i < cnt; ++i)
+            for (int a = 0; a < 3; ++a) {
+                df[3 * i + a] = d_of(a, ks[i]);
+                da[3 * i + a] = df[3 * i + a];
+                da[3 * i + a].y = -da[3 * i + a].y;
+            }
+        HIPCHK(hipMemcpyAsync(Db + 3L * i0, df.data(), df.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(Dab + 3L * i0, da.data(), da.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
     }
 
     // The wavenumber enters the tables through v_s and D_f = j k0 Z0
@@ -698,6 +702,8 @@ struct vie_ctx {
     std::vector<double> btab;      // their eps tables (bnmat complex each)
     int bnmat = 0;
     double2* beps = nullptr;       // per-instance permittivity (V complex, post-processing scratch)
+    double2* btabd = nullptr;      // the eps tables on the device, 256 entries per instance
+    long btabd_cap = 0;            // instances btabd holds
     bool guess = false;            // the next solve starts from the guess in the VW vector
     bool x_held = false;           // x64 holds a solution in the current layout (a guess for the next solve)
     double dvg = 0.0;              // divergence stop factor of the solvers (0: off)
@@ -2516,21 +2522,36 @@ void rhs_host(vie_ctx* c, const double* b, double2* dst)
 }
 
 // FarField1's Forward_RCS (vie_fft/NearFar.h:16-109) of the field J (internal
-// layout, device) for wavenumber k0.
-double forward_scatter_of(vie_ctx* c, const double2* J, double k0, double theta, double phi, double psi)
+// layout, device) for wavenumber k0, in two halves so that many fields can
+// share one read-back: out_gsz() partial-sum blocks; far_launch writes 3 of
+// them per field into part, far_finish turns them (on the host) into the value.
+unsigned out_gsz(const vie_ctx* c) { return grid1(c->V, NT, 1024); }
+
+void far_dir(const vie_ctx* c, double theta, double phi, double vd[3])
+{
+    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0;
+    const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+    for (int a = 0; a < 3; ++a) vd[a] = vc[c->perm[a]];  // internal axes
+}
+
+void far_launch(vie_ctx* c, const double2* J, double k0, double theta, double phi, double2* part)
 {
     const int* n = c->desc.n;
     const double* d = c->desc.d;
-    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
-    const double vc[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
-    const double vd[3] = {vc[c->perm[0]], vc[c->perm[1]], vc[c->perm[2]]};  // internal axes
-    const unsigned gsz = grid1(c->V, NT, 1024);
-    hipLaunchKernelGGL(k_farsum, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], k0, vd[0],
-                       vd[1], vd[2], J, c->partial);
+    double vd[3];
+    far_dir(c, theta, phi, vd);
+    hipLaunchKernelGGL(k_farsum, dim3(out_gsz(c)), dim3(NT), 0, c->stream, n[0], n[1], n[2], d[0], d[1], d[2], k0,
+                       vd[0], vd[1], vd[2], J, part);
     HIPCHK(hipGetLastError());
-    std::vector<double2> h(3 * gsz);
-    HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+double far_finish(const vie_ctx* c, const double2* h, double k0, double theta, double phi, double psi)
+{
+    const double* d = c->desc.d;
+    const unsigned gsz = out_gsz(c);
+    const double th = kPi * theta / 180.0, ph = kPi * phi / 180.0, ps = kPi * psi / 180.0;
+    double vd[3];
+    far_dir(c, theta, phi, vd);
     std::complex<double> S[3];
     const double dV = d[0] * d[1] * d[2];
     for (int cpt = 0; cpt < 3; ++cpt) {
@@ -2544,23 +2565,42 @@ double forward_scatter_of(vie_ctx* c, const double2* J, double k0, double theta,
     return std::cos(ps) * tm - std::sin(ps) * te;
 }
 
+double forward_scatter_of(vie_ctx* c, const double2* J, double k0, double theta, double phi, double psi)
+{
+    far_launch(c, J, k0, theta, phi, c->partial);
+    std::vector<double2> h(3 * out_gsz(c));
+    HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return far_finish(c, h.data(), k0, theta, phi, psi);
+}
+
 // NearField's Ene_Metal (vie_fft/NearFar.h:180-250) of the field J for
-// material `mat`, permittivity eps (device, V) and wavenumber k0.
-double absorbed_of(vie_ctx* c, const double2* J, const double2* eps, double k0, int mat)
+// material `mat`, permittivity eps (device, V) and wavenumber k0: out_gsz()
+// partial sums into part (abs_launch), their sum (abs_finish).
+void abs_launch(vie_ctx* c, const double2* J, const double2* eps, double k0, int mat, double2* part)
 {
     const int* n = c->desc.n;
     const double* d = c->desc.d;
-    const unsigned gsz = grid1(c->V, NT, 1024);
     const int* p = c->perm;  // the caller's extents along the internal axes
-    hipLaunchKernelGGL(k_absorb, dim3(gsz), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[p[0]], c->nu[p[1]],
-                       c->nu[p[2]], d[0] * d[1] * d[2], k0, kZ0, J, eps, c->mat_dev, mat, c->partial);
+    hipLaunchKernelGGL(k_absorb, dim3(out_gsz(c)), dim3(NT), 0, c->stream, n[0], n[1], n[2], c->nu[p[0]],
+                       c->nu[p[1]], c->nu[p[2]], d[0] * d[1] * d[2], k0, kZ0, J, eps, c->mat_dev, mat, part);
     HIPCHK(hipGetLastError());
-    std::vector<double2> h(gsz);
+}
+
+double abs_finish(const vie_ctx* c, const double2* h)
+{
+    double acc = 0;
+    for (unsigned b = 0; b < out_gsz(c); ++b) acc += h[b].x;
+    return acc;
+}
+
+double absorbed_of(vie_ctx* c, const double2* J, const double2* eps, double k0, int mat)
+{
+    abs_launch(c, J, eps, k0, mat, c->partial);
+    std::vector<double2> h(out_gsz(c));
     HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    double acc = 0;
-    for (unsigned b = 0; b < gsz; ++b) acc += h[b].x;
-    return acc;
+    return abs_finish(c, h.data());
 }
 
 // Bytes of device storage one more instance of a batch needs (solver
@@ -2657,8 +2697,7 @@ void batch_reserve(vie_ctx* c, int nb)
 // context's permittivity for every instance).  The G~ octants of all of them
 // come from one set of launches (one workgroup row per instance); each
 // instance's tables are computed as the single-instance path computes them.
-template <class R> void batch_tables(vie_ctx* c, Op<R>* op, int i0, int cnt, const double* k0s,
-                                     const double2* gall, const double2* epsv)
+template <class R> void batch_tables(vie_ctx* c, Op<R>* op, int i0, int cnt, const double* k0s, const double2* gall)
 {
     op->batch_alloc(c->nbcap);
     const long tot = op->gob;
@@ -2670,17 +2709,22 @@ template <class R> void batch_tables(vie_ctx* c, Op<R>* op, int i0, int cnt, con
                            cnt * tot, gall, op->Gb + i0 * tot);
         HIPCHK(hipGetLastError());
     }
+    op->batch_k0(i0, cnt, k0s, c->stream);
+}
+
+// d_f of instance i (vie_fft/fft.h:405-588) for both operators from its
+// permittivity e (device, V)
+void batch_diag(vie_ctx* c, Op<double>* o64, Op<float>* o32, int i, double k0, const double2* e)
+{
     const int* n = c->desc.n;
     const double* d = c->desc.d;
-    for (int i = 0; i < cnt; ++i) {
-        op->batch_k0(i0 + i, k0s[i], c->stream);
-        double cim[3];
-        for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / k0s[i];
-        const double2* e = epsv ? epsv + (long)i * c->V : c->eps_dev;
-        hipLaunchKernelGGL(k_diag<R>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
-                           cim[2], e, op->dgb + (long)(i0 + i) * 3 * c->V);
-        HIPCHK(hipGetLastError());
-    }
+    double cim[3];
+    for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / k0;
+    hipLaunchKernelGGL(k_diag<double>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                       cim[2], e, o64->dgb + (long)i * 3 * c->V);
+    hipLaunchKernelGGL(k_diag<float>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
+                       cim[2], e, o32->dgb + (long)i * 3 * c->V);
+    HIPCHK(hipGetLastError());
 }
 
 void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const double* eps_tables, int n_mat)
@@ -2699,7 +2743,6 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
     double2* w2 = nullptr;
     double* kd = nullptr;
     double2* ev = nullptr;
-    double2* tab = nullptr;
     double* cd_ = nullptr;
     try {
         w2 = dalloc<double2>(cnt * tot);
@@ -2728,26 +2771,39 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
         hipLaunchKernelGGL(k_scale_c64, dim3(grid1(cnt * tot, 256, 1L << 30)), dim3(256), 0, c->stream, cnt * tot,
                            (double)n[0] * n[1] * n[2] / ((double)l[0] * l[1] * l[2]), src);
         HIPCHK(hipGetLastError());
-        // per-instance permittivity from the tables
+        batch_tables<double>(c, o64, i0, cnt, k0s, src);
+        batch_tables<float>(c, o32, i0, cnt, k0s, src);
+        // per-instance permittivity from the tables (all uploaded at once),
+        // one instance at a time through one V-sized buffer, and its d_f
         if (eps_tables) {
             require(c, c->mat_dev != nullptr, "permittivity tables need a medium with material ids");
             require(c, n_mat >= 1 && n_mat <= 256, "n_mat must be 1..256");
             for (long v = 0; v < 2L * cnt * n_mat; ++v)
                 if (!std::isfinite(eps_tables[v])) throw Err(VIE_ERR_NAN, "non-finite permittivity");
-            ev = dalloc<double2>(cnt * c->V);
-            tab = dalloc<double2>(256);
-            for (int i = 0; i < cnt; ++i) {
-                std::vector<double2> t(256, make_double2(kEpsBackground, 0.0));
-                for (int m = 0; m < n_mat; ++m)
-                    t[m] = make_double2(eps_tables[2L * (i * n_mat + m)], eps_tables[2L * (i * n_mat + m) + 1]);
-                copy_sync(tab, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
-                hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev, tab,
-                                   ev + (long)i * c->V);
-                HIPCHK(hipGetLastError());
+            ev = dalloc<double2>(c->V);
+            if (c->btabd_cap < c->nbcap) {
+                dfree(c->btabd);
+                c->btabd = nullptr;
+                c->btabd_cap = 0;
+                c->btabd = dalloc<double2>(256L * c->nbcap);
+                c->btabd_cap = c->nbcap;
             }
+            double2* tab_d = c->btabd + 256L * i0;  // kept for the per-instance outputs
+            std::vector<double2> t(256L * cnt, make_double2(kEpsBackground, 0.0));
+            for (int i = 0; i < cnt; ++i)
+                for (int m = 0; m < n_mat; ++m)
+                    t[256L * i + m] =
+                        make_double2(eps_tables[2L * (i * n_mat + m)], eps_tables[2L * (i * n_mat + m) + 1]);
+            copy_sync(tab_d, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+            for (int i = 0; i < cnt; ++i) {
+                hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev,
+                                   tab_d + 256L * i, ev);
+                HIPCHK(hipGetLastError());
+                batch_diag(c, o64, o32, i0 + i, k0s[i], ev);
+            }
+        } else {
+            for (int i = 0; i < cnt; ++i) batch_diag(c, o64, o32, i0 + i, k0s[i], c->eps_dev);
         }
-        batch_tables<double>(c, o64, i0, cnt, k0s, src, ev);
-        batch_tables<float>(c, o32, i0, cnt, k0s, src, ev);
         HIPCHK(hipStreamSynchronize(c->stream));
         if (eps_tables) {  // kept for the per-instance absorption
             if ((int)c->btab.size() != 2 * c->nbcap * n_mat || c->bnmat != n_mat) {
@@ -2761,7 +2817,6 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
         dfree(w2);
         dfree(kd);
         dfree(ev);
-        dfree(tab);
         dfree(cd_);
         throw;
     }
@@ -2769,7 +2824,6 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
     dfree(w2);
     dfree(kd);
     dfree(ev);
-    dfree(tab);
     dfree(cd_);
     for (int i = 0; i < cnt; ++i) {
         c->bk0[i0 + i] = k0s[i];
@@ -2784,21 +2838,10 @@ const double2* batch_eps(vie_ctx* c, int i)
 {
     if (!c->bmulti || c->bop[i] != 2) return c->eps_dev;
     if (!c->beps) c->beps = dalloc<double2>(c->V);
-    std::vector<double2> t(256, make_double2(kEpsBackground, 0.0));
-    for (int m = 0; m < c->bnmat; ++m)
-        t[m] = make_double2(c->btab[2L * (i * c->bnmat + m)], c->btab[2L * (i * c->bnmat + m) + 1]);
-    double2* tab = dalloc<double2>(256);
-    try {
-        copy_sync(tab, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice, c->stream);
-        hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev, tab, c->beps);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->stream));
-    } catch (...) {
-        dfree(tab);
-        throw;
-    }
-    dfree(tab);
-    return c->beps;
+    hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev,
+                       c->btabd + 256L * i, c->beps);
+    HIPCHK(hipGetLastError());
+    return c->beps;  // ready in stream order
 }
 
 // ||b_i - A64 x_i|| / ||b_i|| for the bn instances (FP64 operators), into res.
@@ -2947,6 +2990,7 @@ int vie_destroy(vie_ctx* c)
     dfree(c->btol);
     dfree(c->bmaxit);
     dfree(c->beps);
+    dfree(c->btabd);
     if (c->snap) (void)hipHostFree(c->snap);
     for (hipEvent_t e : c->snap_ev)
         if (e) (void)hipEventDestroy(e);
@@ -3928,6 +3972,36 @@ int vie_batch_absorbed(vie_ctx* c, int i, int mat, double* value)
         require(c, i < c->nbcap, "instance beyond the reserved batch");
         require(c, c->mat_dev != nullptr, "vie_batch_absorbed needs material ids (vie_set_medium)");
         *value = absorbed_of(c, c->x64 + 3 * c->V * i, batch_eps(c, i), batch_k0(c, i), mat);
+        return VIE_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int vie_batch_outputs(vie_ctx* c, int nb, const double* theta, const double* phi, const double* psi, int mat,
+                      double* forward, double* absorbed)
+{
+    if (!c || nb < 0 || (nb > 0 && (!theta || !phi || !psi || !forward))) return VIE_ERR_ARG;
+    try {
+        require(c, c->sol_ready, "no solution on the device");
+        require(c, nb <= c->nbcap, "instances beyond the reserved batch");
+        if (absorbed) require(c, c->mat_dev != nullptr, "absorbed power needs material ids (vie_set_medium)");
+        if (nb == 0) return VIE_OK;
+        const unsigned gsz = out_gsz(c);
+        const long per = (absorbed ? 4L : 3L) * gsz;  // partial sums per instance
+        ensure_partial(c, per * nb);
+        for (int i = 0; i < nb; ++i) {
+            const double2* J = c->x64 + 3 * c->V * i;
+            far_launch(c, J, batch_k0(c, i), theta[i], phi[i], c->partial + per * i);
+            if (absorbed) abs_launch(c, J, batch_eps(c, i), batch_k0(c, i), mat, c->partial + per * i + 3 * gsz);
+        }
+        std::vector<double2> h(per * nb);
+        HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nb; ++i) {
+            forward[i] = far_finish(c, &h[per * i], batch_k0(c, i), theta[i], phi[i], psi[i]);
+            if (absorbed) absorbed[i] = abs_finish(c, &h[per * i + 3 * gsz]);
+        }
         return VIE_OK;
     } catch (const Err& e) {
         return fail(c, e);

@@ -211,10 +211,15 @@ def run_sweep_batched(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB
             t1 = time.perf_counter()
             _, infos = s.batch_solve(nb, method=method, tol=tol, maxit=maxit, mixed=mixed)
             t2 = time.perf_counter()
+            # every instance's outputs with one read-back
+            angles = ([sc.theta for sc in part], [sc.phi for sc in part], [psi for _, psi in points[b0:b0 + nb]])
+            fw, ab = s.batch_outputs(*angles, mat=metal)
+            if active is not None and vol_active > 0:
+                _, aa = s.batch_outputs(*angles, mat=active)
             for i, (sc, (lam, psi)) in enumerate(zip(part, points[b0:b0 + nb])):
-                q_ext = s.batch_forward_scatter(i, sc.theta, sc.phi, psi) / sc.area
-                q_abs = s.batch_absorbed(i, metal) * 120.0 * np.pi / sc.area
-                act = 3.0 * s.batch_absorbed(i, active) / vol_active if active is not None and vol_active > 0 else 0.0
+                q_ext = fw[i] / sc.area
+                q_abs = ab[i] * 120.0 * np.pi / sc.area
+                act = 3.0 * aa[i] / vol_active if active is not None and vol_active > 0 else 0.0
                 info = infos[i]
                 r = {"wavelength_nm": float(lam), "psi": float(psi), "q_ext": q_ext, "q_abs": q_abs,
                      "active_density": act, "iterations": info["iterations"], "outer": info["outer"],

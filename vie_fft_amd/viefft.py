@@ -37,6 +37,7 @@ EXPORTS = [
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
     "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
     "vie_history", "vie_set_divergence_stop", "vie_tuning", "vie_voxelize", "vie_set_medium_shapes",
+    "vie_batch_outputs",
 ]
 
 
@@ -157,6 +158,7 @@ def load():
     L.vie_batch_get_solution.argtypes = [vp, ct.c_int, dp]
     L.vie_batch_forward_scatter.argtypes = [vp, ct.c_int, ct.c_double, ct.c_double, ct.c_double, dp]
     L.vie_batch_absorbed.argtypes = [vp, ct.c_int, ct.c_int, dp]
+    L.vie_batch_outputs.argtypes = [vp, ct.c_int, dp, dp, dp, ct.c_int, dp, dp]
     _lib = L
     return L
 
@@ -458,6 +460,19 @@ class Solver:
         v = ct.c_double()
         self._check(self.lib.vie_batch_absorbed(self.h, int(i), int(mat), ct.byref(v)))
         return v.value
+
+    def batch_outputs(self, theta, phi, psi, mat=None):
+        """Forward scatter (and, with mat, absorbed power) of instances 0..len(theta)-1
+        with one read-back: (forward, absorbed or None) arrays."""
+        th, ph, ps = (np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel()) for a in (theta, phi, psi))
+        nb = th.size
+        if ph.size != nb or ps.size != nb:
+            raise ValueError("theta, phi and psi need one angle per instance")
+        fw = np.zeros(nb)
+        ab = np.zeros(nb) if mat is not None else None
+        self._check(self.lib.vie_batch_outputs(self.h, nb, _dp(th), _dp(ph), _dp(ps), int(mat or 0), _dp(fw),
+                                               _dp(ab) if ab is not None else None))
+        return fw, ab
 
     # -- instrumentation -----------------------------------------------------
     def tuning(self):
