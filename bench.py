@@ -41,7 +41,7 @@ XPASS_F32_128_PLAN = {"axis_order": (2, 0, 1), "x_window": (40, 88), "x_pencil":
 # VALU instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, median launch):
 # profiles/k_xreg_f32_128cube_valu.md.  The pass is VALU-bound; MI355X issues at most
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles = 614.4e9 wave64 VALU instructions per second.
-XPASS_F32_128_VALU = 24272896
+XPASS_F32_128_VALU = 24502272
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # register-resident x-pass kernels by pencil length (FP32), without the last

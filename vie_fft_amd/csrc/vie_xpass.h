@@ -134,13 +134,17 @@ __device__ __forceinline__ void xf_get(T_* v, const T_* lds, int t, int col)
         }
     }
 }
-template <class T_, class C, class PL, int s>
+// DB: the caller alternates between two exchange buffers, so the barrier
+// after the reads is not needed (a buffer is written again only two
+// exchanges later, after every thread has passed the barrier in between,
+// which waits for its reads)
+template <class T_, class C, class PL, int s, bool DB = false>
 __device__ __forceinline__ void xf_xchg(T_* v, T_* lds, int t, int col)
 {
     xf_put<T_, C, PL, s>(v, lds, t, col);
     __syncthreads();
     xf_get<T_, C, PL, s>(v, lds, t, col);
-    __syncthreads();
+    if constexpr (!DB) __syncthreads();
 }
 
 // exchange after inverse stage s (s > 0): write at stage-s input positions
@@ -181,13 +185,13 @@ __device__ __forceinline__ void xi_get(T_* v, const T_* lds, int t, int col)
         }
     }
 }
-template <class T_, class C, class PL, int s>
+template <class T_, class C, class PL, int s, bool DB = false>
 __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
 {
     xi_put<T_, C, PL, s>(v, lds, t, col);
     __syncthreads();
     xi_get<T_, C, PL, s>(v, lds, t, col);
-    __syncthreads();
+    if constexpr (!DB) __syncthreads();
 }
 
 // A copy of v the compiler cannot see through.  With OPQ each component gets
@@ -200,7 +204,9 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
-template <class T_, class C, class PL, bool OPQ, int s = 0>
+// DB: exchange k (in order: forward stages, then inverse stages, three
+// components each) uses buffer k mod 2 of lds (2 C::LDS entries)
+template <class T_, class C, class PL, bool OPQ, bool DB = false, int s = 0>
 __device__ __forceinline__ void x_forward(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
 {
     if constexpr (s < PL::n) {
@@ -208,13 +214,14 @@ __device__ __forceinline__ void x_forward(T_ (*v)[C::E], T_* lds, const T_* tw, 
         for (int c = 0; c < 3; ++c) {
             const int tc = OPQ ? opaque(t) : t, cc = OPQ ? opaque(col) : col;
             xf_stage<T_, C, PL, s>(v[c], tw, tc);
-            if constexpr (s + 1 < PL::n) xf_xchg<T_, C, PL, s>(v[c], lds, tc, cc);
+            if constexpr (s + 1 < PL::n)
+                xf_xchg<T_, C, PL, s, DB>(v[c], DB ? lds + ((3 * s + c) & 1) * C::LDS : lds, tc, cc);
         }
-        x_forward<T_, C, PL, OPQ, s + 1>(v, lds, tw, t, col);
+        x_forward<T_, C, PL, OPQ, DB, s + 1>(v, lds, tw, t, col);
     }
 }
 
-template <class T_, class C, class PL, bool OPQ, int s = PL::n - 1>
+template <class T_, class C, class PL, bool OPQ, bool DB = false, int s = PL::n - 1>
 __device__ __forceinline__ void x_inverse(T_ (*v)[C::E], T_* lds, const T_* tw, int t, int col)
 {
     if constexpr (s >= 0) {
@@ -222,9 +229,10 @@ __device__ __forceinline__ void x_inverse(T_ (*v)[C::E], T_* lds, const T_* tw, 
         for (int c = 0; c < 3; ++c) {
             const int tc = OPQ ? opaque(t) : t, cc = OPQ ? opaque(col) : col;
             xi_stage<T_, C, PL, s>(v[c], tw, tc);
-            if constexpr (s > 0) xi_xchg<T_, C, PL, s>(v[c], lds, tc, cc);
+            constexpr int k0 = 3 * (PL::n - 1) + 3 * (PL::n - 1 - s);  // exchanges before this stage's
+            if constexpr (s > 0) xi_xchg<T_, C, PL, s, DB>(v[c], DB ? lds + ((k0 + c) & 1) * C::LDS : lds, tc, cc);
         }
-        x_inverse<T_, C, PL, OPQ, s - 1>(v, lds, tw, t, col);
+        x_inverse<T_, C, PL, OPQ, DB, s - 1>(v, lds, tw, t, col);
     }
 }
 
@@ -256,7 +264,10 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     constexpr bool GSL = GM == 1;
-    __shared__ T_ lds[C::LDS];
+    // two exchange buffers (one barrier per exchange) where four workgroups
+    // per CU still fit in LDS with them
+    constexpr bool DB = 4 * (2 * C::LDS + 3 * P) * sizeof(T_) <= 160 * 1024;
+    __shared__ T_ lds[(DB ? 2 : 1) * C::LDS];
     __shared__ T_ tw[P];
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
     __shared__ T_ gsl[GSL ? (N + 1) * B : 1];    // G octant slice [fx][col]
@@ -343,7 +354,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
                 gpre[b * rL + q] = gat(__umul24((unsigned)fxof(t + (P / E) * b + q * (P / rL)), gs));
     }
     __syncthreads();
-    x_forward<T_, C, PL, OPQ>(v, lds, tw, t, col);
+    x_forward<T_, C, PL, OPQ, DB>(v, lds, tw, t, col);
 
     // D_f = j k0 Z0 delta_f / NP is imaginary: D_f X = (-Im D_f X.y, Im D_f X.x)
     const R D0 = it.D[0].y, D1 = it.D[1].y, D2 = it.D[2].y;
@@ -370,7 +381,7 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
             v[2][e] = cmul(Gv, cadd(mkc(-D2 * X2.y, D2 * X2.x), cmul(uz, sig)));
         }
     }
-    x_inverse<T_, C, PL, OPQ>(v, lds, tw, t, col);
+    x_inverse<T_, C, PL, OPQ, DB>(v, lds, tw, t, col);
     if (colok) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
