@@ -246,6 +246,16 @@ __device__ __forceinline__ int fold_split(int t, int n)
     return (kf & 1) ? (n / 2 + 1) + (kf >> 1) : (kf >> 1);
 }
 
+// Waves per SIMD (= workgroups per CU: a workgroup has one wave per SIMD)
+// that the x pass's register allocation allows, by configuration (the
+// compiler's VGPR counts: 4 for the FP32 passes up to 64 and the FP64 ones up
+// to 32, 2 for the 16-thread FP64 64 pass at 238 VGPRs; others are not
+// double-buffered).
+template <class R, int N, int T, int WPE> struct XWaves {
+    static constexpr int v = (sizeof(R) == 4 && N <= 64) || (sizeof(R) == 8 && N <= 32) ? 4
+                             : (sizeof(R) == 8 && N == 64 && T == 16 && WPE == 0) ? 2 : 8;
+};
+
 // grid (2Ny * ceil(2Nz / B)); N == Nx; in place on W2.
 // twP[e] = exp(-2 pi j e / P), e < P.  WPE: waves per SIMD the register
 // allocation aims for (0: compiler default).  GM: how the G octant is read --
@@ -264,9 +274,9 @@ void k_xreg(Geo g, cx<R>* __restrict__ W2, const cx<R>* __restrict__ twP, MixTab
     constexpr int P = C::P, E = C::E, B = C::B;
     constexpr int r1 = PL::r[0], rL = PL::r[PL::n - 1];
     constexpr bool GSL = GM == 1;
-    // two exchange buffers (one barrier per exchange) where four workgroups
-    // per CU still fit in LDS with them
-    constexpr bool DB = 4 * (2 * C::LDS + 3 * P) * sizeof(T_) <= 160 * 1024;
+    // two exchange buffers (one barrier per exchange) where the workgroups
+    // per CU that the registers allow (XWaves) still fit in LDS with them
+    constexpr bool DB = XWaves<R, N, T, WPE>::v * (2 * C::LDS + 3 * P) * sizeof(T_) <= 160 * 1024;
     __shared__ T_ lds[(DB ? 2 : 1) * C::LDS];
     __shared__ T_ tw[P];
     __shared__ T_ uv[2 * P];                      // U and Vv of the x axis by frequency
