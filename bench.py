@@ -248,6 +248,7 @@ def main():
     if plan == XPASS_F32_128_PLAN and ms13 > 0:
         out["roofline"]["valu"] = {"insts_per_launch": XPASS_F32_128_VALU, "issue_peak_per_s": VALU_ISSUE_PEAK,
                                    "frac": XPASS_F32_128_VALU / VALU_ISSUE_PEAK / (ms13 * 1e-3)}
+    out["jacobi"] = jacobi_leg(s, sc, args, sync, plan, x64, q64, t64)
     # measured practical HBM rate (device copy) beside the published peak
     _, _, copy_gbps = copy_rate(torch, n[0])
     out["roofline"]["copy_GBps"] = copy_gbps
@@ -259,6 +260,51 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
     print(json.dumps(out))
+
+
+def jacobi_leg(s, sc, args, sync, plan, x64, q64, t64_plain):
+    """The same workload with the Jacobi preconditioner (vie_set_preconditioner:
+    right-preconditioned BiCGSTAB on A (m (.) y) = b, m = 1/d_f, x = m (.) y;
+    the stop rule reads the same true residual): K timed mixed and K timed
+    FP64 solves after one untimed pair, their accuracy against the plain FP64
+    solution, and the whole-solve roofline fractions (the preconditioned
+    product moves the same bytes: its z forward pass reads m, its z inverse
+    pass no longer reads d)."""
+    from vie_fft_amd import viefft as VF
+
+    s.set_preconditioner(VF.PRECOND_JACOBI)
+    s.solve_mixed(tol=args.tol)
+    s.solve_resident(VF.BICGSTAB, tol=args.tol)
+    sync()
+    t0 = time.perf_counter()
+    im = [s.solve_mixed(tol=args.tol) for _ in range(args.steps)]
+    sync()
+    tm = (time.perf_counter() - t0) / args.steps
+    xm = s.solution()
+    qm = cross_sections(s, sc)
+    sync()
+    t0 = time.perf_counter()
+    i64 = [s.solve_resident(VF.BICGSTAB, tol=args.tol) for _ in range(args.steps)]
+    sync()
+    t64 = (time.perf_counter() - t0) / args.steps
+    xj = s.solution()
+    qj = cross_sections(s, sc)
+    s.set_preconditioner(VF.PRECOND_NONE)
+    itm, outer, it64 = im[-1]["iterations"], im[-1]["outer"], i64[-1]["iterations"]
+
+    def acc(x, q):
+        return {"solution_rel_l2": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+                "q_ext_rel": abs(q[0] - q64[0]) / abs(q64[0]), "q_abs_rel": abs(q[1] - q64[1]) / abs(q64[1])}
+
+    return {"mixed": {"solves_per_s": 1.0 / tm, "ms_per_solve": tm * 1e3, "iterations": itm,
+                      "us_per_iteration": tm * 1e6 / itm, "residual_replacements": outer,
+                      "true_relres": im[-1]["true_relres"], "vs_plain_fp64": acc(xm, qm)},
+            "fp64": {"solves_per_s": 1.0 / t64, "ms_per_solve": t64 * 1e3, "iterations": it64,
+                     "us_per_iteration": t64 * 1e6 / it64, "true_relres": i64[-1]["true_relres"],
+                     "vs_plain_fp64": acc(xj, qj)},
+            "speedup_mixed_vs_fp64": t64 / tm,
+            "speedup_mixed_vs_plain_fp64": t64_plain / tm,
+            "solve_roofline": solve_roofline(plan, tm, itm, outer, t64, it64)}
 
 
 def cross_sections(s, sc):

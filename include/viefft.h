@@ -246,6 +246,19 @@ int vie_solve_resident(vie_ctx* ctx, int method, double tol, int maxit, vie_info
    context ends with VIE_DIVERGED once the method's stop quantity exceeds
    factor times the smallest value it reached; 0 turns it off. */
 int vie_set_divergence_stop(vie_ctx* ctx, double factor);
+/* Jacobi preconditioner of the BiCGSTAB solves (no counterpart in the
+   reference, whose BICGSTAB_FFT, vie_fft/CG_Iteration.h:531-776, is
+   unpreconditioned; VIE_PRECOND_NONE, the default, keeps its iterates).  With
+   VIE_PRECOND_JACOBI every later BiCGSTAB solve of the context -- vie_solve,
+   vie_solve_x0, vie_solve_resident and the mixed solve -- iterates on the
+   right-preconditioned system A (m (.) y) = b with m = 1/d_f, the reciprocal
+   of the diagonal face term (LXX_DO/LYY_DO/LZZ_DO, vie_fft/fft.h:405-588),
+   and returns x = m (.) y: the stop rule still reads ||b - A x|| / ||b||, so
+   the solution meets the same tolerance in fewer iterations on media whose
+   permittivities differ strongly.  Other methods, batches, and media with an
+   active face whose d_f vanishes return VIE_ERR_ARG. */
+enum { VIE_PRECOND_NONE = 0, VIE_PRECOND_JACOBI = 1 };
+int vie_set_preconditioner(vie_ctx* ctx, int kind);
 /* Warm start (the reference always starts from x = 0, vie_fft/Model.h:189-191):
    the next vie_solve_resident / vie_solve_mixed_resident starts from x0
    (host, 3 V complex, masked by the flags like the solution), or from the

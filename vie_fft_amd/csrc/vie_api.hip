@@ -159,6 +159,11 @@ template <class R> struct Op {
     int* fold[6] = {};
     T* G = nullptr;
     T* dg = nullptr;           // per-face diagonal d_f (3V)
+    // Jacobi preconditioner m = 1/d_f on active faces (3V), built on first use
+    // after each change of dg (dgen counts the changes, pmgen is the dgen pm
+    // was built for)
+    T* pm = nullptr;
+    long dgen = 0, pmgen = -1;
     uint8_t* flags = nullptr;
     uint32_t* zr = nullptr;    // per z-row active range lo | hi << 16 (3 Nx Ny)
     int wx0 = 0, wx1 = 0;      // x window: the x planes with an active face
@@ -502,6 +507,7 @@ template <class R> struct Op {
         dfree(Gs);
         dfree(Gw);
         dfree(dg);
+        dfree(pm);
         dfree(flags);
         dfree(zr);
         if (own_w) {
@@ -533,6 +539,8 @@ template <class R> struct Op {
         // batched products are the solvers' (windowed, outputs kept zero)
         const bool batched = bt && (bt->nb > 1 || bt->ilist);
         if (batched && !(lin.zr && lin.wskip)) throw Err(VIE_ERR_ARG, "internal: unwindowed product in a batch");
+        if (lin.scale && (adj || batched || !mask))
+            throw Err(VIE_ERR_ARG, "internal: preconditioned product must be a forward, masked single product");
         if (batched) gw.bt = *bt;
         const bool multi = batched && bmulti;
         if (multi && (nbt == 0 || bl[0] != (nxs ? nxs : n[0]) || bl[1] != (nys ? nys : n[1]) ||
@@ -587,6 +595,7 @@ template <class R> struct Op {
         dd.conj = adj ? 1 : 0;
         dd.skip = lin.wskip;
         dd.nomask = mask ? 0 : 1;
+        dd.unit = lin.scale ? 1 : 0;  // lin.scale is this operator's pm: d (.) m = 1
         t0(5);
         HIPCHK(run_zinv<R>(gw, W1, src, y, dd, flags, dots, tw[az], twist[az], twP[az], done, st, &nblk));
         t0(5);
@@ -707,6 +716,7 @@ struct vie_ctx {
     bool guess = false;            // the next solve starts from the guess in the VW vector
     bool x_held = false;           // x64 holds a solution in the current layout (a guess for the next solve)
     double dvg = 0.0;              // divergence stop factor of the solvers (0: off)
+    int precond = 0;               // vie_preconditioner of the BiCGSTAB solves (0: none, the reference's)
 };
 
 // Krylov state records of a batch are this many ints / complex values apart.
@@ -809,6 +819,9 @@ template <class R> long lincomb(vie_ctx* c, LinComb<R> L, const int* done)
 // flag per instance state record): instances whose flag is set are skipped.
 void scalar(vie_ctx* c, KState* s, int op, long nblk, int nd, const int* mask = nullptr)
 {
+    if (nd < 1 || nd > 4 || nblk <= 0 || nblk >= (1L << 27))
+        throw Err(VIE_ERR_ARG, "internal: scalar step with " + std::to_string(nd) + " dots of " +
+                                   std::to_string(nblk) + " partial sums");
     if (c->timer.cls == 7) c->timer.mark(c->stream);
     const bool single = c->cur.nb == 1 && !c->cur.ilist;
     VIE_LAUNCH(k_scalar, dim3(c->cur.nb), dim3(NTS), c->stream, op, s, c->partial, nblk, nd,
@@ -950,6 +963,7 @@ template <class R> void upload_medium(vie_ctx* c, Op<R>* op)
     hipLaunchKernelGGL(k_diag<R>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
                        cim[2], c->eps_dev, op->dg);
     HIPCHK(hipGetLastError());
+    ++op->dgen;
     copy_sync(op->flags, c->flags_h.data(), c->flags_h.size(), hipMemcpyHostToDevice, c->stream);
     // x, y and z windows: the planes holding an active face of any component
     // (all of them when windows are off or no face is active); the z window
@@ -1512,13 +1526,46 @@ template <class F> KState drive_batch(vie_ctx* c, KState* st, F iter)
     return c->bstate[0];
 }
 
-// BiCGSTAB (vie_fft/CG_Iteration.h:531-776) on b -> x, both of precision R.
+// The Jacobi preconditioner of op (m = 1/d_f on active faces), rebuilt when
+// d_f has changed since it was built.  Right preconditioning: the solvers
+// iterate on A (m (.) y) = b, whose residual is b - A x for x = m (.) y, so the
+// stop rule reads the true residual as without it; with m = 1/d the
+// preconditioned product has a unit diagonal term (Diag::unit), so its z
+// inverse pass reads x instead of d and x, and its z forward pass reads m.
+// An active face whose d_f vanishes has no Jacobi scale: VIE_ERR_ARG.
+template <class R> const cx<R>* jacobi_of(vie_ctx* c, Op<R>* op)
+{
+    if (op->pmgen == op->dgen) return op->pm;
+    const long n = 3 * c->V;
+    if (!op->pm) op->pm = dalloc<cx<R>>(n);
+    int* bad = dalloc<int>(1);
+    int hbad = 0;
+    try {
+        HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_jacobi<R>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, op->dg, op->flags, op->pm, bad);
+        HIPCHK(hipGetLastError());
+        copy_sync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    } catch (...) {
+        dfree(bad);
+        throw;
+    }
+    dfree(bad);
+    if (hbad) throw Err(VIE_ERR_ARG, "Jacobi preconditioner: the diagonal term d_f vanishes on " +
+                                         std::to_string(hbad) + " active faces");
+    op->pmgen = op->dgen;
+    return op->pm;
+}
+
+// BiCGSTAB (vie_fft/CG_Iteration.h:531-776) on b -> x, both of precision R;
+// with c->precond the products are A (m (.) y) and x = m (.) y at the end.
 template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, cx<R>* x, double tol, int maxit)
 {
     KState* st = c->state;
     const long n = 3 * c->V;
     cx<R>*r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ), *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
     const int* done = &st->done;
+    if (c->precond && c->in_batch) throw Err(VIE_ERR_ARG, "the Jacobi preconditioner applies to single solves");
+    const cx<R>* pm = c->precond ? jacobi_of<R>(c, op) : nullptr;
     state_init(c, st, tol, maxit);
     // p, q and their products hold zero on inactive faces from here on: the
     // iterations leave those entries unwritten (wskip)
@@ -1544,6 +1591,7 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
+        lp.scale = pm;
         long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
@@ -1552,6 +1600,7 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
+        lq.scale = pm;
         nb = op->apply(lq, q, Aq, dots_with<R>(c, 2, {0, 1, 0, 0}, q, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_OMEGA, nb, 2);
         LinComb<R> L{};
@@ -1565,7 +1614,12 @@ template <class R> KState solve_bicgstab(vie_ctx* c, Op<R>* op, const cx<R>* b, 
         nb = lincomb<R>(c, L, done);
         scalar(c, st, SC_BICGSTAB_END, nb, 2);
     };
-    return drive(c, st, iter);
+    const KState h = drive(c, st, iter);
+    if (pm) {  // x = m (.) y
+        hipLaunchKernelGGL(k_cmul_diag<R>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, pm, x);
+        HIPCHK(hipGetLastError());
+    }
+    return h;
 }
 
 // CG on the normal equations (vie_fft/CG_Iteration.h:176-357).
@@ -1865,6 +1919,9 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
     float2 *b = vec<R>(c, VB), *x = vec<R>(c, VX), *r = vec<R>(c, VR), *p = vec<R>(c, VP), *q = vec<R>(c, VQ),
            *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
     const int* done = &st->done;
+    // right-preconditioned inner iterations: the FP32 iterate lives in the
+    // preconditioned variables and enters x64 as m (.) x32
+    const float2* pm = c->precond ? jacobi_of<float>(c, op) : nullptr;
     hipLaunchKernelGGL(k_convert<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->b64, b);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_fill_zero<double>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, c->x64);
@@ -1918,6 +1975,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         lp.x[0] = r; lp.x[1] = p; lp.x[2] = Ap;
         lp.coef = &st->cf[CF_BS_P];
         lp.out = p;
+        lp.scale = pm;
         long nb = op->apply(lp, p, Ap, dots_with<R>(c, 1, {1, 0}, b, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_ALPHA, nb, 1);
         Lin<R> lq{};
@@ -1926,6 +1984,7 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         lq.x[0] = r; lq.x[1] = Ap;
         lq.coef = &st->cf[CF_BS_Q];
         lq.out = q;
+        lq.scale = pm;
         nb = op->apply(lq, q, Aq, dots_with<R>(c, 2, {0, 1, 0, 0}, q, nullptr), false, done, c->stream, &c->timer);
         scalar(c, st, SC_BICGSTAB_OMEGA, nb, 2);
         LinComb<R> L{};
@@ -1961,8 +2020,12 @@ KState solve_bicgstab_reliable(vie_ctx* c, double tol, int maxit, double delta, 
         h = read_state(c, st);
         primed = false;
         const bool stop = h.status != KS_OK || h.iter >= h.maxit;
-        // reliable update: x64 += x32 ; x32 = 0 ; r = b - A64 x64 (in FP64, then FP32)
-        hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
+        // reliable update: x64 += x32 (m (.) x32 when preconditioned) ; x32 = 0 ;
+        // r = b - A64 x64 (in FP64, then FP32)
+        if (pm)
+            hipLaunchKernelGGL(k_axpy_scaled_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, pm, x, c->x64);
+        else
+            hipLaunchKernelGGL(k_axpy_to64<float>, dim3(grid1(n)), dim3(NT), 0, c->stream, n, 1.0, x, c->x64);
         HIPCHK(hipGetLastError());
         fill0<R>(c, x, n);
         op64->apply(lin_kw<double>(op64->zr, c->x64), c->x64, c->t64, nodots<double>(), false, nullptr, c->stream,
@@ -2028,6 +2091,7 @@ void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
            *Ap = vec<R>(c, VAP), *Aq = vec<R>(c, VAQ);
     const int* done = &st->done;
     const int* sel = &ax->done;  // replacement steps: instances with ax.done == 0
+    if (c->precond) throw Err(VIE_ERR_ARG, "the Jacobi preconditioner applies to single solves");
     batch_all(c);
     hipLaunchKernelGGL(k_convert<float>, dim3(grid1(nt, NT, 16384)), dim3(NT), 0, c->stream, nt, c->b64, b);
     HIPCHK(hipGetLastError());
@@ -2159,6 +2223,8 @@ void solve_bicgstab_reliable_batch(vie_ctx* c, double delta)
 template <class R> KState solve_method(vie_ctx* c, int method, const cx<R>* b, cx<R>* x, double tol, int maxit)
 {
     Op<R>* op = make_op<R>(c);
+    if (c->precond && method != VIE_BICGSTAB)
+        throw Err(VIE_ERR_ARG, "the Jacobi preconditioner applies to BiCGSTAB (method 3)");
     switch (method) {
     case VIE_CG: return solve_cg<R>(c, op, b, x, tol, maxit);
     case VIE_BICG: return solve_bicg<R>(c, op, b, x, tol, maxit);
@@ -3033,12 +3099,16 @@ static void rebuild_diag(vie_ctx* c, bool new_medium = true)
     const double* d = c->desc.d;
     double cim[3];
     for (int f = 0; f < 3; ++f) cim[f] = -(d[f] / 2.0) * kZ0 / c->desc.k0;
-    if (c->op64)
+    if (c->op64) {
         hipLaunchKernelGGL(k_diag<double>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
                            cim[2], c->eps_dev, c->op64->dg);
-    if (c->op32)
+        ++c->op64->dgen;
+    }
+    if (c->op32) {
         hipLaunchKernelGGL(k_diag<float>, dim3(grid1(c->V)), dim3(NT), 0, c->stream, n[0], n[1], n[2], cim[0], cim[1],
                            cim[2], c->eps_dev, c->op32->dg);
+        ++c->op32->dgen;
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!new_medium) return;
@@ -4328,6 +4398,17 @@ int vie_tuning(const vie_ctx* c, int* out, double* us)
     };
     put(c->op64, out, us, us ? us + 2 : nullptr);
     put(c->op32, out + 5, us ? us + 1 : nullptr, us ? us + 3 : nullptr);
+    return VIE_OK;
+}
+
+int vie_set_preconditioner(vie_ctx* c, int kind)
+{
+    if (!c) return VIE_ERR_ARG;
+    if (kind != VIE_PRECOND_NONE && kind != VIE_PRECOND_JACOBI) {
+        c->err = "preconditioner must be VIE_PRECOND_NONE or VIE_PRECOND_JACOBI";
+        return VIE_ERR_ARG;
+    }
+    c->precond = kind;
     return VIE_OK;
 }
 

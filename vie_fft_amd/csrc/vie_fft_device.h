@@ -287,12 +287,15 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblk)
 // range are then not read.
 // wskip (with zr): out, and the product this prologue feeds, already hold
 // zero outside the active faces, so those entries are not written.
+// scale (optional, 3V): the product transforms scale (.) z instead of z (out
+// still receives z): the right-preconditioned product A (m (.) z).
 template <class R> struct Lin {
     const cx<R>* x[3];
     const double2* coef;  // 3 complex; nullptr means (1, 0, 0)
     cx<R>* out;           // optional: store z (full vector, before padding)
     const uint32_t* zr;
     int wskip;
+    const cx<R>* scale;
 };
 
 // Dot-product epilogue: partial[k * nblk + blk] = sum conj(a_k) b_k (or

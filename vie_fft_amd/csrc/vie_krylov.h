@@ -265,6 +265,48 @@ __global__ void k_axpy_to64(long n, double s, const cx<R>* e, double2* x)
     }
 }
 
+// x64 += m (.) e (m and e in the inner precision): the correction update of
+// the mixed solve when its inner iterations are right-preconditioned (the
+// inner iterate e lives in the preconditioned variables, x = m (.) e).
+template <class R>
+__global__ void k_axpy_scaled_to64(long n, const cx<R>* m, const cx<R>* e, double2* x)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const double mx = (double)m[t].x, my = (double)m[t].y, ex = (double)e[t].x, ey = (double)e[t].y;
+        x[t].x += mx * ex - my * ey;
+        x[t].y += mx * ey + my * ex;
+    }
+}
+
+// x = m (.) x: a right-preconditioned solve's iterate back to the unknowns.
+template <class R>
+__global__ void k_cmul_diag(long n, const cx<R>* m, cx<R>* x)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        x[t] = cmul(m[t], x[t]);
+}
+
+// Jacobi preconditioner of the diagonal term: m_f = 1/d_f on active faces, 0
+// elsewhere (computed in FP64 from the operator's own d_f).  *bad counts the
+// active faces whose d_f is zero or whose reciprocal is not finite.
+template <class R>
+__global__ void k_jacobi(long n, const cx<R>* d, const uint8_t* flags, cx<R>* m, int* bad)
+{
+    int nb = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        double2 r = make_double2(0, 0);
+        if (flags[t]) {
+            r = zdiv(make_double2(1, 0), make_double2((double)d[t].x, (double)d[t].y));
+            if (!(isfinite(r.x) && isfinite(r.y)) || (d[t].x == 0 && d[t].y == 0)) {
+                ++nb;
+                r = make_double2(0, 0);
+            }
+        }
+        m[t] = mkc((R)r.x, (R)r.y);
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+
 // out = (R)(s * in): scaled conversion of an FP64 residual to the inner precision.
 template <class R>
 __global__ void k_scale_from64(long n, double s, const double2* in, cx<R>* out)

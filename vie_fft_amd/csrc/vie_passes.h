@@ -118,6 +118,8 @@ template <class R> struct Diag {
     int skip;             // y already holds zero on inactive faces: leave them unwritten
     long dstr = 0;        // batches: instance i reads d + i dstr (0: shared)
     int nomask = 0;       // every face counts as active: no output mask (Current_Reset left out)
+    int unit = 0;         // the diagonal term is x itself (d (.) m = 1: the Jacobi-preconditioned
+                          // product, whose input the z forward pass scaled by m = 1/d); d is not read
 };
 
 // Block-wide sum of a double2; sh holds NT/64 entries.  Result valid in thread 0.
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
     if constexpr (N % 2 == 0) {
         // all loads first, then the combination, stores and LDS writes
         constexpr int IT = (B * N / 2 + NT - 1) / NT;
-        T x0[IT][2], x1[IT][2], x2[IT][2];
+        T x0[IT][2], x1[IT][2], x2[IT][2], sc[IT][2];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int e = threadIdx.x + it * NT;
@@ -257,6 +259,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
             const bool rok = e < B * N / 2 && r0 + col < nw;
             const long row = rok ? rw[col] : 0;
             x0[it][0] = x0[it][1] = x1[it][0] = x1[it][1] = x2[it][0] = x2[it][1] = czero<T>();
+            sc[it][0] = sc[it][1] = czero<T>();
             // with row ranges, pairs that miss the row's active range are zero
             const unsigned rg = e < B * N / 2 ? zrs[col] : 0u;
             const int z = n + zs;  // z plane of entry n
@@ -265,6 +268,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                 ld2(lin.x[0] + idx, x0[it][0], x0[it][1]);
                 if (lin.x[1]) ld2(lin.x[1] + idx, x1[it][0], x1[it][1]);
                 if (lin.x[2]) ld2(lin.x[2] + idx, x2[it][0], x2[it][1]);
+                if (lin.scale) ld2(lin.scale + idx, sc[it][0], sc[it][1]);
             }
         }
 #pragma unroll
@@ -291,6 +295,10 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                 if (lin.out && (!ZS || z < nzr) &&
                     (!lin.wskip || (z + 1 >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     st2(lin.out + idx, v0, v1);
+                if (lin.scale) {
+                    v0 = cmul(v0, sc[it][0]);
+                    v1 = cmul(v1, sc[it][1]);
+                }
                 const bool dropall = (s == 0 && row >= rows - g.ny) || (s == 1 && (row % g.ny) == g.ny - 1);
                 if (dropall) v0 = v1 = czero<T>();
                 if (s == 2 && z + 1 == nzr - 1) v1 = czero<T>();
@@ -317,6 +325,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(Geo g, Lin<R> lin, cx<R>* __restric
                 if (lin.out && (!ZS || z < nzr) &&
                     (!lin.wskip || (z >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))))
                     lin.out[idx] = v;
+                if (lin.scale && z >= (int)(zrs[col] & 0xffffu) && z < (int)(zrs[col] >> 16))
+                    v = cmul(v, lin.scale[idx]);
                 const bool drop = (s == 2 && z == nzr - 1) || (s == 0 && row >= rows - g.ny) ||
                                   (s == 1 && (row % g.ny) == g.ny - 1);
                 if (drop) v = czero<T>();
@@ -582,8 +592,12 @@ __global__ __launch_bounds__(NT) void k_zinv(Geo g, const cx<R>* __restrict__ W1
         const bool on = dg.nomask || flags[idx] != 0;
         T yv = czero<T>();
         if (on) {  // d and x are read only where the face is active
-            const T dv = dg.conj ? cconj(dgd[idx]) : dgd[idx];
-            yv = cadd(out, cmul(dv, src[ia]));
+            if (dg.unit) {
+                yv = cadd(out, src[ia]);
+            } else {
+                const T dv = dg.conj ? cconj(dgd[idx]) : dgd[idx];
+                yv = cadd(out, cmul(dv, src[ia]));
+            }
         }
         if (on || !dg.skip) y[ia] = yv;
         const bool rd = on || !dots.masked;

@@ -70,38 +70,42 @@ __device__ __forceinline__ double2 zc(double re, double im) { return make_double
 // The scalar step runs as one workgroup of NTS threads.
 constexpr int NTS = 256;
 
-// out[d] = sum over blocks of part[d * nblk + block], d < nd (<= 4), in a fixed
+// out[d] = sum over blocks of part[d * nblk + block], d < ND (<= 4), in a fixed
 // order (strided per thread, xor-butterfly per wave, waves in index order).
-__device__ __forceinline__ void reduce_partials(const double2* part, long nblk, int nd, double2* out,
-                                                double2 (*red)[NTS / 64])
+// Each thread issues the loads of RU strided entries of every dot before it
+// adds any of them, so up to RU NTS partials per dot cost one memory latency
+// (the partial sums come from a kernel that has just ended, so they are read
+// from HBM or a remote cache, not from this XCD's L2); the additions keep the
+// strided order, so the sums do not depend on RU.  Offsets are 32-bit from a
+// per-dot uniform base (nblk < 2^27 in every launch).
+template <int ND>
+__device__ __forceinline__ void reduce_nd(const double2* part, int nblk, double2* out, double2 (*red)[NTS / 64])
 {
-    double2 a[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    long i = threadIdx.x;
-    for (; i + 3 * NTS < nblk; i += 4 * NTS) {
+    constexpr int RU = 8;
+    double2 a[ND];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            if (d >= nd) break;
-            double2 p[4];
+    for (int d = 0; d < ND; ++d) a[d] = make_double2(0, 0);
+    for (int i0 = threadIdx.x; i0 < nblk; i0 += RU * NTS) {
+        double2 p[ND][RU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) p[u] = part[d * nblk + i + u * NTS];
+        for (int d = 0; d < ND; ++d)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                a[d].x += p[u].x;
-                a[d].y += p[u].y;
+            for (int u = 0; u < RU; ++u) {
+                const int i = i0 + u * NTS;
+                p[d][u] = i < nblk ? part[(long)d * nblk + i] : make_double2(0, 0);
             }
-        }
-    }
-    for (; i < nblk; i += NTS) {
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            if (d >= nd) break;
-            a[d].x += part[d * nblk + i].x;
-            a[d].y += part[d * nblk + i].y;
-        }
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int u = 0; u < RU; ++u)
+                if (i0 + u * NTS < nblk) {
+                    a[d].x += p[d][u].x;
+                    a[d].y += p[d][u].y;
+                }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
+    for (int d = 0; d < ND; ++d) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             a[d].x += __shfl_xor(a[d].x, off);
@@ -110,7 +114,7 @@ __device__ __forceinline__ void reduce_partials(const double2* part, long nblk, 
         if (lane == 0) red[d][wid] = a[d];
     }
     __syncthreads();
-    if (threadIdx.x < nd) {
+    if (threadIdx.x < ND) {
         double2 t = make_double2(0, 0);
         for (int w = 0; w < NTS / 64; ++w) {
             t.x += red[threadIdx.x][w].x;
@@ -119,6 +123,16 @@ __device__ __forceinline__ void reduce_partials(const double2* part, long nblk, 
         out[threadIdx.x] = t;
     }
     __syncthreads();
+}
+__device__ __forceinline__ void reduce_partials(const double2* part, long nblk, int nd, double2* out,
+                                                double2 (*red)[NTS / 64])
+{
+    switch (nd) {
+    case 1: reduce_nd<1>(part, (int)nblk, out, red); break;
+    case 2: reduce_nd<2>(part, (int)nblk, out, red); break;
+    case 3: reduce_nd<3>(part, (int)nblk, out, red); break;
+    default: reduce_nd<4>(part, (int)nblk, out, red); break;
+    }
 }
 
 // Scalar programs, one per solver step; v[] are the reduced dot products.

@@ -481,18 +481,19 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
         hi = (int)(rg >> 16);
     }
     // all loads before any store (lin.out may alias an input)
-    T_ v[E], w1[E], w2[E];
+    T_ v[E], w1[E], w2[E], sc[E];
 #pragma unroll
     for (int b = 0; b < E / r1; ++b) {
         const int i = t + T * b;
 #pragma unroll
         for (int q = 0; q < r1; ++q) {
             const int n = i + q * (P / r1), e = b * r1 + q;
-            v[e] = w1[e] = w2[e] = czero<T_>();
+            v[e] = w1[e] = w2[e] = sc[e] = czero<T_>();
             if (n < N && rowok && n + zs >= lo && n + zs < hi) {
                 v[e] = lin.x[0][base + n];
                 if (lin.x[1]) w1[e] = lin.x[1][base + n];
                 if (lin.x[2]) w2[e] = lin.x[2][base + n];
+                if (lin.scale) sc[e] = lin.scale[base + n];
             }
         }
     }
@@ -505,6 +506,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_r(Geo g, Lin<R> lin, cx<R>* __restr
             T_ z = cadd(cadd(cmul(c[0], v[e]), cmul(c[1], w1[e])), cmul(c[2], w2[e]));
             if (n < N && rowok && n + zs < nzr) {
                 if (lin.out && (!lin.wskip || (n + zs >= lo && n + zs < hi))) lin.out[base + n] = z;
+                if (lin.scale) z = cmul(z, sc[e]);  // zero outside the row's range
                 if (dropall || (s == 2 && n + zs == nzr - 1)) z = czero<T_>();
             } else {
                 z = czero<T_>();
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                     const long ia = bb + n;
                     sv[e] = dv[e] = e1v[e] = e2v[e] = czero<T_>();
                     if (fl[e] || (!dots.masked && need1 && e1src)) sv[e] = src[ia];
-                    if (fl[e]) dv[e] = dgd[base + n];
+                    if (fl[e] && !dg.unit) dv[e] = dgd[base + n];
                     if (fl[e] || !dots.masked) {
                         if (need1 && !e1src) e1v[e] = dots.e1[ia];
                         if (need2) e2v[e] = dots.e2[ia];
@@ -615,7 +617,7 @@ __global__ __launch_bounds__(NT) void k_zinv_r(Geo g, const cx<R>* __restrict__ 
                 if (n < N && n + zs < nzr) {
                     const long ia = bb + n;
                     T_ yv = czero<T_>();
-                    if (fl[e]) yv = cadd(v[e], cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
+                    if (fl[e]) yv = cadd(v[e], dg.unit ? sv[e] : cmul(dg.conj ? cconj(dv[e]) : dv[e], sv[e]));
                     if (fl[e] || !dg.skip) y[ia] = yv;
                     dots_add(acc, dots, yv, e1src ? sv[e] : need1 ? e1v[e] : czero<T_>(),
                              need2 ? e2v[e] : czero<T_>());

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libviefft.so")
 FP64, FP32 = 0, 1
 MIXED_REFINE, MIXED_RELIABLE = 0, 1
 CG, BICG, BICGSTAB, CORS, BICORSTAB = 1, 2, 3, 4, 5
+PRECOND_NONE, PRECOND_JACOBI = 0, 1
 METHODS = {"cg": CG, "bicg": BICG, "bicgstab": BICGSTAB, "cors": CORS, "bicorstab": BICORSTAB}
 OK, ERR_ARG, ERR_DEVICE, ERR_OOM, NOT_CONVERGED, BREAKDOWN, ERR_NAN, ERR_UNSUPPORTED, DIVERGED = range(9)
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_OOM", 4: "NOT_CONVERGED", 5: "BREAKDOWN",
@@ -37,7 +38,7 @@ EXPORTS = [
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
     "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
     "vie_history", "vie_set_divergence_stop", "vie_tuning", "vie_voxelize", "vie_set_medium_shapes",
-    "vie_batch_outputs",
+    "vie_batch_outputs", "vie_set_preconditioner",
 ]
 
 
@@ -126,6 +127,7 @@ def load():
     L.vie_set_guess.argtypes = [vp, dp]
     L.vie_history.argtypes = [vp, ct.c_int, dp, ct.POINTER(ct.c_int)]
     L.vie_set_divergence_stop.argtypes = [vp, ct.c_double]
+    L.vie_set_preconditioner.argtypes = [vp, ct.c_int]
     L.vie_tuning.argtypes = [vp, ct.POINTER(ct.c_int), dp]
     L.vie_voxelize.argtypes = [vp, ct.c_int, ct.POINTER(Shape), ub, ub]
     L.vie_set_medium_shapes.argtypes = [vp, ct.c_int, ct.POINTER(Shape), dp, ct.c_int]
@@ -491,6 +493,11 @@ class Solver:
     def set_divergence_stop(self, factor):
         """Later solves end with DIVERGED once the stop quantity exceeds factor x its minimum (0: off)."""
         self._check(self.lib.vie_set_divergence_stop(self.h, float(factor)))
+
+    def set_preconditioner(self, kind):
+        """Later BiCGSTAB solves use the Jacobi preconditioner (PRECOND_JACOBI) or none
+        (PRECOND_NONE, the reference's iterates)."""
+        self._check(self.lib.vie_set_preconditioner(self.h, int(kind)))
 
     def history(self, max_n=65536):
         """Stop quantity after each iteration of the last single solve."""
