@@ -7,12 +7,15 @@ transforms, BiCGSTAB to relative residual 1e-6.  A "step" is one complete
 solve from x = 0 with the right-hand side, medium and Green's function
 already resident in HBM.  `value` is the mixed-precision solve rate
 (FP32-complex BiCGSTAB recurrences and operator, FP64 solution, residual
-replaced by the true FP64 residual every 10x reduction, FP64 stop rule); the
-plain FP64 solve is reported beside it.  The roofline entry covers the
+replaced by the true FP64 residual every 10x reduction, FP64 stop rule) of the
+reference's unpreconditioned BiCGSTAB; the plain FP64 solve is reported beside
+it, and the key `jacobi` repeats both with the Jacobi preconditioner (same
+true-residual stop rule, fewer iterations).  The roofline entry covers the
 dominant kernel of the mixed solve (the FP32 x-axis pass: forward transform,
 spectral Green's-function product, inverse transform, over the x window of
 the internal axis order on its pencil length), timed per launch with HIP
-events on the library's stream during the timed solves.
+events that its dispatch records on the library's stream, in one more mixed
+solve after the timed ones.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -339,11 +342,13 @@ def dimer_sweep_leg(tol):
             "max_true_relres": max(r["true_relres"] for r in res)}
 
 
-def spheres_256_leg():
-    """256^3 grid, 60 seeded random spheres (cell 1.5 nm, 550 nm): every axis
-    window is the whole grid, so the passes run at their full lengths.  One
-    untimed mixed solve, then 2 timed mixed solves and 1 timed FP64 solve to
-    1e-6; per-iteration times and the dense-model fraction of 8 TB/s."""
+def spheres_256_leg(tol=1e-8):
+    """256^3 grid, 60 seeded random spheres (cell 1.5 nm, 550 nm), mixed
+    precision to the configuration's true residual 1e-8: every axis window is
+    the whole grid, so the passes run at their full lengths.  One untimed
+    mixed solve, then 2 timed mixed solves, 1 timed FP64 solve and 1 timed
+    Jacobi-preconditioned mixed solve; per-iteration times, the dense-model
+    fraction of 8 TB/s and the mixed solution's error against the FP64 one."""
     from vie_fft_amd import geometry as G
     from vie_fft_amd import viefft as VF
 
@@ -352,18 +357,32 @@ def spheres_256_leg():
         s.set_medium(sc.material(), sc.eps_table, sc.flags())
         s.green_compute()
         s.set_rhs_incident(*sc.rhs_angles())
-        s.solve_mixed(tol=1e-6)
+        s.solve_mixed(tol=tol)
         t0 = time.perf_counter()
-        im = [s.solve_mixed(tol=1e-6) for _ in range(2)]
+        im = [s.solve_mixed(tol=tol) for _ in range(2)]
         tm = (time.perf_counter() - t0) / 2
+        xm = s.solution()
         t0 = time.perf_counter()
-        i64 = s.solve_resident(VF.BICGSTAB, tol=1e-6)
+        i64 = s.solve_resident(VF.BICGSTAB, tol=tol)
         t64 = time.perf_counter() - t0
+        x64 = s.solution()
+        s.set_preconditioner(VF.PRECOND_JACOBI)
+        t0 = time.perf_counter()
+        ij = s.solve_mixed(tol=tol)
+        tj = time.perf_counter() - t0
+        xj = s.solution()
+        s.set_preconditioner(VF.PRECOND_NONE)
         plan = s.plan()
         tuning = s.tuning()
     dm, _ = solve_models(plan, 8.0, im[-1]["iterations"], im[-1]["outer"])
     d6, _ = solve_models(plan, 16.0, i64["iterations"])
-    return {"mixed_s": tm, "mixed_iterations": im[-1]["iterations"], "mixed_replacements": im[-1]["outer"],
+    nrm = float(np.linalg.norm(x64))
+    return {"tol": tol, "mixed_s": tm, "mixed_iterations": im[-1]["iterations"],
+            "mixed_replacements": im[-1]["outer"],
+            "mixed_vs_fp64_rel_l2": float(np.linalg.norm(xm - x64)) / nrm,
+            "jacobi_mixed_s": tj, "jacobi_mixed_iterations": ij["iterations"],
+            "jacobi_mixed_true_relres": ij["true_relres"],
+            "jacobi_mixed_vs_fp64_rel_l2": float(np.linalg.norm(xj - x64)) / nrm,
             "mixed_us_per_iteration": tm * 1e6 / im[-1]["iterations"], "mixed_true_relres": im[-1]["true_relres"],
             "fp64_s": t64, "fp64_iterations": i64["iterations"], "fp64_us_per_iteration": t64 * 1e6 / i64["iterations"],
             "fp64_true_relres": i64["true_relres"], "mixed_dense_frac": dm / tm / HBM_PEAK,
