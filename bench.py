@@ -262,6 +262,7 @@ def main():
     if not args.no_extra:
         out["dimer_sweep_96cube"] = dimer_sweep_leg(args.tol)
         out["spheres_256cube"] = spheres_256_leg()
+        out.update(small_configs_leg(args.tol))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
     print(json.dumps(out))
@@ -310,6 +311,92 @@ def jacobi_leg(s, sc, args, sync, plan, x64, q64, t64_plain):
             "speedup_mixed_vs_fp64": t64 / tm,
             "speedup_mixed_vs_plain_fp64": t64_plain / tm,
             "solve_roofline": solve_roofline(plan, tm, itm, outer, t64, it64)}
+
+
+def _timed(fn, sync, reps):
+    """Mean seconds of reps calls of fn after one untimed call, and the last result."""
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    sync()
+    return (time.perf_counter() - t0) / reps, r
+
+
+def small_configs_leg(tol):
+    """The remaining configurations, each timed on the device with its data
+    resident (mean of 3 solves after one untimed):
+    * sphere_32cube: the CPU-runnable plain case, 32^3, sphere R = 10 cells,
+      eps -4+0.7i, 400 nm, BiCGSTAB to 1e-6 (FP64 and mixed), Q_ext / Q_abs;
+    * layered_128x128x64: the plain FP64 case, layered slab + 4x4 spheres,
+      550 nm, BiCGSTAB to 1e-6 (FP64 and mixed) and CORS with a divergence
+      stop (factor 1e3, at most 2000 iterations: CORS diverges on this medium,
+      as the oracle's CORS does);
+    * design_64cube: the adjoint-gradient case, 64^3 per-voxel eps, 600 nm:
+      forward FP64 solve to 1e-10, then the absorbed power and its gradient
+      with respect to all 262144 voxel permittivities (one adjoint solve to
+      1e-10 and the fused gradient kernel)."""
+    import torch
+
+    from vie_fft_amd import geometry as G
+    from vie_fft_amd import viefft as VF
+
+    sync = torch.cuda.synchronize
+    out = {}
+    sc = G.sphere_case(32, radius=10.0, eps_phys=-4.0 + 0.7j, delta_nm=2.0, wavelength_nm=400.0)
+    with VF.Solver(sc.n, sc.d, sc.k0, VF.FP64) as s:
+        s.set_medium(sc.material(), sc.eps_table, sc.flags())
+        s.green_compute()
+        s.set_rhs_incident(*sc.rhs_angles())
+        t64, i64 = _timed(lambda: s.solve_resident(VF.BICGSTAB, tol=tol), sync, 3)
+        q = cross_sections(s, sc)
+        tm, im = _timed(lambda: s.solve_mixed(tol=tol), sync, 3)
+        qm = cross_sections(s, sc)
+    out["sphere_32cube"] = {"fp64_ms": t64 * 1e3, "fp64_iterations": i64["iterations"],
+                            "fp64_true_relres": i64["true_relres"], "mixed_ms": tm * 1e3,
+                            "mixed_iterations": im["iterations"], "mixed_true_relres": im["true_relres"],
+                            "q_ext": q[0], "q_abs": q[1], "q_ext_mixed_rel": abs(qm[0] - q[0]) / abs(q[0]),
+                            "q_abs_mixed_rel": abs(qm[1] - q[1]) / abs(q[1])}
+    sc = G.layered_case()
+    with VF.Solver(sc.n, sc.d, sc.k0, VF.FP64) as s:
+        s.set_medium(sc.material(), sc.eps_table, sc.flags())
+        s.green_compute()
+        s.set_rhs_incident(*sc.rhs_angles())
+        t64, i64 = _timed(lambda: s.solve_resident(VF.BICGSTAB, tol=tol), sync, 3)
+        tm, im = _timed(lambda: s.solve_mixed(tol=tol), sync, 3)
+        s.set_divergence_stop(1e3)
+
+        def cors():
+            st = s.lib.vie_solve_resident(s.h, VF.CORS, tol, 2000, None)
+            return st, s.history()
+
+        tc, (cst, hist) = _timed(cors, sync, 1)
+        s.set_divergence_stop(0.0)
+    out["layered_128x128x64"] = {"fp64_solves_per_s": 1.0 / t64, "fp64_iterations": i64["iterations"],
+                                 "fp64_us_per_iteration": t64 * 1e6 / i64["iterations"],
+                                 "fp64_true_relres": i64["true_relres"], "mixed_solves_per_s": 1.0 / tm,
+                                 "mixed_iterations": im["iterations"], "mixed_true_relres": im["true_relres"],
+                                 "cors_status": VF.STATUS.get(cst, cst), "cors_iterations": len(hist),
+                                 "cors_min_relres": float(hist.min()) if len(hist) else None,
+                                 "cors_last_relres": float(hist[-1]) if len(hist) else None, "cors_ms": tc * 1e3}
+    sc = G.design_case()
+    eps, flags = sc.eps(), sc.flags()
+    with VF.Solver(sc.n, sc.d, sc.k0, VF.FP64) as s:
+        s.set_medium_eps(eps, flags)
+        s.green_compute()
+        s.set_rhs(s.incident(*sc.rhs_angles()) * flags)
+
+        def grad():
+            fwd = s.solve_resident(VF.BICGSTAB, tol=1e-10)
+            P, gre, gim, adj = s.absorption_grad(tol=1e-10)
+            return fwd, adj, P
+
+        tg, (fwd, adj, P) = _timed(grad, sync, 3)
+    out["design_64cube"] = {"forward_and_gradient_ms": tg * 1e3, "forward_iterations": fwd["iterations"],
+                            "adjoint_iterations": adj["iterations"], "absorbed_power": P,
+                            "voxels": int(np.prod(sc.n)), "tol": 1e-10}
+    return out
 
 
 def cross_sections(s, sc):
