@@ -259,6 +259,16 @@ int vie_set_divergence_stop(vie_ctx* ctx, double factor);
    active face whose d_f vanishes return VIE_ERR_ARG. */
 enum { VIE_PRECOND_NONE = 0, VIE_PRECOND_JACOBI = 1 };
 int vie_set_preconditioner(vie_ctx* ctx, int kind);
+
+/* ---- device memory ------------------------------------------------------ */
+/* The library keeps the device buffers of 64 MB and more that a context frees
+   (up to a third of the device's memory) and reuses them for later requests
+   of the same size, so contexts and batches created one after another (sweeps)
+   allocate nothing; kept buffers still count as used for hipMemGetInfo.  This
+   call (no counterpart in the reference) hands the current device's kept
+   buffers back to the HIP runtime, for callers that need the memory for other
+   allocations; *released (may be NULL) receives the byte count. */
+int vie_release_buffers(size_t* released);
 /* Warm start (the reference always starts from x = 0, vie_fft/Model.h:189-191):
    the next vie_solve_resident / vie_solve_mixed_resident starts from x0
    (host, 3 V complex, masked by the flags like the solution), or from the

@@ -56,17 +56,137 @@ struct Err : std::runtime_error {
                       std::string(#x) + ": " + hipGetErrorString(e_));                        \
     } while (0)
 
-template <class T> T* dalloc(size_t n)
+// ---- large device buffers are kept for reuse
+// A freed buffer of at least kPoolMin bytes is kept (per device, up to a third
+// of the device's memory) and handed out again for a request of the same
+// rounded size, instead of going back to the HIP runtime: contexts created one
+// after another with the same sizes (sweeps, batches of tens of GB) then
+// allocate nothing.  The runtime's own release of such frees is deferred and
+// has blocked a later hipMalloc for seconds after a few tens of GB of churn.
+// A kept buffer holds the contents it had (as memory the runtime recycles
+// does).  Freeing synchronises the device first, as hipFree does, so no
+// kernel still using the buffer can overlap its next owner's work.
+constexpr size_t kPoolMin = size_t(64) << 20;
+constexpr size_t kPoolAlign = size_t(2) << 20;
+struct PoolBuf {
+    int dev;
+    size_t bytes;
+    void* p;
+};
+static std::mutex g_pool_mu;
+static std::vector<PoolBuf> g_pool;          // kept buffers
+static std::map<void*, PoolBuf> g_pool_live;  // handed-out buffers of pooled size
+static std::map<int, size_t> g_pool_kept;     // kept bytes per device
+
+static size_t pool_cap(int dev)
+{
+    static std::map<int, size_t> cap;
+    auto it = cap.find(dev);
+    if (it != cap.end()) return it->second;
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    return cap[dev] = tot / 3;
+}
+
+// bytes of the current device's kept buffers (memory that a new buffer can
+// take although hipMemGetInfo counts it as used)
+size_t pool_kept_bytes()
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    return g_pool_kept[dev];
+}
+
+static void* raw_alloc(size_t bytes)
 {
     void* p = nullptr;
+    if (bytes < kPoolMin) {
+        if (hipMalloc(&p, bytes) != hipSuccess) p = nullptr;
+        return p;
+    }
+    bytes = (bytes + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (size_t i = 0; i < g_pool.size(); ++i)
+        if (g_pool[i].dev == dev && g_pool[i].bytes == bytes) {
+            const PoolBuf b = g_pool[i];
+            g_pool.erase(g_pool.begin() + (long)i);
+            g_pool_kept[dev] -= bytes;
+            g_pool_live[b.p] = b;
+            return b.p;
+        }
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        // out of memory: give this device's kept buffers back and try again
+        (void)hipGetLastError();
+        for (size_t i = 0; i < g_pool.size();)
+            if (g_pool[i].dev == dev) {
+                (void)hipFree(g_pool[i].p);
+                g_pool.erase(g_pool.begin() + (long)i);
+            } else {
+                ++i;
+            }
+        g_pool_kept[dev] = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    }
+    g_pool_live[p] = PoolBuf{dev, bytes, p};
+    return p;
+}
+
+static void raw_free(void* p)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pool_live.find(p);
+        if (it != g_pool_live.end()) {
+            const PoolBuf b = it->second;
+            g_pool_live.erase(it);
+            if (g_pool_kept[b.dev] + b.bytes <= pool_cap(b.dev)) {
+                int cur = 0;
+                (void)hipGetDevice(&cur);
+                if (cur != b.dev) (void)hipSetDevice(b.dev);
+                (void)hipDeviceSynchronize();
+                if (cur != b.dev) (void)hipSetDevice(cur);
+                g_pool.push_back(b);
+                g_pool_kept[b.dev] += b.bytes;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
+// hipFree every kept buffer of the current device; returns the bytes released
+static size_t pool_release()
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    size_t rel = 0;
+    for (size_t i = 0; i < g_pool.size();)
+        if (g_pool[i].dev == dev) {
+            (void)hipFree(g_pool[i].p);
+            rel += g_pool[i].bytes;
+            g_pool.erase(g_pool.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    g_pool_kept[dev] = 0;
+    return rel;
+}
+
+template <class T> T* dalloc(size_t n)
+{
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
-    if (e != hipSuccess) throw Err(VIE_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    void* p = raw_alloc(n * sizeof(T));
+    if (!p) throw Err(VIE_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
     return (T*)p;
 }
 template <class T> void dfree(T*& p)
 {
-    if (p) (void)hipFree(p);
+    raw_free(p);
     p = nullptr;
 }
 
@@ -2691,6 +2811,7 @@ void batch_reserve(vie_ctx* c, int nb)
     if (nb <= c->nbcap) return;
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
+    fr += pool_kept_bytes();  // kept buffers are available to this batch
     const double need = (nb - c->nbcap) * batch_bytes_per_instance(c);
     if (need > 0.9 * (double)fr)
         throw Err(VIE_ERR_OOM, "a batch of " + std::to_string(nb) + " instances needs about " +
@@ -2707,8 +2828,8 @@ void batch_reserve(vie_ctx* c, int nb)
     dfree(c->ilist);
     dfree(c->btol);
     dfree(c->bmaxit);
-    for (void*& v : c->vec64) if (v) { (void)hipFree(v); v = nullptr; }
-    for (void*& v : c->vec32) if (v) { (void)hipFree(v); v = nullptr; }
+    for (void*& v : c->vec64) if (v) { raw_free(v); v = nullptr; }
+    for (void*& v : c->vec32) if (v) { raw_free(v); v = nullptr; }
     if (c->snap) {
         (void)hipHostFree(c->snap);
         c->snap = nullptr;
@@ -3060,8 +3181,8 @@ int vie_destroy(vie_ctx* c)
     if (c->snap) (void)hipHostFree(c->snap);
     for (hipEvent_t e : c->snap_ev)
         if (e) (void)hipEventDestroy(e);
-    for (void* p : c->vec64) if (p) (void)hipFree(p);
-    for (void* p : c->vec32) if (p) (void)hipFree(p);
+    for (void* p : c->vec64) if (p) raw_free(p);
+    for (void* p : c->vec32) if (p) raw_free(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VIE_OK;
@@ -3852,6 +3973,7 @@ int vie_batch_capacity(vie_ctx* c, int* nb_max)
     try {
         size_t fr = 0, tot = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
+        fr += pool_kept_bytes();
         const double more = 0.8 * (double)fr / batch_bytes_per_instance(c);
         *nb_max = (int)std::min<double>(c->nbcap + std::max(0.0, more), 1 << 20);
         return VIE_OK;
@@ -4398,6 +4520,13 @@ int vie_tuning(const vie_ctx* c, int* out, double* us)
     };
     put(c->op64, out, us, us ? us + 2 : nullptr);
     put(c->op32, out + 5, us ? us + 1 : nullptr, us ? us + 3 : nullptr);
+    return VIE_OK;
+}
+
+int vie_release_buffers(size_t* released)
+{
+    const size_t r = pool_release();
+    if (released) *released = r;
     return VIE_OK;
 }
 

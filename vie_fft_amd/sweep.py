@@ -92,6 +92,17 @@ def merge_point_files(out_dir, count):
                 os.remove(p)
 
 
+def scene_voxels(s, sc):
+    """(material, flags) of a scene: its analytic shapes voxelised on the
+    device (vie_voxelize: byte-equal to geometry.py's rules, a few
+    milliseconds at 96^3), or the host rules for per-voxel media."""
+    from . import geometry as G
+
+    if sc.eps_voxel is None and all(isinstance(sh, (G.Sphere, G.Band)) for sh in sc.shapes):
+        return s.voxelize(sc)
+    return sc.material(), sc.flags()
+
+
 def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=1, active=None, maxit=0,
               out_dir=None, log=None, warm=False):
     """Solve every (wavelength_nm, psi) of `points` on one resident context.
@@ -128,7 +139,7 @@ def run_sweep(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB, metal=
             sc = scene_at(lam, psi)
             if s is None:
                 s = VF.Solver(sc.n, sc.d, sc.k0, VF.FP64)
-                material, flags = sc.material(), sc.flags()
+                material, flags = scene_voxels(s, sc)
                 vol_active = active_face_volume(material, active, sc.d) if active is not None else 0.0
             if cur != lam:
                 t0 = time.perf_counter()
@@ -192,7 +203,7 @@ def run_sweep_batched(scene_at, points, tol=1e-6, mixed=True, method=VF.BICGSTAB
     wl_index = {}
     t_setup = t_solve = t_post = 0.0
     with VF.Solver(sc0.n, sc0.d, sc0.k0, VF.FP64) as s:
-        material, flags = sc0.material(), sc0.flags()
+        material, flags = scene_voxels(s, sc0)
         vol_active = active_face_volume(material, active, sc0.d) if active is not None else 0.0
         t0 = time.perf_counter()
         s.set_medium(material, sc0.eps_table, flags)

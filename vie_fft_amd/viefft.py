@@ -38,7 +38,7 @@ EXPORTS = [
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
     "vie_batch_absorbed", "vie_far_field_noalloc", "vie_apply_raw", "vie_set_guess", "vie_solve_x0",
     "vie_history", "vie_set_divergence_stop", "vie_tuning", "vie_voxelize", "vie_set_medium_shapes",
-    "vie_batch_outputs", "vie_set_preconditioner",
+    "vie_batch_outputs", "vie_set_preconditioner", "vie_release_buffers",
 ]
 
 
@@ -128,6 +128,7 @@ def load():
     L.vie_history.argtypes = [vp, ct.c_int, dp, ct.POINTER(ct.c_int)]
     L.vie_set_divergence_stop.argtypes = [vp, ct.c_double]
     L.vie_set_preconditioner.argtypes = [vp, ct.c_int]
+    L.vie_release_buffers.argtypes = [ct.POINTER(ct.c_size_t)]
     L.vie_tuning.argtypes = [vp, ct.POINTER(ct.c_int), dp]
     L.vie_voxelize.argtypes = [vp, ct.c_int, ct.POINTER(Shape), ub, ub]
     L.vie_set_medium_shapes.argtypes = [vp, ct.c_int, ct.POINTER(Shape), dp, ct.c_int]
@@ -171,6 +172,16 @@ def _dp(a):
 
 def _ub(a):
     return a.ctypes.data_as(ct.POINTER(ct.c_ubyte))
+
+
+def release_buffers():
+    """Hand the device buffers the library keeps for reuse back to the HIP
+    runtime (vie_release_buffers); returns the bytes released."""
+    n = ct.c_size_t()
+    st = load().vie_release_buffers(ct.byref(n))
+    if st != OK:
+        raise VieError(st, "vie_release_buffers failed")
+    return n.value
 
 
 class Solver:
