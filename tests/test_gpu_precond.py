@@ -13,7 +13,8 @@ the true residual ||b - A x|| / ||b||.
 * Solutions: on a layered medium like the metric's (five materials, face
   diagonals 15x apart) the preconditioned solve reaches the same solution as
   the plain one in fewer iterations, in FP64 and in the mixed solve, also
-  after the permittivities change (the preconditioner is rebuilt).
+  after the permittivities change (the preconditioner is rebuilt), and from a
+  warm start (the correction solve is preconditioned too).
 * Everything it does not cover returns VIE_ERR_ARG.
 """
 import numpy as np
@@ -115,6 +116,27 @@ def test_preconditioned_mixed_solve_matches_fp64():
     assert rel(xm, x64) <= 1e-5, rel(xm, x64)
     assert abs(qm - q64) <= 1e-6 * abs(q64)
     assert pre["iterations"] < 0.8 * plain["iterations"], (pre["iterations"], plain["iterations"])
+
+
+def test_preconditioned_warm_start():
+    # a guess runs as the correction solve A e = b - A w, itself preconditioned
+    sc = _layered()
+    with scene_solver(sc) as s:
+        b = s.incident()
+        s.set_preconditioner(VF.PRECOND_JACOBI)
+        x_ref, cold = s.solve(b, VF.BICGSTAB, tol=1e-10, maxit=5000)
+        guess, _ = s.solve(b, VF.BICGSTAB, tol=1e-3, maxit=5000)
+        x, warm = s.solve_x0(b, guess, VF.BICGSTAB, tol=1e-10, maxit=5000)
+        assert cold["converged"] and warm["converged"] and warm["true_relres"] <= 2e-10
+        assert rel(x, x_ref) <= 1e-7, rel(x, x_ref)
+        assert warm["iterations"] < cold["iterations"], (warm["iterations"], cold["iterations"])
+        # the mixed solve from the resident solution as its guess
+        s.set_rhs(b)
+        s.solve_resident(VF.BICGSTAB, tol=1e-3, maxit=5000)
+        s.set_guess()
+        mixed = s.solve_mixed(tol=1e-8)
+        assert mixed["converged"] and mixed["true_relres"] < 1e-8
+        assert rel(s.solution(), x_ref) <= 1e-6, rel(s.solution(), x_ref)
 
 
 def _status(fn):

@@ -78,6 +78,7 @@ static std::vector<PoolBuf> g_pool;          // kept buffers
 static std::map<void*, PoolBuf> g_pool_live;  // handed-out buffers of pooled size
 static std::map<int, size_t> g_pool_kept;     // kept bytes per device
 
+// a third of device dev's memory; dev must be the current device
 static size_t pool_cap(int dev)
 {
     static std::map<int, size_t> cap;
@@ -143,12 +144,13 @@ static void raw_free(void* p)
         if (it != g_pool_live.end()) {
             const PoolBuf b = it->second;
             g_pool_live.erase(it);
-            if (g_pool_kept[b.dev] + b.bytes <= pool_cap(b.dev)) {
-                int cur = 0;
-                (void)hipGetDevice(&cur);
-                if (cur != b.dev) (void)hipSetDevice(b.dev);
-                (void)hipDeviceSynchronize();
-                if (cur != b.dev) (void)hipSetDevice(cur);
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            if (cur != b.dev) (void)hipSetDevice(b.dev);
+            const bool keep = g_pool_kept[b.dev] + b.bytes <= pool_cap(b.dev);
+            if (keep) (void)hipDeviceSynchronize();
+            if (cur != b.dev) (void)hipSetDevice(cur);
+            if (keep) {
                 g_pool.push_back(b);
                 g_pool_kept[b.dev] += b.bytes;
                 return;
