@@ -39,3 +39,24 @@ def test_create_without_device_fails_cleanly():
         pytest.skip("a GPU is present")
     with pytest.raises(VF.VieError):
         VF.Solver((8, 8, 8), (0.02,) * 3, 1.0)
+
+
+NAN = float("nan")
+
+
+@pytest.mark.parametrize("n,d,k0,precision,status", [
+    ((257, 8, 8), (0.02,) * 3, 1.0, VF.FP64, VF.ERR_UNSUPPORTED),   # beyond the largest compiled length
+    ((8, 8, 1000), (0.02,) * 3, 1.0, VF.FP32, VF.ERR_UNSUPPORTED),
+    ((0, 8, 8), (0.02,) * 3, 1.0, VF.FP64, VF.ERR_ARG),             # empty grid
+    ((8, 8, 8), (0.02, 0.0, 0.02), 1.0, VF.FP64, VF.ERR_ARG),       # zero cell
+    ((8, 8, 8), (0.02, NAN, 0.02), 1.0, VF.FP64, VF.ERR_ARG),
+    ((8, 8, 8), (0.02,) * 3, 0.0, VF.FP64, VF.ERR_ARG),             # zero wavenumber
+    ((8, 8, 8), (0.02,) * 3, float("inf"), VF.FP64, VF.ERR_ARG),
+    ((8, 8, 8), (0.02,) * 3, 1.0, 7, VF.ERR_ARG),                   # unknown precision
+])
+def test_create_rejects_bad_descriptors_before_any_device_call(n, d, k0, precision, status):
+    # vie_create checks its descriptor before it touches a device, so these
+    # statuses come back the same with or without a GPU
+    with pytest.raises(VF.VieError) as e:
+        VF.Solver(n, d, k0, precision)
+    assert e.value.status == status
