@@ -512,14 +512,28 @@ template <class R> struct Op {
     // size da for wavenumber kk
     void k0_tables(int N, double da, double kk, T* v, T* ua) const
     {
-        const double NP = 8.0 * (double)g.V;
-        const std::complex<double> J(0, 1);
+        std::vector<std::complex<double>> w(2 * N);
+        k0_phase(N, w.data());
+        k0_scale(N, da, kk, w.data(), v, ua);
+    }
+    // its wavenumber-free factor w_t = 1 - e^{-j th}, th = pi k / N for split
+    // index t (frequency k = 2 (t mod N) + t / N), from long-double sines --
+    // computed once per axis for a whole batch of wavenumbers
+    static void k0_phase(int N, std::complex<double>* w)
+    {
         for (int t = 0; t < 2 * N; ++t) {
             const int k = 2 * (t % N) + t / N;
             const long double th = (long double)kPi * k / N;
             const double s = (double)sinl(th), h = (double)sinl(th / 2);
-            const std::complex<double> w(2.0 * h * h, s);  // 1 - e^{-j th}
-            const std::complex<double> vv = J * kZ0 / (kk * da * NP) * w;
+            w[t] = std::complex<double>(2.0 * h * h, s);
+        }
+    }
+    void k0_scale(int N, double da, double kk, const std::complex<double>* w, T* v, T* ua) const
+    {
+        const double NP = 8.0 * (double)g.V;
+        const std::complex<double> J(0, 1);
+        for (int t = 0; t < 2 * N; ++t) {
+            const std::complex<double> vv = J * kZ0 / (kk * da * NP) * w[t];
             v[t] = hc<R>(vv);
             ua[t] = hc<R>(std::conj(vv));
         }
@@ -586,8 +600,10 @@ template <class R> struct Op {
     void batch_k0(int i0, int cnt, const double* ks, hipStream_t st)
     {
         std::vector<T> v(2L * lb * cnt), ua(2L * lb * cnt), df(3L * cnt), da(3L * cnt);
+        std::vector<std::complex<double>> w(2L * lb);
         for (int a = 0; a < 3; ++a) {
-            for (int i = 0; i < cnt; ++i) k0_tables(bl[a], d[a], ks[i], &v[2L * lb * i], &ua[2L * lb * i]);
+            k0_phase(bl[a], w.data());
+            for (int i = 0; i < cnt; ++i) k0_scale(bl[a], d[a], ks[i], w.data(), &v[2L * lb * i], &ua[2L * lb * i]);
             HIPCHK(hipMemcpyAsync(Vb[a] + 2L * lb * i0, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(Uab[a] + 2L * lb * i0, ua.data(), ua.size() * sizeof(T), hipMemcpyHostToDevice, st));
             HIPCHK(hipStreamSynchronize(st));  // v and ua are refilled for the next axis
@@ -2931,7 +2947,7 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
     double2* w1 = dalloc<double2>(cnt * tot);
     double2* w2 = nullptr;
     double* kd = nullptr;
-    double2* ev = nullptr;
+    double* cd_im = nullptr;
     double* cd_ = nullptr;
     try {
         w2 = dalloc<double2>(cnt * tot);
@@ -2969,7 +2985,6 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
             require(c, n_mat >= 1 && n_mat <= 256, "n_mat must be 1..256");
             for (long v = 0; v < 2L * cnt * n_mat; ++v)
                 if (!std::isfinite(eps_tables[v])) throw Err(VIE_ERR_NAN, "non-finite permittivity");
-            ev = dalloc<double2>(c->V);
             if (c->btabd_cap < c->nbcap) {
                 dfree(c->btabd);
                 c->btabd = nullptr;
@@ -2984,12 +2999,16 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
                     t[256L * i + m] =
                         make_double2(eps_tables[2L * (i * n_mat + m)], eps_tables[2L * (i * n_mat + m) + 1]);
             copy_sync(tab_d, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
-            for (int i = 0; i < cnt; ++i) {
-                hipLaunchKernelGGL(k_eps_table, dim3(grid1(c->V)), dim3(NT), 0, c->stream, c->V, c->mat_dev,
-                                   tab_d + 256L * i, ev);
-                HIPCHK(hipGetLastError());
-                batch_diag(c, o64, o32, i0 + i, k0s[i], ev);
-            }
+            // every instance's d_f (both precisions) in one launch, c_f as batch_diag forms it
+            std::vector<double> cims(3L * cnt);
+            for (int i = 0; i < cnt; ++i)
+                for (int f = 0; f < 3; ++f) cims[3L * i + f] = -(d[f] / 2.0) * kZ0 / k0s[i];
+            cd_im = dalloc<double>(cims.size());
+            copy_sync(cd_im, cims.data(), cims.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+            hipLaunchKernelGGL(k_diag_batch, dim3(grid1(c->V), std::min(cnt, 65535)), dim3(NT), 0, c->stream, cnt,
+                               n[0], n[1], n[2], cd_im, c->mat_dev, tab_d, o64->dgb + (long)i0 * 3 * c->V,
+                               o32->dgb + (long)i0 * 3 * c->V, 3 * c->V);
+            HIPCHK(hipGetLastError());
         } else {
             for (int i = 0; i < cnt; ++i) batch_diag(c, o64, o32, i0 + i, k0s[i], c->eps_dev);
         }
@@ -3005,14 +3024,14 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
         dfree(w1);
         dfree(w2);
         dfree(kd);
-        dfree(ev);
+        dfree(cd_im);
         dfree(cd_);
         throw;
     }
     dfree(w1);
     dfree(w2);
     dfree(kd);
-    dfree(ev);
+    dfree(cd_im);
     dfree(cd_);
     for (int i = 0; i < cnt; ++i) {
         c->bk0[i0 + i] = k0s[i];
@@ -4008,8 +4027,8 @@ int vie_batch_set_rhs_incident(vie_ctx* c, int i, double theta, double phi, doub
         require(c, c->medium_ready, "vie_set_medium has not been called");
         require(c, i < c->nbcap, "instance beyond the reserved batch (vie_batch_reserve)");
         require(c, !c->bmulti || c->bop[i], "instance without an operator (vie_batch_set_operators)");
+        // stream-ordered before every later use; nothing on the host to wait for
         rhs_incident(c, batch_k0(c, i), theta, phi, psi, c->b64 + 3 * c->V * i);
-        HIPCHK(hipStreamSynchronize(c->stream));
         c->brhs[i] = 1;
         if (i == 0) c->rhs_ready = true;
         c->sol_ready = false;

@@ -827,6 +827,41 @@ __global__ __launch_bounds__(NT) void k_diag(int n0, int n1, int n2, double c0, 
     }
 }
 
+// k_diag for the instances of a batch in one launch (instance blockIdx.y +
+// k gridDim.y), both precisions: instance i's permittivity is read through
+// the material ids from its table tabs[256 i ..], its c_f is j cims[3 i + f],
+// and its d_f goes to d64 / d32 + i istr.  Same arithmetic as k_diag.
+__global__ __launch_bounds__(NT) void k_diag_batch(int cnt, int n0, int n1, int n2, const double* __restrict__ cims,
+                                                   const uint8_t* __restrict__ mat, const double2* __restrict__ tabs,
+                                                   double2* __restrict__ d64, float2* __restrict__ d32, long istr)
+{
+    const long V = (long)n0 * n1 * n2;
+    const long st[3] = {(long)n1 * n2, n2, 1};
+    const int nn[3] = {n0, n1, n2};
+    for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
+        const double2* tab = tabs + 256L * i;
+        const double cim[3] = {cims[3 * i], cims[3 * i + 1], cims[3 * i + 2]};
+        double2* o64 = d64 + (long)i * istr;
+        float2* o32 = d32 + (long)i * istr;
+        for (long v = (long)blockIdx.x * NT + threadIdx.x; v < V; v += (long)gridDim.x * NT) {
+            const int c[3] = {(int)(v / ((long)n1 * n2)), (int)((v / n2) % n1), (int)(v % n2)};
+            const double2 ev = tab[mat[v]];
+            const double2 a = zdiv(make_double2(1, 0), make_double2(ev.x - 1.0, ev.y));
+            for (int f = 0; f < 3; ++f) {
+                double2 r = make_double2(0, 0);
+                if (c[f] < nn[f] - 1) {
+                    const double2 e = tab[mat[v + st[f]]];
+                    const double2 b = zdiv(make_double2(1, 0), make_double2(e.x - 1.0, e.y));
+                    const double2 sm = make_double2(a.x + b.x, a.y + b.y);
+                    r = make_double2(-cim[f] * sm.y, cim[f] * sm.x);  // j cim * sm
+                }
+                o64[f * V + v] = r;
+                o32[f * V + v] = make_float2((float)r.x, (float)r.y);
+            }
+        }
+    }
+}
+
 // Incident_Wave (vie_fft/GreenFunction.h:144-248) masked by the flags
 // (vie_fft/CG_Iteration.h:559-562): b_c = Flag_c exp(-j k.r_c) seg_c, r_c the
 // face centre of component c, seg_c its pulse integral times the polarisation.
