@@ -373,7 +373,8 @@ def small_configs_leg(tol):
 
         tc, (cst, hist) = _timed(cors, sync, 1)
         s.set_divergence_stop(0.0)
-    out["layered_128x128x64"] = {"fp64_solves_per_s": 1.0 / t64, "fp64_iterations": i64["iterations"],
+        tuning = s.tuning()
+    out["layered_128x128x64"] = {"tuning": tuning, "fp64_solves_per_s": 1.0 / t64, "fp64_iterations": i64["iterations"],
                                  "fp64_us_per_iteration": t64 * 1e6 / i64["iterations"],
                                  "fp64_true_relres": i64["true_relres"], "mixed_solves_per_s": 1.0 / tm,
                                  "mixed_iterations": im["iterations"], "mixed_true_relres": im["true_relres"],
