@@ -253,19 +253,34 @@ def main():
         out["roofline"]["valu"] = {"insts_per_launch": XPASS_F32_128_VALU, "issue_peak_per_s": VALU_ISSUE_PEAK,
                                    "frac": XPASS_F32_128_VALU / VALU_ISSUE_PEAK / (ms13 * 1e-3)}
     if not args.no_extra:
-        out["jacobi"] = jacobi_leg(s, sc, args, sync, plan, x64, q64, t64)
+        _leg(out, "jacobi", lambda: jacobi_leg(s, sc, args, sync, plan, x64, q64, t64))
     # measured practical HBM rate (device copy) beside the published peak
     _, _, copy_gbps = copy_rate(torch, n[0])
     out["roofline"]["copy_GBps"] = copy_gbps
     out["roofline"]["frac_of_copy"] = achieved / 1e9 / copy_gbps
     s.close()
     if not args.no_extra:
-        out["dimer_sweep_96cube"] = dimer_sweep_leg(args.tol)
-        out["spheres_256cube"] = spheres_256_leg()
-        out.update(small_configs_leg(args.tol))
+        _leg(out, "dimer_sweep_96cube", lambda: dimer_sweep_leg(args.tol))
+        _leg(out, "spheres_256cube", spheres_256_leg)
+        _leg(out, "small_configs", lambda: small_configs_leg(args.tol), merge=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, flags, it64)
+        _leg(out, "cpu_baseline", lambda: cpu_baseline(sc, flags, it64))
     print(json.dumps(out))
+
+
+def _leg(out, key, fn, merge=False):
+    """One more configuration beside the metric: its figures under `key` (or
+    merged, for a leg that returns several keys), or the error that stopped it,
+    so that the metric's line is printed whatever happens to the others."""
+    try:
+        r = fn()
+    except Exception as e:  # noqa: BLE001 -- reported in the line, not hidden
+        out[key] = {"error": f"{type(e).__name__}: {e}"}
+        return
+    if merge:
+        out.update(r)
+    else:
+        out[key] = r
 
 
 def jacobi_leg(s, sc, args, sync, plan, x64, q64, t64_plain):
