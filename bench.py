@@ -351,7 +351,9 @@ def small_configs_leg(tol):
     * design_64cube: the adjoint-gradient case, 64^3 per-voxel eps, 600 nm:
       forward FP64 solve to 1e-10, then the absorbed power and its gradient
       with respect to all 262144 voxel permittivities (one adjoint solve to
-      1e-10 and the fused gradient kernel)."""
+      1e-10 and the fused gradient kernel); and the same with both solves
+      mixed precision (vie_solve_mixed_resident, vie_absorption_grad_mixed),
+      its gradient against the FP64 one."""
     import torch
 
     from vie_fft_amd import geometry as G
@@ -406,12 +408,24 @@ def small_configs_leg(tol):
         def grad():
             fwd = s.solve_resident(VF.BICGSTAB, tol=1e-10)
             P, gre, gim, adj = s.absorption_grad(tol=1e-10)
-            return fwd, adj, P
+            return fwd, adj, P, gre
 
-        tg, (fwd, adj, P) = _timed(grad, sync, 3)
+        def grad_mixed():
+            fwd = s.solve_mixed(tol=1e-10)
+            P, gre, gim, adj = s.absorption_grad(tol=1e-10, mixed=True)
+            return fwd, adj, P, gre
+
+        tg, (fwd, adj, P, g64) = _timed(grad, sync, 3)
+        tm, (fwdm, adjm, Pm, gm) = _timed(grad_mixed, sync, 3)
     out["design_64cube"] = {"forward_and_gradient_ms": tg * 1e3, "forward_iterations": fwd["iterations"],
                             "adjoint_iterations": adj["iterations"], "absorbed_power": P,
-                            "voxels": int(np.prod(sc.n)), "tol": 1e-10}
+                            "voxels": int(np.prod(sc.n)), "tol": 1e-10,
+                            "mixed_forward_and_gradient_ms": tm * 1e3,
+                            "mixed_forward_iterations": fwdm["iterations"],
+                            "mixed_adjoint_iterations": adjm["iterations"],
+                            "mixed_adjoint_true_relres": adjm["true_relres"],
+                            "mixed_gradient_rel_l2": float(np.linalg.norm(gm - g64) / np.linalg.norm(g64)),
+                            "mixed_absorbed_power_rel": abs(Pm - P) / P}
     return out
 
 

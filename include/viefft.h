@@ -344,6 +344,14 @@ int vie_absorption(vie_ctx* ctx, double* value);
    for the current medium (vie_solve_resident or vie_solve before). */
 int vie_absorption_grad(vie_ctx* ctx, int method, double tol, int maxit, double* value, double* grad_re,
                         double* grad_im, vie_info* info);
+/* vie_absorption_grad with the adjoint system solved by the mixed-precision
+   BiCGSTAB of vie_solve_mixed_resident (FP32 operator and recurrences, FP64
+   solution, residual replacements and stop rule ||g - A nu|| / ||g|| < tol;
+   FP64 contexts).  info->outer = residual replacements, info->true_relres and
+   info->err = the adjoint solve's true FP64 residual.  No reference
+   counterpart (the design-gradient workload). */
+int vie_absorption_grad_mixed(vie_ctx* ctx, double tol, int maxit, double* value, double* grad_re,
+                              double* grad_im, vie_info* info);
 
 /* ---- instrumentation ----------------------------------------------------- */
 /* The last single solve's stop quantity after each iteration (the values the

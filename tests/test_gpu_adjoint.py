@@ -1,7 +1,8 @@
 """GPU absorbed power and its adjoint permittivity gradient (the design-gradient
 workload: 64^3 grid, 5 nm cells, 600 nm, every cell a design variable with
 eps = U[1, 12] - 0.1j) against the oracle and against central finite
-differences of the objective through full re-solves."""
+differences of the objective through full re-solves; the mixed-precision
+adjoint (vie_absorption_grad_mixed) against the FP64 one."""
 import numpy as np
 import pytest
 
@@ -88,6 +89,45 @@ def test_gradient_matches_finite_differences_design64():
                 vals.append(s.absorption())
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - gre[v]) <= 1e-3 * abs(gre[v]) + 1e-6 * gmax, (v, fd, gre[v])
+
+
+@pytest.mark.parametrize("case", ["design16", "noncubic"])
+def test_mixed_adjoint_equals_fp64_adjoint(case):
+    """vie_absorption_grad_mixed: the adjoint system by the mixed-precision
+    BiCGSTAB (FP32 iterations, FP64 residual replacements) to the same true
+    residual gives the FP64 gradient."""
+    if case == "design16":
+        sc = G.design_case(n=16)
+        n, d, k0, eps, flags = sc.n, sc.d, sc.k0, sc.eps(), sc.flags()
+    else:
+        n, d, k0, eps, flags = _noncubic_case()
+    with VF.Solver(n, d, k0) as s:
+        s.set_medium_eps(eps, flags)
+        s.green_compute()
+        s.set_rhs(s.incident() * flags)
+        s.solve_resident(VF.BICGSTAB, tol=1e-13)
+        J = s.solution()
+        P, gre, gim, a64 = s.absorption_grad(tol=1e-12)
+        Pm, grm, gim_m, am = s.absorption_grad(tol=1e-11, mixed=True)
+        assert np.array_equal(s.solution(), J)  # the forward solution stays resident
+    assert a64["converged"] and am["converged"] and am["true_relres"] < 1e-11 and am["outer"] >= 1
+    assert Pm == P
+    assert rel(grm, gre) <= 1e-8 and rel(gim_m, gim) <= 1e-8
+
+
+def test_mixed_forward_and_adjoint_design64():
+    """The design workload with both solves mixed (to 1e-10): the FP64 gradient within 1e-7."""
+    sc = G.design_case(n=64)
+    eps, flags = sc.eps(), sc.flags()
+    s, b = _solver(sc, eps, flags)
+    with s:
+        s.solve_resident(VF.BICGSTAB, tol=1e-10)
+        P, gre, gim, _ = s.absorption_grad(tol=1e-10)
+        fwd = s.solve_mixed(tol=1e-10)
+        Pm, grm, gim_m, am = s.absorption_grad(tol=1e-10, mixed=True)
+    assert fwd["converged"] and am["converged"] and am["true_relres"] < 1e-10
+    assert abs(Pm - P) <= 1e-9 * P
+    assert rel(grm, gre) <= 1e-7 and rel(gim_m, gim) <= 1e-7
 
 
 def test_gradient_needs_a_solution():

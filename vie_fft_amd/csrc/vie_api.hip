@@ -4511,14 +4511,22 @@ int vie_absorption(vie_ctx* c, double* value)
     }
 }
 
-int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* value, double* grad_re,
-                        double* grad_im, vie_info* info)
+// The adjoint gradient: method >= VIE_CG solves the adjoint system with that
+// method in the context's precision; mixed runs it as the mixed-precision
+// BiCGSTAB of vie_solve_mixed_resident (FP32 iterations, FP64 residual
+// replacements and stop rule).
+static int absorption_grad(vie_ctx* c, bool mixed, int method, double tol, int maxit, double* value,
+                           double* grad_re, double* grad_im, vie_info* info)
 {
     if (!c || !grad_re) return VIE_ERR_ARG;
     try {
         check_ready(c);
         require(c, c->sol_ready, "no solution on the device");
-        require(c, method >= VIE_CG && method <= VIE_BICORSTAB, "unknown method");
+        if (mixed) {
+            require(c, c->desc.precision == VIE_FP64, "mixed solve needs an FP64 context");
+        } else {
+            require(c, method >= VIE_CG && method <= VIE_BICORSTAB, "unknown method");
+        }
         require(c, tol > 0, "tol must be positive");
         if (maxit <= 0) maxit = (int)std::min<long>(c->V, 1L << 30);
         const long n = 3 * c->V;
@@ -4527,7 +4535,26 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
         auto t0 = std::chrono::steady_clock::now();
         const double P = absorption_dev(c, g);
         KState h;
-        if (c->desc.precision == VIE_FP64) {
+        double terr = -1.0;
+        int nrep = 0;
+        if (mixed) {
+            // the mixed solver works on the context's FP64 right-hand side and
+            // solution: they point at g and nu during the adjoint solve (the
+            // forward solution in x64 is needed again by the gradient kernel)
+            struct Swap {
+                vie_ctx* c;
+                double2* b;
+                double2* x;
+                ~Swap()
+                {
+                    c->b64 = b;
+                    c->x64 = x;
+                }
+            } keep{c, c->b64, c->x64};
+            c->b64 = g;
+            c->x64 = nu;
+            h = solve_bicgstab_reliable(c, tol, maxit, 0.1, &terr, &nrep);
+        } else if (c->desc.precision == VIE_FP64) {
             h = solve_method<double>(c, method, g, nu, tol, maxit);
         } else {
             float2* g32 = vec<float>(c, VB);
@@ -4559,15 +4586,27 @@ int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* v
         if (info) {
             info->iterations = h.iter;
             info->converged = h.converged;
-            info->err = h.err;
-            info->outer = 0;
+            info->err = mixed ? terr : h.err;
+            info->outer = nrep;
             info->seconds = std::chrono::duration<double>(t1 - t0).count();
-            info->true_relres = -1.0;
+            info->true_relres = terr;
         }
         return status_of(c, h);
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int vie_absorption_grad(vie_ctx* c, int method, double tol, int maxit, double* value, double* grad_re,
+                        double* grad_im, vie_info* info)
+{
+    return absorption_grad(c, false, method, tol, maxit, value, grad_re, grad_im, info);
+}
+
+int vie_absorption_grad_mixed(vie_ctx* c, double tol, int maxit, double* value, double* grad_re, double* grad_im,
+                              vie_info* info)
+{
+    return absorption_grad(c, true, VIE_BICGSTAB, tol, maxit, value, grad_re, grad_im, info);
 }
 
 int vie_tuning(const vie_ctx* c, int* out, double* us)

@@ -32,7 +32,7 @@ EXPORTS = [
     "vie_set_medium_eps", "vie_green_compute", "vie_incident", "vie_apply", "vie_apply_async",
     "vie_solve", "vie_set_rhs", "vie_solve_resident", "vie_get_solution", "vie_solve_mixed_resident",
     "vie_forward_scatter", "vie_absorbed", "vie_time_kernel", "vie_kernel_time",
-    "vie_model_bytes_matvec", "vie_plan", "vie_absorption", "vie_absorption_grad",
+    "vie_model_bytes_matvec", "vie_plan", "vie_absorption", "vie_absorption_grad", "vie_absorption_grad_mixed",
     "vie_set_k0", "vie_set_rhs_incident", "vie_far_field", "vie_rcs", "vie_solve_batched",
     "vie_batch_reserve", "vie_batch_capacity", "vie_batch_set_operators", "vie_batch_set_rhs_incident",
     "vie_batch_set_rhs", "vie_batch_solve", "vie_batch_get_solution", "vie_batch_forward_scatter",
@@ -151,6 +151,7 @@ def load():
     L.vie_solve_batched.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_int, dp, dp,
                                     ct.POINTER(Info)]
     L.vie_absorption_grad.argtypes = [vp, ct.c_int, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
+    L.vie_absorption_grad_mixed.argtypes = [vp, ct.c_double, ct.c_int, dp, dp, dp, ct.POINTER(Info)]
     ip = ct.POINTER(ct.c_int)
     L.vie_batch_reserve.argtypes = [vp, ct.c_int]
     L.vie_batch_capacity.argtypes = [vp, ip]
@@ -409,15 +410,21 @@ class Solver:
         self._check(self.lib.vie_absorption(self.h, ct.byref(v)))
         return v.value
 
-    def absorption_grad(self, method=BICGSTAB, tol=1e-10, maxit=0):
-        """(P, dP/dRe eps, dP/dIm eps, info of the adjoint solve) for the resident solution."""
+    def absorption_grad(self, method=BICGSTAB, tol=1e-10, maxit=0, mixed=False):
+        """(P, dP/dRe eps, dP/dIm eps, info of the adjoint solve) for the resident solution;
+        mixed: the adjoint solve by the mixed-precision BiCGSTAB (FP64 contexts)."""
         method = METHODS.get(method, method) if isinstance(method, str) else method
         v = ct.c_double()
         gre = np.zeros(self.n)
         gim = np.zeros(self.n)
         info = Info()
-        self._check(self.lib.vie_absorption_grad(self.h, int(method), float(tol), int(maxit), ct.byref(v),
-                                                 _dp(gre), _dp(gim), ct.byref(info)), allow=(OK, NOT_CONVERGED))
+        if mixed:
+            st = self.lib.vie_absorption_grad_mixed(self.h, float(tol), int(maxit), ct.byref(v), _dp(gre), _dp(gim),
+                                                    ct.byref(info))
+        else:
+            st = self.lib.vie_absorption_grad(self.h, int(method), float(tol), int(maxit), ct.byref(v), _dp(gre),
+                                              _dp(gim), ct.byref(info))
+        self._check(st, allow=(OK, NOT_CONVERGED))
         return v.value, gre, gim, info.as_dict()
 
     # -- batches of instances (one geometry, several wavelengths / polarisations)
