@@ -201,6 +201,9 @@ __device__ __forceinline__ void xi_xchg(T_* v, T_* lds, int t, int col)
 __device__ __forceinline__ int opaque(int v)
 {
     asm volatile("" : "+v"(v));
+    // lane coordinates lie in [0, NT): with that known again, i % S of the
+    // stages is a mask rather than a signed remainder with sign fix-ups
+    __builtin_assume(v >= 0 && v < NT);
     return v;
 }
 
