@@ -136,4 +136,22 @@ def test_gradient_needs_a_solution():
         with pytest.raises(VF.VieError):
             s.absorption_grad()
         with pytest.raises(VF.VieError):
+            s.absorption_grad(mixed=True)
+        with pytest.raises(VF.VieError):
             s.absorption()
+
+
+def test_mixed_gradient_needs_an_fp64_context():
+    sc = G.design_case(n=8)
+    eps, flags = sc.eps(), sc.flags()
+    with VF.Solver(sc.n, sc.d, sc.k0, VF.FP32) as s:
+        s.set_medium_eps(eps, flags)
+        s.green_compute()
+        s.set_rhs(s.incident(*sc.rhs_angles()) * flags)
+        s.solve_resident(VF.BICGSTAB, tol=1e-5)
+        with pytest.raises(VF.VieError) as e:
+            s.absorption_grad(mixed=True)
+        assert e.value.status == VF.ERR_ARG
+        # the plain FP32 gradient still works on it
+        P, gre, gim, info = s.absorption_grad(tol=1e-5)
+        assert P > 0 and info["converged"]
