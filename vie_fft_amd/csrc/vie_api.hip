@@ -2984,43 +2984,56 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
     Op<float>* o32 = make_op<float>(c);
     o64->batch_alloc(c->nbcap);
     o32->batch_alloc(c->nbcap);
-    const int* l = o32->bl;
-    const long tot = o32->gob;
-    // G~ octants of the windowed products' grid (l), scaled like short_octant
-    double2* w1 = dalloc<double2>(cnt * tot);
-    double2* w2 = nullptr;
-    double* kd = nullptr;
+    std::vector<double*> owned;  // device scratch, freed at the end (and on errors)
+    auto scratch = [&](long nd) {
+        owned.push_back(nullptr);
+        owned.back() = dalloc<double>(nd);
+        return owned.back();
+    };
     double* cd_im = nullptr;
-    double* cd_ = nullptr;
     try {
-        w2 = dalloc<double2>(cnt * tot);
-        kd = dalloc<double>(cnt);
+        double* kd = scratch(cnt);
         copy_sync(kd, k0s, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream);
-        const int nmax = std::max(l[0], std::max(l[1], l[2]));
-        std::vector<double> ctab(3 * 2 * nmax);
-        for (int ax = 0; ax < 3; ++ax)
-            for (int j = 0; j < 2 * l[ax]; ++j) ctab[ax * 2 * nmax + j] = (double)cosl((long double)kPi * j / l[ax]);
-        cd_ = dalloc<double>(ctab.size());
-        copy_sync(cd_, ctab.data(), ctab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
-        const double* d = c->desc.d;
-        const dim3 gg(grid1(tot, 256, 1L << 30), cnt);
-        hipLaunchKernelGGL(k_green_offsets, gg, dim3(256), 0, c->stream, l[0], l[1], l[2], d[0], d[1], d[2], 0.0, kd,
-                           w1);
-        HIPCHK(hipGetLastError());
-        double2* src = w1;
-        double2* dst = w2;
-        for (int ax = 0; ax < 3; ++ax) {  // x: w1 -> w2, y: w2 -> w1, z: w1 -> w2
-            hipLaunchKernelGGL(k_green_cos, gg, dim3(256), 0, c->stream, l[0] + 1, l[1] + 1, l[2] + 1, ax,
-                               cd_ + ax * 2 * nmax, src, dst);
+        // G~ octants of an operator's windowed-product grid l (tot entries per
+        // instance), scaled like short_octant, for every instance.  The two
+        // operators' grids differ when the tuner gave them different short
+        // periods; then each gets its own.
+        auto octants = [&](const int* l, long tot) -> const double2* {
+            double2* w1 = reinterpret_cast<double2*>(scratch(2 * cnt * tot));
+            double2* w2 = reinterpret_cast<double2*>(scratch(2 * cnt * tot));
+            const int nmax = std::max(l[0], std::max(l[1], l[2]));
+            std::vector<double> ctab(3 * 2 * nmax);
+            for (int ax = 0; ax < 3; ++ax)
+                for (int j = 0; j < 2 * l[ax]; ++j)
+                    ctab[ax * 2 * nmax + j] = (double)cosl((long double)kPi * j / l[ax]);
+            double* cd = scratch((long)ctab.size());
+            copy_sync(cd, ctab.data(), ctab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+            const double* d = c->desc.d;
+            const dim3 gg(grid1(tot, 256, 1L << 30), cnt);
+            hipLaunchKernelGGL(k_green_offsets, gg, dim3(256), 0, c->stream, l[0], l[1], l[2], d[0], d[1], d[2], 0.0,
+                               kd, w1);
             HIPCHK(hipGetLastError());
-            std::swap(src, dst);
-        }
+            double2* src = w1;
+            double2* dst = w2;
+            for (int ax = 0; ax < 3; ++ax) {  // x: w1 -> w2, y: w2 -> w1, z: w1 -> w2
+                hipLaunchKernelGGL(k_green_cos, gg, dim3(256), 0, c->stream, l[0] + 1, l[1] + 1, l[2] + 1, ax,
+                                   cd + ax * 2 * nmax, src, dst);
+                HIPCHK(hipGetLastError());
+                std::swap(src, dst);
+            }
+            const int* n = c->desc.n;
+            hipLaunchKernelGGL(k_scale_c64, dim3(grid1(cnt * tot, 256, 1L << 30)), dim3(256), 0, c->stream,
+                               cnt * tot, (double)n[0] * n[1] * n[2] / ((double)l[0] * l[1] * l[2]), src);
+            HIPCHK(hipGetLastError());
+            return src;
+        };
+        const bool same = o64->bl[0] == o32->bl[0] && o64->bl[1] == o32->bl[1] && o64->bl[2] == o32->bl[2];
+        const double2* g32 = octants(o32->bl, o32->gob);
+        const double2* g64 = same ? g32 : octants(o64->bl, o64->gob);
+        batch_tables<double>(c, o64, i0, cnt, k0s, g64);
+        batch_tables<float>(c, o32, i0, cnt, k0s, g32);
+        const double* d = c->desc.d;
         const int* n = c->desc.n;
-        hipLaunchKernelGGL(k_scale_c64, dim3(grid1(cnt * tot, 256, 1L << 30)), dim3(256), 0, c->stream, cnt * tot,
-                           (double)n[0] * n[1] * n[2] / ((double)l[0] * l[1] * l[2]), src);
-        HIPCHK(hipGetLastError());
-        batch_tables<double>(c, o64, i0, cnt, k0s, src);
-        batch_tables<float>(c, o32, i0, cnt, k0s, src);
         // per-instance permittivity from the tables (all uploaded at once),
         // one instance at a time through one V-sized buffer, and its d_f
         if (eps_tables) {
@@ -3064,18 +3077,12 @@ void batch_operators(vie_ctx* c, int i0, int cnt, const double* k0s, const doubl
             std::copy(eps_tables, eps_tables + 2L * cnt * n_mat, c->btab.begin() + 2L * i0 * n_mat);
         }
     } catch (...) {
-        dfree(w1);
-        dfree(w2);
-        dfree(kd);
+        for (double*& q : owned) dfree(q);
         dfree(cd_im);
-        dfree(cd_);
         throw;
     }
-    dfree(w1);
-    dfree(w2);
-    dfree(kd);
+    for (double*& q : owned) dfree(q);
     dfree(cd_im);
-    dfree(cd_);
     for (int i = 0; i < cnt; ++i) {
         c->bk0[i0 + i] = k0s[i];
         c->bop[i0 + i] = eps_tables ? 2 : 1;
