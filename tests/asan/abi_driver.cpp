@@ -180,6 +180,17 @@ static void single_context(int n, vie_precision prec)
         std::vector<vie_info> in2(2);
         SOLVED(vie_solve_batched(c, 2, VIE_BATCH_MIXED, VIE_BICGSTAB, 1e-8, 0, B.data(), X.data(), in2.data()));
         EXPECT(finite(X));
+    } else {
+        // an FP32 context's batch: per-instance operators, plain solves
+        OK(vie_batch_reserve(c, 3));
+        const double k0s[3] = {2 * kPi / 4.0, 2 * kPi / 5.0, 2 * kPi / 6.0};
+        const double tabs[12] = {1.1, 0.0, 4.0, -0.5, 1.1, 0.0, 3.0, -0.2, 1.1, 0.0, 2.5, -0.1};
+        OK(vie_batch_set_operators(c, 0, 3, k0s, tabs, 2));
+        for (int i = 0; i < 3; ++i) OK(vie_batch_set_rhs_incident(c, i, 10.0 * i, 0.0, 0.0));
+        std::vector<vie_info> infos(3);
+        SOLVED(vie_batch_solve(c, 3, VIE_BATCH_PLAIN, VIE_BICGSTAB, 1e-5, 0, nullptr, nullptr, infos.data()));
+        OK(vie_batch_get_solution(c, 1, x.data()));
+        EXPECT(finite(x));
     }
     EXPECT(vie_last_error(c) != nullptr);
     g_ctx = nullptr;
