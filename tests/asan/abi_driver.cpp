@@ -13,14 +13,15 @@
 
 static int g_fail = 0;
 static vie_ctx* g_ctx = nullptr;  // the context whose last error a failure prints
+// A failed check ends the process at once (no static destructors: the HIP
+// runtime's own teardown with contexts still alive is not what is checked)
 #define EXPECT(cond)                                                             \
     do {                                                                         \
         if (!(cond)) {                                                           \
-            std::printf("FAILED line %d: %s (%s)\n", __LINE__, #cond,            \
-                        g_ctx ? vie_last_error(g_ctx) : "");                     \
+            std::printf("FAILED line %d: %s (%s)\nabi_driver FAILED\n", __LINE__, \
+                        #cond, g_ctx ? vie_last_error(g_ctx) : "");              \
             std::fflush(stdout);                                                 \
-            g_fail = 1;                                                          \
-            return;                                                              \
+            std::_Exit(1);                                                       \
         }                                                                        \
     } while (0)
 #define OK(call) EXPECT((call) == VIE_OK)
@@ -235,6 +236,88 @@ static void embedded_context()
     OK(vie_destroy(c));
 }
 
+// call sequences across features on a non-cubic, embedded grid: batches
+// re-reserved at another size, a new geometry after a batch, a wavelength
+// step after a batch, the preconditioner and the mixed adjoint
+static void sequences()
+{
+    vie_desc d{};
+    d.n[0] = 20;
+    d.n[1] = 13;
+    d.n[2] = 17;
+    d.d[0] = 0.04;
+    d.d[1] = 0.05;
+    d.d[2] = 0.03;
+    d.k0 = 2 * kPi / 5.0;
+    d.precision = VIE_FP64;
+    vie_ctx* c = nullptr;
+    OK(vie_create(&c, &d));
+    g_ctx = c;
+    const long V = 20L * 13 * 17;
+    vie_shape sh[2] = {};
+    sh[0].kind = VIE_SPHERE;
+    sh[0].mat = 1;
+    sh[0].c[0] = 8;
+    sh[0].c[1] = 6.5;
+    sh[0].c[2] = 8;
+    sh[0].r = 4;
+    sh[1].kind = VIE_ZBAND;
+    sh[1].mat = 2;
+    sh[1].z0 = 11;
+    sh[1].z1 = 15;
+    const double eps[6] = {1.1, 0.0, 4.0, -0.5, 2.2, -0.1};
+    OK(vie_set_medium_shapes(c, 2, sh, eps, 3));
+    OK(vie_green_compute(c));
+    vie_info info{};
+    std::vector<double> x(6 * V), gre(V);
+    for (int nb : {3, 5, 2}) {  // batches of different sizes one after the other
+        OK(vie_batch_reserve(c, nb));
+        std::vector<double> k0s(nb), tabs(6 * nb);
+        for (int i = 0; i < nb; ++i) {
+            k0s[i] = 2 * kPi / (4.0 + 0.5 * i);
+            for (int m = 0; m < 6; ++m) tabs[6 * i + m] = eps[m] * (m >= 2 ? 1.0 + 0.05 * i : 1.0);
+        }
+        OK(vie_batch_set_operators(c, 0, nb, k0s.data(), tabs.data(), 3));
+        for (int i = 0; i < nb; ++i) OK(vie_batch_set_rhs_incident(c, i, 15.0 * i, 30.0, 0.0));
+        std::vector<vie_info> infos(nb);
+        SOLVED(vie_batch_solve(c, nb, VIE_BATCH_MIXED, VIE_BICGSTAB, 1e-7, 0, nullptr, nullptr, infos.data()));
+        OK(vie_batch_get_solution(c, nb - 1, x.data()));
+        EXPECT(finite(x));
+    }
+    // a wavelength step after the batches, then single solves
+    OK(vie_set_k0(c, 2 * kPi / 6.0));
+    OK(vie_green_compute(c));
+    OK(vie_set_rhs_incident(c, 0.0, 0.0, 0.0));
+    SOLVED(vie_solve_resident(c, VIE_BICGSTAB, 1e-9, 0, &info));
+    OK(vie_set_preconditioner(c, VIE_PRECOND_JACOBI));
+    SOLVED(vie_solve_mixed_resident(c, VIE_MIXED_RELIABLE, VIE_BICGSTAB, 1e-9, 0.0, 0, &info));
+    OK(vie_set_preconditioner(c, VIE_PRECOND_NONE));
+    SOLVED(vie_absorption_grad_mixed(c, 1e-9, 0, nullptr, gre.data(), nullptr, &info));
+    EXPECT(finite(gre));
+    // a new geometry (other windows) after a batch, then a batch on it
+    sh[0].c[0] = 12;
+    sh[0].r = 5;
+    OK(vie_set_medium_shapes(c, 2, sh, eps, 3));
+    OK(vie_green_compute(c));
+    OK(vie_batch_reserve(c, 2));
+    const double k0b[2] = {2 * kPi / 4.0, 2 * kPi / 7.0};
+    const double tb[12] = {1.1, 0.0, 3.0, -0.3, 2.0, -0.1, 1.1, 0.0, 5.0, -0.6, 2.5, -0.2};
+    OK(vie_batch_set_operators(c, 0, 2, k0b, tb, 3));
+    for (int i = 0; i < 2; ++i) OK(vie_batch_set_rhs_incident(c, i, 0.0, 0.0, 90.0 * i));
+    std::vector<vie_info> infos(2);
+    {  // CORS may break down or stagnate on this medium: any solver status, no error
+        const int st = vie_batch_solve(c, 2, VIE_BATCH_PLAIN, VIE_CORS, 1e-6, 500, nullptr, nullptr, infos.data());
+        EXPECT(st == VIE_OK || st == VIE_NOT_CONVERGED || st == VIE_BREAKDOWN || st == VIE_DIVERGED);
+    }
+    SOLVED(vie_batch_solve(c, 2, VIE_BATCH_MIXED, VIE_BICGSTAB, 1e-8, 0, nullptr, nullptr, infos.data()));
+    double fw[2], ab[2];
+    const double z2[2] = {0, 0}, ps[2] = {0, 90};
+    OK(vie_batch_outputs(c, 2, z2, z2, ps, 2, fw, ab));
+    EXPECT(std::isfinite(fw[0]) && std::isfinite(ab[1]));
+    g_ctx = nullptr;
+    OK(vie_destroy(c));
+}
+
 static void bad_arguments()
 {
     vie_desc d{};
@@ -266,6 +349,7 @@ int main()
     if (!g_fail) single_context(16, VIE_FP64);
     if (!g_fail) single_context(16, VIE_FP32);
     if (!g_fail) embedded_context();
+    if (!g_fail) sequences();
     size_t rel = 0;
     if (!g_fail && vie_release_buffers(&rel) != VIE_OK) g_fail = 1;
     std::printf(g_fail ? "abi_driver FAILED\n" : "abi_driver ok\n");

@@ -561,21 +561,26 @@ template <class R> struct Op {
     void batch_alloc(int cap)
     {
         const int l[3] = {nxs ? nxs : n[0], nys ? nys : n[1], nzs ? nzs : n[2]};
-        if (cap <= nbt && l[0] == bl[0] && l[1] == bl[1] && l[2] == bl[2]) return;
-        batch_free();
-        for (int a = 0; a < 3; ++a) bl[a] = l[a];
-        lb = std::max(l[0], std::max(l[1], l[2]));
-        gob = (long)(l[0] + 1) * (l[1] + 1) * (l[2] + 1);
-        Gb = dalloc<T>(cap * gob);
-        for (int a = 0; a < 3; ++a) {
-            Vb[a] = dalloc<T>(2L * lb * cap);
-            Uab[a] = dalloc<T>(2L * lb * cap);
+        if (!(cap <= nbt && l[0] == bl[0] && l[1] == bl[1] && l[2] == bl[2])) {
+            batch_free();
+            for (int a = 0; a < 3; ++a) bl[a] = l[a];
+            lb = std::max(l[0], std::max(l[1], l[2]));
+            gob = (long)(l[0] + 1) * (l[1] + 1) * (l[2] + 1);
+            Gb = dalloc<T>(cap * gob);
+            for (int a = 0; a < 3; ++a) {
+                Vb[a] = dalloc<T>(2L * lb * cap);
+                Uab[a] = dalloc<T>(2L * lb * cap);
+            }
+            Db = dalloc<T>(3L * cap);
+            Dab = dalloc<T>(3L * cap);
+            dgb = dalloc<T>(3 * g.V * cap);
+            nbt = cap;
         }
-        Db = dalloc<T>(3L * cap);
-        Dab = dalloc<T>(3L * cap);
-        dgb = dalloc<T>(3 * g.V * cap);
-        nbt = cap;
-        // the windowed products' slots of the shared (k0-free) tables
+        // The windowed products' slots of the shared (k0-free) tables, taken
+        // again on every call although the per-instance buffers above are
+        // kept: set_short rebuilds the short-period tables (every re-tune
+        // does, also when it ends on the lengths it started from), so
+        // addresses taken at an earlier call may be freed ones.
         const int sl[3] = {nxs ? 3 : 0, nys ? 4 : 1, nzs ? 5 : 2};
         for (int a = 0; a < 3; ++a) {
             mfb.U[a] = U[sl[a]];
